@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark: FP32 3D Yee + CPML time step on one MI355X.
+
+Workload: the strip-waveguide configuration (384 x 192 x 160 cells, 12-cell
+CPML on all faces, per-voxel eps, Ez planar Gaussian-pulse source, two flux
+planes at 50 frequencies with float64 DFT accumulators), FP32 fields, FAST
+mode.  A step is one full time step (H half step, E half step, sources, due
+DFT updates).  Fields, materials and monitors are resident in HBM before the
+timed region starts.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  Mcell-updates/s = Nx*Ny*Nz*K / wall_s / 1e6
+(src/Simulation.jl:517-518).  The roofline entry uses the E-update kernel
+(the dominant one): algorithmic bytes per launch / mean launch time from HIP
+events recorded around each launch on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "khronos.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(spec, cores: int, budget_s: float = 15.0):
+    """Time the CPU oracle (C restatement of the reference CPU path, OpenMP)
+    on the same workload for a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_engine import OracleEngine  # the checker, used here only as the CPU baseline
+    eng = OracleEngine(spec)
+    eng.step(1)  # warm-up: first touch of every array
+    t0 = time.perf_counter()
+    eng.step(1)
+    t1 = time.perf_counter() - t0
+    n = max(2, min(40, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    eng.step(n)
+    dt = time.perf_counter() - t0
+    cells = spec.N[0] * spec.N[1] * spec.N[2]
+    del eng
+    return {
+        "value": round(cells * n / dt / 1e6, 3),
+        "unit": "Mcell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"strip-waveguide 384x192x160 FP32, {n} steps after 2 warm-up steps "
+                  f"(C oracle, -O3 -fopenmp, {cores} threads)",
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        sys.stderr.write("bench.py measures one GPU: the time step runs on a single MI355X "
+                         "(multi-GPU runs are not part of this build); use --gpus 1\n")
+        return 2
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+
+    from khronos_hip.workloads import strip_waveguide, algorithmic_bytes
+
+    sim = strip_waveguide(precision="f32", mode=args.mode)
+    sim.prepare()
+    eng = sim.engine
+    cells = sim.Nx * sim.Ny * sim.Nz
+    ab = algorithmic_bytes(sim.spec)
+
+    sim.step(args.warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+
+    launches = args.steps * (2 + len(sim.spec.dfts))
+    eng.set_timing(launches + 64)
+    eng.get_timing()  # clear
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    eng.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = eng.get_timing()
+    eng.set_timing(0)
+
+    value = cells * args.steps / wall / 1e6
+    e_ms = tm["e_ms"] / max(tm["e_n"], 1)
+    h_ms = tm["h_ms"] / max(tm["h_n"], 1)
+    achieved = ab["E"] / (e_ms * 1e-3) / 1e9
+    b_alg = (ab["H"] + ab["E"]) / cells
+
+    line = {
+        "metric": "Mcell-updates/s (FP32 3D Yee + CPML time step)",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "strip waveguide 384x192x160, 12-cell CPML, per-voxel eps, Ez Gaussian-pulse "
+                        "planar source, 2 flux planes x 50 freqs (f64 DFT accumulators), FP32 fields",
+            "cells": cells,
+            "mode": args.mode,
+            "B_alg_bytes_per_cell_update": round(b_alg, 2),
+            "h_kernel_ms": round(h_ms, 5),
+            "e_kernel_ms": round(e_ms, 5),
+            "dft_ms_per_step": round(tm["dft_ms"] / args.steps, 5),
+            "hbm_fraction_whole_step": round(cells * args.steps / wall * b_alg / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "step_e_kernel<float,false>",
+            "alg_bytes_per_launch": ab["E"],
+        },
+    }
+    if not args.no_cpu_baseline:
+        spec = sim.spec
+        del sim, eng
+        line["cpu_baseline"] = cpu_baseline(spec, cores)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

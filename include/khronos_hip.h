@@ -1,0 +1,181 @@
+/*
+ * khronos_hip.h -- C ABI of libkhronos_hip, the MI355X (gfx950) FDTD time step.
+ *
+ * The library replaces the reference's per-step device work: the Yee E/H curl
+ * updates with the fused PML auxiliary cascade, current-source injection and
+ * DFT-monitor accumulation, plus the flux / DFT-norm reductions read at the end
+ * of a run.  Host setup (grid volumes, PML profiles, permittivity rasters,
+ * source amplitudes, waveforms) stays with the caller, exactly as it does in
+ * the reference, and is handed over as plain arrays in the reference's own
+ * layouts (column-major, x fastest).
+ *
+ * Conventions
+ *   - every entry point returns 0 on success and a negative KH_E* code on
+ *     failure; kh_last_error() then holds a message (thread-local).  Nothing
+ *     aborts and no C++ exception crosses the ABI.
+ *   - "backend precision" is the simulation's real type (KH_F32 -> float,
+ *     KH_F64 -> double).  Complex arrays interleave (re, im) in that type.
+ *   - grid indices are the reference's 1-based GridVolume indices
+ *     (src/utils.jl:103-214); a volume starting at start_idx with extent dims
+ *     covers start_idx .. start_idx + dims - 1 on each axis.
+ *   - a context is used by one host thread at a time; distinct contexts are
+ *     independent.  The library copies every input array; host buffers are
+ *     never retained.
+ */
+#ifndef KHRONOS_HIP_H
+#define KHRONOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KH_ABI_VERSION 1
+
+enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
+
+/* Field component codes (src/DataStructures.jl:24-35). */
+enum kh_component { KH_EX = 0, KH_EY = 1, KH_EZ = 2, KH_HX = 3, KH_HY = 4, KH_HZ = 5 };
+
+/* KH_MODE_FAST: cells with no PML, source or polarisation term advance E and
+ *   H directly (E += eps^-1 * dt * curl H); other cells run the full D/B
+ *   cascade.  Results agree with the reference within rounding.
+ * KH_MODE_EXACT: every cell keeps D/B and follows the reference CPU path's
+ *   operation order (src/Kernels/ReferenceKernels.jl:276-313, :468-512). */
+enum kh_mode { KH_MODE_FAST = 0, KH_MODE_EXACT = 1 };
+
+enum kh_error {
+    KH_OK = 0,
+    KH_E_INVALID = -1,   /* bad argument */
+    KH_E_NOMEM = -2,     /* device allocation failed */
+    KH_E_DEVICE = -3,    /* HIP runtime error */
+    KH_E_UNSUPPORTED = -4,
+    KH_E_STATE = -5      /* call out of order (e.g. source added after stepping) */
+};
+
+typedef struct kh_ctx kh_ctx;
+
+/* Everything kh_create needs.  Pointers may be NULL where noted. */
+typedef struct kh_sim_desc {
+    int32_t nx, ny, nz;        /* cells per axis (the reference's Nx, Ny, Nz) */
+    int32_t precision;         /* enum kh_precision */
+    int32_t mode;              /* enum kh_mode */
+    int32_t dft_accum_f64;     /* 1: DFT accumulators in float64 (default form);
+                                  0: in the backend precision, as the reference */
+    double dx, dy, dz, dt;     /* backend-precision values (exact in double) */
+    /* PML profiles, length 2*n+1 each, backend precision
+     * (src/Boundaries.jl:64-72, :112-163).  NULL = no PML arrays at all. */
+    const void *sigma_b[3];
+    const void *sigma_d[3];
+    /* Per-voxel inverse permittivity / permeability of Ex,Ey,Ez / Hx,Hy,Hz in
+     * the reference geometry layout: component voxel count, no ghosts
+     * (src/Geometry.jl:459-473).  NULL = use the scalar below. */
+    const void *eps_inv[3];
+    const void *mu_inv[3];
+    double eps_inv_scalar;     /* src/Geometry.jl:662 */
+    double mu_inv_scalar;
+    double source_cutoff_time; /* last_source_time (src/Sources/TimeSources.jl:46);
+                                  +inf for sources without cutoff */
+} kh_sim_desc;
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *kh_last_error(void);
+
+/* KH_ABI_VERSION. */
+int kh_version(void);
+
+/* Device bytes kh_create would allocate for `desc` (fields, auxiliaries,
+ * material arrays); sources and monitors add their own arrays.
+ * Replaces: estimate_memory (src/Memory.jl:120-251). */
+int kh_mem_footprint(const kh_sim_desc *desc, size_t *bytes);
+
+/* Allocate a simulation on the current device and upload its materials.
+ * Replaces: init_fields / create_all_chunks / init_boundaries uploads
+ * (src/Simulation.jl:92-283, src/Fields.jl:131-142). */
+int kh_create(const kh_sim_desc *desc, kh_ctx **out);
+
+/* Free every device array of the context. */
+int kh_destroy(kh_ctx *ctx);
+
+/* Register a current source: S[start + i] += Re(f(t) * amplitude[i]) on the
+ * component's half step while sources are active.  `amplitude` is complex,
+ * backend precision, dims[0]*dims[1]*dims[2] values, column-major.
+ * Replaces: assemble_sources upload + update_source! (src/Sources/Sources.jl:44-131,
+ * :346-357). */
+int kh_add_source(kh_ctx *ctx, int32_t component, const int32_t start_idx[3],
+                  const int32_t dims[3], const void *amplitude, int32_t *source_id);
+
+/* Waveform of a source: values[2*n], values[2*n+1] = Re, Im of f(t_n) for
+ * timestep n = 0..n_steps-1, where t_n is the time the reference evaluates
+ * the source at on that step (t for H sources, t + dt/2 for E sources;
+ * src/Kernels/Kernels.jl:64-74).  Steps past the table inject zero.
+ * Replaces: eval_time_source per step (src/Sources/TimeSources.jl:123-132). */
+int kh_set_waveform(kh_ctx *ctx, int32_t source_id, int64_t n_steps, const double *values);
+
+/* Register a DFT monitor on one component: every `decimation`-th step,
+ * F[i,k] += dt * exp(i*2*pi*freqs[k]*t) * field[start + i]
+ * (H at t, E at t + dt/2).  Replaces: init_monitors(DFTMonitor) and
+ * update_dft_monitor! (src/Monitors/Monitors.jl:202-290, :361-379). */
+int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3],
+               const int32_t dims[3], int32_t n_freq, const double *freqs,
+               int32_t decimation, int32_t *dft_id);
+
+/* Advance n_steps time steps in the reference's sub-step order
+ * (H sources, H update, H monitors, E sources, E update, E monitors).
+ * Asynchronous with respect to the host.  Replaces: step! (src/Kernels/Kernels.jl:20-88). */
+int kh_step(kh_ctx *ctx, int64_t n_steps);
+
+/* Wait for all queued work of the context. */
+int kh_sync(kh_ctx *ctx);
+
+/* Current timestep counter (src/DataStructures.jl:750). */
+int kh_get_timestep(kh_ctx *ctx, int64_t *timestep);
+
+/* Whether sources are still injected (src/DataStructures.jl:751). */
+int kh_sources_active(kh_ctx *ctx, int32_t *active);
+
+/* Copy one field component to/from `host`, in the reference's array layout:
+ * (component voxel count + 2) per axis, ghosts included, column-major,
+ * backend precision.  `bytes` must equal that size.  Synchronous. */
+int kh_read_field(kh_ctx *ctx, int32_t component, void *host, size_t bytes);
+int kh_write_field(kh_ctx *ctx, int32_t component, const void *host, size_t bytes);
+
+/* Copy a DFT accumulator to `host` as complex128, shape (dims..., n_freq),
+ * column-major.  Synchronous.  Replaces: Array(monitor.fields). */
+int kh_read_dft(kh_ctx *ctx, int32_t dft_id, double *host, size_t bytes);
+
+/* Poynting flux of a planar monitor from its four tangential DFTs
+ * (e1, e2, h1, h2), normal_axis 1..3, area element dA; out[n_freq] in
+ * float64.  Reduction on the device.  Replaces: get_flux
+ * (src/Monitors/FluxMonitor.jl:92-156). */
+int kh_flux(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2,
+            int32_t normal_axis, double dA, double *out);
+
+/* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
+ * dft_id = -1 gives the sum of that over all monitors.
+ * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */
+int kh_dft_norm(kh_ctx *ctx, int32_t dft_id, double *out);
+
+/* Zero fields, auxiliaries, DFT accumulators and the timestep; re-arm sources.
+ * Replaces: reset_fields! (src/Simulation.jl:571-637). */
+int kh_reset(kh_ctx *ctx);
+
+/* Kernel timing for measurement: while enabled (enable > 0), every update
+ * and DFT launch is bracketed by HIP events on the context's stream; `enable`
+ * event pairs are created up front so that timing allocates nothing while it
+ * runs.  kh_get_timing returns {total ms of H updates, count, total ms of E
+ * updates, count, total ms of DFT launches, count} and clears the counters. */
+int kh_set_timing(kh_ctx *ctx, int32_t enable);
+int kh_get_timing(kh_ctx *ctx, double out[6]);
+
+/* The hipStream_t the context launches on (for callers that order their own
+ * work after it). */
+int kh_get_stream(kh_ctx *ctx, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KHRONOS_HIP_H */

@@ -1,0 +1,764 @@
+// khronos_hip.hip -- host side of libkhronos_hip (the C ABI in include/khronos_hip.h).
+//
+// One context = one simulation resident in HBM on the current device, one
+// non-blocking HIP stream.  kh_step issues, per time step, the reference's
+// sub-step order (src/Kernels/Kernels.jl:64-87): H sources are folded into the
+// H update kernel, then due H-component DFT launches, E update (+E sources),
+// then due E-component DFT launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/khronos_hip.h"
+#include "kh_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct KhError : std::runtime_error {
+    int code;
+    KhError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            throw KhError(_e == hipErrorOutOfMemory ? KH_E_NOMEM : KH_E_DEVICE,           \
+                          std::string(#expr) + ": " + hipGetErrorString(_e));             \
+        }                                                                                 \
+    } while (0)
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return KH_OK;
+    } catch (const KhError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return KH_E_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return KH_E_INVALID;
+    }
+}
+
+void require(bool c, const std::string &m, int code = KH_E_INVALID) {
+    if (!c) throw KhError(code, m);
+}
+
+constexpr int ALIGN = 32;  // elements; x index 1 of every row is aligned
+
+struct Grid {
+    int n[3];
+    int64_t px, py, pz;  // padded extents (pitch in x)
+    int64_t front;       // elements before grid index (0,0,0)
+    int64_t total;       // elements per allocation
+    void init(int nx, int ny, int nz) {
+        n[0] = nx; n[1] = ny; n[2] = nz;
+        px = ((int64_t)nx + 2 + ALIGN - 1) / ALIGN * ALIGN;
+        py = ny + 2;
+        pz = nz + 2;
+        front = ALIGN - 1;
+        total = front + px * py * pz + ALIGN;
+    }
+    int64_t sy() const { return px; }
+    int64_t sz() const { return px * py; }
+    int64_t lin(int gx, int gy, int gz) const { return gx + px * ((int64_t)gy + py * (int64_t)gz); }
+};
+
+void comp_count(const int n[3], int c, int out[3]) {
+    // src/utils.jl:26-37
+    int nn[3] = {n[0], n[1], n[2] < 1 ? 1 : n[2]};
+    int a = c % 3;
+    if (c < 3) {
+        for (int i = 0; i < 3; ++i) out[i] = nn[i] + 1;
+        out[a] = nn[a];
+    } else {
+        for (int i = 0; i < 3; ++i) out[i] = nn[i];
+        out[a] = nn[a] + 1;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct CtxBase {
+    virtual ~CtxBase() = default;
+    virtual void add_source(int comp, const int32_t *s, const int32_t *d, const void *amp, int32_t *id) = 0;
+    virtual void set_waveform(int id, int64_t n, const double *v) = 0;
+    virtual void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) = 0;
+    virtual void step(int64_t n) = 0;
+    virtual void sync() = 0;
+    virtual int64_t timestep() const = 0;
+    virtual bool sources_active() const = 0;
+    virtual void read_field(int comp, void *host, size_t bytes) = 0;
+    virtual void write_field(int comp, const void *host, size_t bytes) = 0;
+    virtual void read_dft(int id, double *host, size_t bytes) = 0;
+    virtual void flux(int e1, int e2, int h1, int h2, int ax, double dA, double *out) = 0;
+    virtual void dft_norm(int id, double *out) = 0;
+    virtual void reset() = 0;
+    virtual void set_timing(int n) = 0;
+    virtual void get_timing(double *out) = 0;
+    virtual void *stream() = 0;
+};
+
+template <typename T>
+struct Ctx final : CtxBase {
+    Grid g;
+    int mode;
+    bool acc64;
+    T dt, dx[3];
+    double cutoff;
+    hipStream_t st = nullptr;
+    std::vector<DevBuf> owned;
+    // device arrays (pre-offset by g.front)
+    T *E[3]{}, *H[3]{}, *D[3]{}, *B[3]{}, *UD[3]{}, *UB[3]{}, *WD[3]{}, *WB[3]{};
+    T *epsi[3]{}, *mui[3]{};
+    T *sigB[3]{}, *sigD[3]{};
+    T eps_s, mu_s;
+    int64_t ts = 0;
+    bool active = true;
+    bool stepped = false;
+
+    struct Source {
+        int comp;
+        int s[3], d[3];
+        T *amp;
+        std::vector<double> wave;
+    };
+    std::vector<Source> srcs;
+
+    struct Dft {
+        int comp;
+        int s[3], d[3];
+        int nf, dec;
+        double *freqs;
+        void *acc;
+        int64_t ncell;
+    };
+    std::vector<Dft> dfts;
+
+    double *scratch = nullptr;  // reduction partials
+    static constexpr int NPART = 256;
+
+    bool timing = false;
+    struct EvPair { hipEvent_t a, b; int kind; };
+    std::vector<EvPair> evs;   // recorded, not yet read
+    std::vector<EvPair> pool;  // created up front so timing adds no allocation
+
+    EvPair take_event(int kind) {
+        if (pool.empty()) {
+            EvPair e{};
+            HIPCHK(hipEventCreate(&e.a));
+            HIPCHK(hipEventCreate(&e.b));
+            pool.push_back(e);
+        }
+        EvPair e = pool.back();
+        pool.pop_back();
+        e.kind = kind;
+        return e;
+    }
+    double tsum[3] = {0, 0, 0};
+    double tcnt[3] = {0, 0, 0};
+
+    void *dalloc(size_t bytes) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        owned.push_back({p, bytes});
+        HIPCHK(hipMemsetAsync(p, 0, bytes, st));
+        return p;
+    }
+    T *field_alloc() {
+        T *p = (T *)dalloc((size_t)g.total * sizeof(T));
+        return p + g.front;
+    }
+
+    Ctx(const kh_sim_desc &d) {
+        g.init(d.nx, d.ny, d.nz);
+        mode = d.mode;
+        acc64 = d.dft_accum_f64 != 0;
+        dt = (T)d.dt;
+        dx[0] = (T)d.dx; dx[1] = (T)d.dy; dx[2] = (T)d.dz;
+        cutoff = d.source_cutoff_time;
+        eps_s = (T)d.eps_inv_scalar;
+        mu_s = (T)d.mu_inv_scalar;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (int a = 0; a < 3; ++a) {
+            E[a] = field_alloc(); H[a] = field_alloc();
+            D[a] = field_alloc(); B[a] = field_alloc();
+            UD[a] = field_alloc(); UB[a] = field_alloc();
+            WD[a] = field_alloc(); WB[a] = field_alloc();
+        }
+        // sampled sigma: s[g] = profile[2g - 1] (1-based) for g = 1..N, 0 at the boundaries
+        for (int a = 0; a < 3; ++a) {
+            int N = g.n[a];
+            for (int which = 0; which < 2; ++which) {
+                const void *src = which == 0 ? d.sigma_b[a] : d.sigma_d[a];
+                std::vector<T> h((size_t)N + 2, T(0));
+                if (src) {
+                    const T *ps = (const T *)src;
+                    for (int i = 1; i <= N; ++i) h[i] = ps[2 * i - 2];
+                }
+                T *dv = (T *)dalloc(h.size() * sizeof(T));
+                HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+                HIPCHK(hipStreamSynchronize(st));
+                (which == 0 ? sigB : sigD)[a] = dv;
+            }
+        }
+        // per-voxel materials, reference geometry layout -> padded grid
+        for (int c = 0; c < 6; ++c) {
+            const void *src = c < 3 ? d.eps_inv[c] : d.mu_inv[c - 3];
+            if (!src) continue;
+            int cnt[3];
+            comp_count(g.n, c, cnt);
+            std::vector<T> h((size_t)g.total - g.front, T(1));
+            const T *ps = (const T *)src;
+            for (int k = 1; k <= g.n[2]; ++k)
+                for (int j = 1; j <= g.n[1]; ++j)
+                    for (int i = 1; i <= g.n[0]; ++i)
+                        h[g.lin(i, j, k)] = ps[(size_t)(i - 1) + (size_t)cnt[0] * ((size_t)(j - 1) + (size_t)cnt[1] * (k - 1))];
+            T *dv = field_alloc();
+            HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            (c < 3 ? epsi : mui)[c % 3] = dv;
+        }
+        scratch = (double *)dalloc(NPART * 64 * sizeof(double));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    ~Ctx() override {
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto &e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        for (auto &e : pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        for (auto &b : owned) (void)hipFree(b.p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void add_source(int comp, const int32_t *s, const int32_t *d, const void *amp, int32_t *id) override {
+        require(comp >= 0 && comp < 6, "component out of range");
+        require(!stepped, "sources must be added before the first step", KH_E_STATE);
+        int cnt[3];
+        comp_count(g.n, comp, cnt);
+        for (int a = 0; a < 3; ++a) {
+            require(d[a] >= 1, "source extent must be positive");
+            require(s[a] >= 1 && s[a] + d[a] - 1 <= cnt[a] + 1,
+                    "source grid volume outside the component grid");
+        }
+        int same = 0;
+        for (auto &o : srcs) same += (o.comp / 3) == (comp / 3);
+        require(same < kh::MAX_SRC, "too many sources on one half step (max 8)", KH_E_UNSUPPORTED);
+        Source o;
+        o.comp = comp;
+        for (int a = 0; a < 3; ++a) { o.s[a] = s[a]; o.d[a] = d[a]; }
+        size_t n = (size_t)d[0] * d[1] * d[2];
+        o.amp = (T *)dalloc(2 * n * sizeof(T));
+        HIPCHK(hipMemcpyAsync(o.amp, amp, 2 * n * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        srcs.push_back(std::move(o));
+        *id = (int32_t)srcs.size() - 1;
+    }
+
+    void set_waveform(int id, int64_t n, const double *v) override {
+        require(id >= 0 && id < (int)srcs.size(), "bad source id");
+        require(n >= 0 && (n == 0 || v), "bad waveform");
+        srcs[id].wave.assign(v, v + 2 * n);
+    }
+
+    void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) override {
+        require(comp >= 0 && comp < 6, "component out of range");
+        require(nf >= 1 && nf <= 4096, "n_freq must be in 1..4096");
+        require(dec >= 1, "decimation must be >= 1");
+        int cnt[3];
+        comp_count(g.n, comp, cnt);
+        for (int a = 0; a < 3; ++a) {
+            require(d[a] >= 1, "monitor extent must be positive");
+            require(s[a] >= 0 && s[a] + d[a] - 1 <= g.n[a] + 1, "monitor grid volume outside the grid");
+        }
+        Dft m;
+        m.comp = comp;
+        for (int a = 0; a < 3; ++a) { m.s[a] = s[a]; m.d[a] = d[a]; }
+        m.nf = nf;
+        m.dec = dec;
+        m.ncell = (int64_t)d[0] * d[1] * d[2];
+        m.freqs = (double *)dalloc(nf * sizeof(double));
+        HIPCHK(hipMemcpyAsync(m.freqs, f, nf * sizeof(double), hipMemcpyHostToDevice, st));
+        m.acc = dalloc((size_t)m.ncell * nf * 2 * (acc64 ? sizeof(double) : sizeof(T)));
+        HIPCHK(hipStreamSynchronize(st));
+        dfts.push_back(m);
+        *id = (int32_t)dfts.size() - 1;
+    }
+
+    double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22
+
+    kh::HalfArgs<T> base_args(bool electric) const {
+        kh::HalfArgs<T> p{};
+        for (int a = 0; a < 3; ++a) {
+            p.A[a] = electric ? E[a] : H[a];
+            p.F[a] = electric ? H[a] : E[a];
+            p.Tq[a] = electric ? D[a] : B[a];
+            p.U[a] = electric ? UD[a] : UB[a];
+            p.W[a] = electric ? WD[a] : WB[a];
+            p.sig[a] = electric ? sigD[a] : sigB[a];
+            p.minv[a] = electric ? epsi[a] : mui[a];
+            p.inv[a] = T(1) / dx[a];
+            p.n[a] = g.n[a];
+        }
+        p.minv_s = electric ? eps_s : mu_s;
+        p.dt = dt;
+        p.sy = g.sy();
+        p.sz = g.sz();
+        p.zb = std::min(16, g.n[2]);
+        p.tiles[0] = (g.n[0] + kh::TX - 1) / kh::TX;
+        p.tiles[1] = (g.n[1] + kh::TY - 1) / kh::TY;
+        p.tiles[2] = (g.n[2] + p.zb - 1) / p.zb;
+        int nt = p.tiles[0] * p.tiles[1] * p.tiles[2];
+        p.per_xcd = (nt + kh::NXCD - 1) / kh::NXCD;
+        return p;
+    }
+
+    void launch_half(bool electric, double t) {
+        kh::HalfArgs<T> p = base_args(electric);
+        p.nsrc = 0;
+        // source boxes always define the cell class; values only while active
+        for (auto &o : srcs) {
+            if ((o.comp < 3) != electric) continue;
+            kh::SrcL<T> &l = p.src[p.nsrc++];
+            l.comp = o.comp % 3;
+            for (int a = 0; a < 3; ++a) { l.s[a] = o.s[a]; l.d[a] = o.d[a]; }
+            l.amp = o.amp;
+            l.fr = T(0);
+            l.fi = T(0);
+            if (active && ts < (int64_t)(o.wave.size() / 2)) {
+                l.fr = (T)o.wave[2 * ts];
+                l.fi = (T)o.wave[2 * ts + 1];
+            }
+        }
+        (void)t;
+        int nblk = p.per_xcd * kh::NXCD;
+        dim3 blk(kh::TX, kh::TY, 1);
+        EvPair ev{};
+        if (timing) {
+            ev = take_event(electric ? 1 : 0);
+            HIPCHK(hipEventRecord(ev.a, st));
+        }
+        if (mode == KH_MODE_EXACT) {
+            if (electric) hipLaunchKernelGGL((kh::step_e_kernel<T, true>), dim3(nblk), blk, 0, st, p);
+            else hipLaunchKernelGGL((kh::step_h_kernel<T, true>), dim3(nblk), blk, 0, st, p);
+        } else {
+            if (electric) hipLaunchKernelGGL((kh::step_e_kernel<T, false>), dim3(nblk), blk, 0, st, p);
+            else hipLaunchKernelGGL((kh::step_h_kernel<T, false>), dim3(nblk), blk, 0, st, p);
+        }
+        HIPCHK(hipGetLastError());
+        if (timing) {
+            HIPCHK(hipEventRecord(ev.b, st));
+            evs.push_back(ev);
+        }
+    }
+
+    void launch_dfts(bool electric, double t) {
+        // time_fac = complex_backend_number(im * 2 * pi * time)  (src/Monitors/Monitors.jl:287)
+        double theta = (double)(T)(6.283185307179586 * t);
+        for (auto &m : dfts) {
+            if ((m.comp < 3) != electric) continue;
+            if (m.dec > 1 && (ts % m.dec) != 0) continue;
+            const T *f = electric ? E[m.comp] : H[m.comp - 3];
+            int threads = 256;
+            int64_t blocks = (m.ncell + threads - 1) / threads;
+            size_t shm = (size_t)m.nf * 2 * sizeof(double);
+            EvPair ev{};
+            if (timing) {
+                ev = take_event(2);
+                HIPCHK(hipEventRecord(ev.a, st));
+            }
+            if (acc64)
+                hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3((unsigned)blocks), dim3(threads), shm, st,
+                                   (double *)m.acc, f, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
+                                   g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
+            else
+                hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3((unsigned)blocks), dim3(threads), shm, st,
+                                   (T *)m.acc, f, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
+                                   g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
+            HIPCHK(hipGetLastError());
+            if (timing) {
+                HIPCHK(hipEventRecord(ev.b, st));
+                evs.push_back(ev);
+            }
+        }
+    }
+
+    void step(int64_t n) override {
+        require(n >= 0, "n_steps must be >= 0");
+        stepped = true;
+        for (int64_t it = 0; it < n; ++it) {
+            double t = round_time(ts);
+            if (active && t > cutoff) active = false;  // src/Kernels/Kernels.jl:27-35
+            double th = t + (double)(dt / T(2));
+            launch_half(false, t);
+            launch_dfts(false, t);
+            launch_half(true, th);
+            launch_dfts(true, th);
+            ++ts;
+        }
+    }
+
+    void sync() override { HIPCHK(hipStreamSynchronize(st)); }
+    int64_t timestep() const override { return ts; }
+    bool sources_active() const override { return active; }
+
+    T *field_ptr(int comp) const { return comp < 3 ? E[comp] : H[comp - 3]; }
+
+    void read_field(int comp, void *host, size_t bytes) override {
+        require(comp >= 0 && comp < 6, "component out of range");
+        int cnt[3];
+        comp_count(g.n, comp, cnt);
+        int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
+        size_t need = (size_t)r[0] * r[1] * r[2] * sizeof(T);
+        require(bytes == need, "field buffer size mismatch");
+        std::vector<T> h((size_t)g.total - g.front);
+        HIPCHK(hipMemcpyAsync(h.data(), field_ptr(comp), h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        T *o = (T *)host;
+        for (int k = 0; k < r[2]; ++k)
+            for (int j = 0; j < r[1]; ++j)
+                for (int i = 0; i < r[0]; ++i) {
+                    T v = T(0);
+                    if (i <= g.n[0] + 1 && j <= g.n[1] + 1 && k <= g.n[2] + 1) v = h[g.lin(i, j, k)];
+                    o[(size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k)] = v;
+                }
+    }
+
+    void write_field(int comp, const void *host, size_t bytes) override {
+        require(comp >= 0 && comp < 6, "component out of range");
+        int cnt[3];
+        comp_count(g.n, comp, cnt);
+        int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
+        size_t need = (size_t)r[0] * r[1] * r[2] * sizeof(T);
+        require(bytes == need, "field buffer size mismatch");
+        // Only the updated cells (grid 1..N) are taken; the boundary planes stay
+        // zero.  D/B and the PML auxiliaries are left untouched.
+        std::vector<T> h((size_t)g.total - g.front, T(0));
+        const T *in = (const T *)host;
+        for (int k = 1; k <= g.n[2]; ++k)
+            for (int j = 1; j <= g.n[1]; ++j)
+                for (int i = 1; i <= g.n[0]; ++i)
+                    h[g.lin(i, j, k)] = in[(size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k)];
+        HIPCHK(hipMemcpyAsync(field_ptr(comp), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void read_dft(int id, double *host, size_t bytes) override {
+        require(id >= 0 && id < (int)dfts.size(), "bad dft id");
+        Dft &m = dfts[id];
+        size_t n = (size_t)m.ncell * m.nf * 2;
+        require(bytes == n * sizeof(double), "dft buffer size mismatch");
+        if (acc64) {
+            HIPCHK(hipMemcpyAsync(host, m.acc, n * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        } else {
+            std::vector<T> h(n);
+            HIPCHK(hipMemcpyAsync(h.data(), m.acc, n * sizeof(T), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (size_t i = 0; i < n; ++i) host[i] = (double)h[i];
+        }
+    }
+
+    void flux(int e1, int e2, int h1, int h2, int ax1, double dA, double *out) override {
+        int ids[4] = {e1, e2, h1, h2};
+        for (int q = 0; q < 4; ++q) require(ids[q] >= 0 && ids[q] < (int)dfts.size(), "bad dft id");
+        require(ax1 >= 1 && ax1 <= 3, "normal axis must be 1..3");
+        int nf = dfts[e1].nf;
+        for (int q = 0; q < 4; ++q) require(dfts[ids[q]].nf == nf, "flux monitors differ in n_freq");
+        int ax = ax1 - 1, t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+        if (t0 > t1) std::swap(t0, t1);
+        int n1 = 1 << 30, n2 = 1 << 30;
+        kh::FluxMon fm[4];
+        for (int q = 0; q < 4; ++q) {
+            Dft &m = dfts[ids[q]];
+            fm[q].acc = m.acc;
+            for (int a = 0; a < 3; ++a) fm[q].d[a] = m.d[a];
+            n1 = std::min(n1, m.d[t0]);
+            n2 = std::min(n2, m.d[t1]);
+        }
+        require(nf <= NPART * 64, "too many frequencies");
+        if (acc64)
+            hipLaunchKernelGGL(kh::flux_kernel<double>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
+                               ax, t0, t1, n1, n2, dA, scratch);
+        else
+            hipLaunchKernelGGL(kh::flux_kernel<T>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
+                               ax, t0, t1, n1, n2, dA, scratch);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, scratch, nf * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void dft_norm(int id, double *out) override {
+        require(id >= -1 && id < (int)dfts.size(), "bad dft id");
+        double total = 0.0;
+        std::vector<double> part(NPART);
+        for (int q = 0; q < (int)dfts.size(); ++q) {
+            if (id >= 0 && q != id) continue;
+            Dft &m = dfts[q];
+            int64_t n = m.ncell * m.nf;
+            if (acc64)
+                hipLaunchKernelGGL(kh::sumsq_kernel<double>, dim3(NPART), dim3(256), 0, st, (const double *)m.acc, n, scratch);
+            else
+                hipLaunchKernelGGL(kh::sumsq_kernel<T>, dim3(NPART), dim3(256), 0, st, (const T *)m.acc, n, scratch);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(part.data(), scratch, NPART * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            double s = 0.0;
+            for (double v : part) s += v;
+            total += std::sqrt(s);
+        }
+        *out = total;
+    }
+
+    void reset() override {
+        HIPCHK(hipStreamSynchronize(st));
+        size_t fb = (size_t)g.total * sizeof(T);
+        for (int a = 0; a < 3; ++a) {
+            T *arrs[8] = {E[a], H[a], D[a], B[a], UD[a], UB[a], WD[a], WB[a]};
+            for (T *p : arrs) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
+        }
+        for (auto &m : dfts)
+            HIPCHK(hipMemsetAsync(m.acc, 0, (size_t)m.ncell * m.nf * 2 * (acc64 ? sizeof(double) : sizeof(T)), st));
+        HIPCHK(hipStreamSynchronize(st));
+        ts = 0;
+        active = true;
+    }
+
+    void set_timing(int n) override {
+        timing = n != 0;
+        while ((int64_t)pool.size() < (int64_t)n) {
+            EvPair e{};
+            HIPCHK(hipEventCreate(&e.a));
+            HIPCHK(hipEventCreate(&e.b));
+            pool.push_back(e);
+        }
+    }
+
+    void get_timing(double *out) override {
+        HIPCHK(hipStreamSynchronize(st));
+        for (auto &e : evs) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            tsum[e.kind] += ms;
+            tcnt[e.kind] += 1;
+            pool.push_back(e);
+        }
+        evs.clear();
+        for (int k = 0; k < 3; ++k) {
+            out[2 * k] = tsum[k];
+            out[2 * k + 1] = tcnt[k];
+            tsum[k] = 0;
+            tcnt[k] = 0;
+        }
+    }
+
+    void *stream() override { return (void *)st; }
+};
+
+size_t footprint(const kh_sim_desc &d) {
+    Grid g;
+    g.init(d.nx, d.ny, d.nz);
+    size_t es = d.precision == KH_F32 ? 4 : 8;
+    size_t arrays = 24;
+    for (int a = 0; a < 3; ++a) arrays += (d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr);
+    size_t sig = 0;
+    for (int a = 0; a < 3; ++a) sig += 2 * ((size_t)g.n[a] + 2) * es;
+    return arrays * (size_t)g.total * es + sig + 256 * 64 * sizeof(double);
+}
+
+void validate(const kh_sim_desc *d) {
+    require(d != nullptr, "desc is null");
+    require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "grid extents must be >= 1");
+    require(d->precision == KH_F32 || d->precision == KH_F64, "precision must be KH_F32 or KH_F64");
+    require(d->mode == KH_MODE_FAST || d->mode == KH_MODE_EXACT, "mode must be KH_MODE_FAST or KH_MODE_EXACT");
+    require(d->dt > 0 && d->dx > 0 && d->dy > 0 && d->dz > 0, "dt and grid spacings must be > 0");
+    int nb = 0;
+    for (int a = 0; a < 3; ++a) nb += (d->sigma_b[a] != nullptr) + (d->sigma_d[a] != nullptr);
+    require(nb == 0 || nb == 6, "sigma_b/sigma_d: give all six profiles or none");
+}
+
+}  // namespace
+
+struct kh_ctx {
+    std::unique_ptr<CtxBase> impl;
+};
+
+extern "C" {
+
+const char *kh_last_error(void) { return g_last_error.c_str(); }
+
+int kh_version(void) { return KH_ABI_VERSION; }
+
+int kh_mem_footprint(const kh_sim_desc *desc, size_t *bytes) {
+    return guarded([&] {
+        validate(desc);
+        require(bytes != nullptr, "bytes is null");
+        *bytes = footprint(*desc);
+    });
+}
+
+int kh_create(const kh_sim_desc *desc, kh_ctx **out) {
+    return guarded([&] {
+        validate(desc);
+        require(out != nullptr, "out is null");
+        *out = nullptr;
+        auto c = std::make_unique<kh_ctx>();
+        if (desc->precision == KH_F32) c->impl = std::make_unique<Ctx<float>>(*desc);
+        else c->impl = std::make_unique<Ctx<double>>(*desc);
+        *out = c.release();
+    });
+}
+
+int kh_destroy(kh_ctx *ctx) {
+    return guarded([&] { delete ctx; });
+}
+
+#define CTX_OK require(ctx != nullptr && ctx->impl != nullptr, "ctx is null")
+
+int kh_add_source(kh_ctx *ctx, int32_t component, const int32_t start_idx[3], const int32_t dims[3],
+                  const void *amplitude, int32_t *source_id) {
+    return guarded([&] {
+        CTX_OK;
+        require(start_idx && dims && amplitude && source_id, "null argument");
+        ctx->impl->add_source(component, start_idx, dims, amplitude, source_id);
+    });
+}
+
+int kh_set_waveform(kh_ctx *ctx, int32_t source_id, int64_t n_steps, const double *values) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->set_waveform(source_id, n_steps, values);
+    });
+}
+
+int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3], const int32_t dims[3],
+               int32_t n_freq, const double *freqs, int32_t decimation, int32_t *dft_id) {
+    return guarded([&] {
+        CTX_OK;
+        require(start_idx && dims && freqs && dft_id, "null argument");
+        ctx->impl->add_dft(component, start_idx, dims, n_freq, freqs, decimation, dft_id);
+    });
+}
+
+int kh_step(kh_ctx *ctx, int64_t n_steps) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->step(n_steps);
+    });
+}
+
+int kh_sync(kh_ctx *ctx) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->sync();
+    });
+}
+
+int kh_get_timestep(kh_ctx *ctx, int64_t *timestep) {
+    return guarded([&] {
+        CTX_OK;
+        require(timestep != nullptr, "null argument");
+        *timestep = ctx->impl->timestep();
+    });
+}
+
+int kh_sources_active(kh_ctx *ctx, int32_t *active) {
+    return guarded([&] {
+        CTX_OK;
+        require(active != nullptr, "null argument");
+        *active = ctx->impl->sources_active() ? 1 : 0;
+    });
+}
+
+int kh_read_field(kh_ctx *ctx, int32_t component, void *host, size_t bytes) {
+    return guarded([&] {
+        CTX_OK;
+        require(host != nullptr, "null argument");
+        ctx->impl->read_field(component, host, bytes);
+    });
+}
+
+int kh_write_field(kh_ctx *ctx, int32_t component, const void *host, size_t bytes) {
+    return guarded([&] {
+        CTX_OK;
+        require(host != nullptr, "null argument");
+        ctx->impl->write_field(component, host, bytes);
+    });
+}
+
+int kh_read_dft(kh_ctx *ctx, int32_t dft_id, double *host, size_t bytes) {
+    return guarded([&] {
+        CTX_OK;
+        require(host != nullptr, "null argument");
+        ctx->impl->read_dft(dft_id, host, bytes);
+    });
+}
+
+int kh_flux(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis, double dA,
+            double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(out != nullptr, "null argument");
+        ctx->impl->flux(e1, e2, h1, h2, normal_axis, dA, out);
+    });
+}
+
+int kh_dft_norm(kh_ctx *ctx, int32_t dft_id, double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(out != nullptr, "null argument");
+        ctx->impl->dft_norm(dft_id, out);
+    });
+}
+
+int kh_reset(kh_ctx *ctx) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->reset();
+    });
+}
+
+int kh_set_timing(kh_ctx *ctx, int32_t enable) {
+    return guarded([&] {
+        CTX_OK;
+        require(enable >= 0, "enable must be >= 0");
+        ctx->impl->set_timing(enable);
+    });
+}
+
+int kh_get_timing(kh_ctx *ctx, double out[6]) {
+    return guarded([&] {
+        CTX_OK;
+        require(out != nullptr, "null argument");
+        ctx->impl->get_timing(out);
+    });
+}
+
+int kh_get_stream(kh_ctx *ctx, void **stream) {
+    return guarded([&] {
+        CTX_OK;
+        require(stream != nullptr, "null argument");
+        *stream = ctx->impl->stream();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+"""khronos_hip -- the reference's FDTD time-step API on an MI355X-native engine.
+
+Host setup (grid volumes, PML profiles, materials, source amplitudes, monitors)
+follows Khronos.jl's user API; every time step runs in libkhronos_hip, the
+hand-written gfx950 HIP library behind include/khronos_hip.h.
+"""
+from .grid import (Ex, Ey, Ez, Hx, Hy, Hz, Center, Field, Volume, GridVolume, grid_volume,
+                   component_voxel_count, interpolation_weight)
+from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, UniformSource,
+                      ArraySource, GaussianProfileSource)
+from .monitors import DFTMonitor, FluxMonitor
+from .geometry import Material, Object, Box, Sphere, Cylinder
+from .simulation import (Simulation, SimSpec, run, run_until, stop_when_dft_decayed, run_benchmark,
+                         reset_fields, get_flux, dft_fields_norm, prepare_simulation)
+
+__all__ = [
+    "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Center", "Field", "Volume", "GridVolume", "grid_volume",
+    "component_voxel_count", "interpolation_weight",
+    "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
+    "GaussianProfileSource", "DFTMonitor", "FluxMonitor", "Material", "Object", "Box", "Sphere",
+    "Cylinder", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
+    "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",
+]

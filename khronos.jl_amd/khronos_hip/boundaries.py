@@ -1,0 +1,88 @@
+"""PML conductivity profiles (host setup).
+
+Mirrors src/Boundaries.jl:23-72 (``sigma_helper`` / ``compute_sigma``) and the
+per-axis assembly of ``init_boundaries`` (src/Boundaries.jl:99-164): quadratic
+grading with R = 1e-15, pre-multiplied by dt/2, sampled on a half-cell grid of
+2N+1 points.  The time step reads entry ``2*i - 1`` (1-based) for cell ``i``
+of every component (src/Kernels/Helpers.jl:277).
+
+The arithmetic follows the reference's type promotion: thicknesses, grid
+spacing and dt are backend-precision numbers, so in single precision the
+position/length terms are evaluated in float32 and the profile itself in
+float64 before the final rounding to the backend type.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def sigma_value(idx: int, n: int, delta, dt, left, right, R=np.float64) -> float:
+    """Sample ``idx`` (1-based) of a profile of length ``n``, as float64.
+
+    ``delta``, ``dt``, ``left``, ``right`` are backend-precision scalars of type
+    ``R`` (src/Boundaries.jl:23-38, :41-42).
+    """
+    delta, dt, left, right = R(delta), R(dt), R(left), R(right)
+    two = R(2)
+
+    def u0(length):
+        # -log(1e-15) / (4 * L * 1 / 3) * (0.5 * dt): the denominator stays in R
+        den = ((R(4) * length) * R(1)) / R(3)
+        return -math.log(1e-15) / float(den) * (0.5 * float(dt))
+
+    def u(x):
+        # x^2 * 0.5 * (sign(x) + 1): x^2 and sign(x)+1 in R, the rest in float64
+        sgn = R(0) if x == 0 else R(math.copysign(1.0, float(x)))
+        return float(x * x) * 0.5 * float(sgn + R(1))
+
+    total = (R(n) * delta) / two
+    pos = (R(idx) * delta) / two
+    if pos < left:
+        return u0(left) * u((left - pos) / left)
+    if (total - pos) < right:
+        return u0(right) * u((right - (total - pos)) / right)
+    return 0.0
+
+
+def compute_sigma(n: int, delta, dt, left, right, R) -> np.ndarray:
+    """Length-``n`` profile in the backend precision (src/Boundaries.jl:64-72)."""
+    s = np.zeros(n, dtype=R)
+    if float(left) != 0.0 or float(right) != 0.0:
+        for idx in range(1, n + 1):
+            s[idx - 1] = R(sigma_value(idx, n, delta, dt, left, right, R))
+    return s
+
+
+def init_boundaries(sim) -> dict:
+    """Return ``{"B": [sx, sy, sz], "D": [sx, sy, sz]}``, or ``None`` when the
+    simulation has no PML (src/Boundaries.jl:94-164).
+
+    Periodic/Bloch/PEC/PMC axes get zero thickness, except that the z profile
+    of the D field keeps the raw ``boundaries`` thickness, as the reference
+    does (src/Boundaries.jl:154-162).
+    """
+    if sim.boundaries is None:
+        return None
+    R = sim.real
+    nb = [[R(v) for v in b] for b in sim.boundaries]
+    eff = [list(b) for b in nb]
+    if sim.boundary_conditions is not None:
+        for axis, bcs in enumerate(sim.boundary_conditions):
+            for side, bc in enumerate(bcs):
+                if bc is not None and getattr(bc, "zero_pml", False):
+                    eff[axis][side] = R(0)
+    N = [sim.Nx, sim.Ny, sim.Nz]
+    out = {"B": [], "D": []}
+    for a in range(3):
+        if a == 2 and sim.ndims <= 2:
+            out["B"].append(None)
+            out["D"].append(None)
+            continue
+        n = 2 * N[a] + 1
+        d = sim.delta_r[a]
+        out["B"].append(compute_sigma(n, d, sim.dt_r, eff[a][0], eff[a][1], R))
+        zb = nb[a] if a == 2 else eff[a]
+        out["D"].append(compute_sigma(n, d, sim.dt_r, zb[0], zb[1], R))
+    return out

@@ -1,0 +1,200 @@
+"""Time-step engine backed by libkhronos_hip (the only engine of the package).
+
+Takes a :class:`~khronos_hip.simulation.SimSpec` (host setup in the
+reference's layouts), creates the device context and exposes stepping and
+readout.  Waveforms of the sources are evaluated on the host, in the backend
+precision, exactly where the reference evaluates them (t for H sources,
+t + dt/2 for E sources; src/Kernels/Kernels.jl:64-74), and uploaded in blocks
+ahead of the steps that use them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .grid import component_voxel_count, component_from_code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class HipEngine:
+    WAVE_BLOCK = 4096
+
+    def __init__(self, spec):
+        self.lib = _lib.load()
+        self.spec = spec
+        R = spec.real
+        self.R = R
+        self._keep = []  # host arrays referenced during kh_create
+        d = _lib.KhSimDesc()
+        d.nx, d.ny, d.nz = spec.N
+        d.precision = _lib.KH_F32 if R is np.float32 else _lib.KH_F64
+        d.mode = _lib.KH_MODE_EXACT if spec.mode == "exact" else _lib.KH_MODE_FAST
+        d.dft_accum_f64 = 1 if spec.dft_accum_f64 else 0
+        d.dx, d.dy, d.dz = spec.delta
+        d.dt = spec.dt
+        if spec.sigma is not None:
+            for a in range(3):
+                sb = np.ascontiguousarray(spec.sigma["B"][a], dtype=R)
+                sd = np.ascontiguousarray(spec.sigma["D"][a], dtype=R)
+                self._keep += [sb, sd]
+                d.sigma_b[a] = _ptr(sb)
+                d.sigma_d[a] = _ptr(sd)
+        for a in range(3):
+            for arrs, slot in ((spec.eps_inv, d.eps_inv), (spec.mu_inv, d.mu_inv)):
+                if arrs[a] is not None:
+                    h = np.asfortranarray(arrs[a], dtype=R)
+                    self._keep.append(h)
+                    slot[a] = _ptr(h)
+        d.eps_inv_scalar = spec.eps_inv_scalar
+        d.mu_inv_scalar = spec.mu_inv_scalar
+        d.source_cutoff_time = spec.cutoff
+        self.desc = d
+        ctx = C.c_void_p()
+        _lib.check(self.lib.kh_create(C.byref(d), C.byref(ctx)))
+        self.ctx = ctx
+        self._keep = []
+        # sources
+        self.src_ids = []
+        self.src_wave_len = []
+        for sd in spec.sources:
+            amp = np.asfortranarray(sd.amplitude.astype(np.complex64 if R is np.float32 else np.complex128))
+            st = (C.c_int32 * 3)(*sd.gv.start_idx)
+            dm = (C.c_int32 * 3)(*amp.shape)
+            sid = C.c_int32()
+            _lib.check(self.lib.kh_add_source(self.ctx, _code(sd.component), st, dm, _ptr(amp), C.byref(sid)))
+            self.src_ids.append(sid.value)
+            self.src_wave_len.append(0)
+        # DFT monitors
+        self.dft_ids = []
+        for md in spec.dfts:
+            st = (C.c_int32 * 3)(*md.gv.start_idx)
+            dm = (C.c_int32 * 3)(*md.gv.dims)
+            fr = np.ascontiguousarray(md.frequencies, dtype=np.float64)
+            did = C.c_int32()
+            _lib.check(self.lib.kh_add_dft(self.ctx, _code(md.component), st, dm, len(fr), _ptr(fr),
+                                           int(md.decimation), C.byref(did)))
+            self.dft_ids.append(did.value)
+        self.ts = 0
+
+    def __del__(self):
+        try:
+            if getattr(self, "ctx", None):
+                self.lib.kh_destroy(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- waveforms
+    def _wave_times(self, electric: bool, n0: int, n1: int):
+        R = self.R
+        dt = R(self.spec.dt)
+        half = float(dt / R(2))
+        ts = [float(R(n) * dt) for n in range(n0, n1)]
+        return [t + half for t in ts] if electric else ts
+
+    def _ensure_waveforms(self, n_end: int):
+        spec = self.spec
+        # sources stop once t > cutoff: no values are needed past that step
+        cap = int(math.floor(spec.cutoff / spec.dt)) + 3 if math.isfinite(spec.cutoff) else None
+        if not hasattr(self, "_wave"):
+            self._wave = [np.zeros(0) for _ in spec.sources]
+        for q, sd in enumerate(spec.sources):
+            have = self.src_wave_len[q]
+            need = n_end if cap is None else min(n_end, cap)
+            if have >= need:
+                continue
+            target = need if cap is not None else max(need, have + self.WAVE_BLOCK)
+            times = self._wave_times(sd.component.electric, have, target)
+            new = np.empty(2 * len(times))
+            for i, t in enumerate(times):
+                v = sd.time_profile.eval(t, self.R)
+                new[2 * i] = v.real
+                new[2 * i + 1] = v.imag
+            self._wave[q] = np.concatenate([self._wave[q], new])
+            _lib.check(self.lib.kh_set_waveform(self.ctx, self.src_ids[q], target, _ptr(self._wave[q])))
+            self.src_wave_len[q] = target
+
+    # -------------------------------------------------------------- stepping
+    def step(self, n: int):
+        if n <= 0:
+            return
+        self._ensure_waveforms(self.ts + n)
+        _lib.check(self.lib.kh_step(self.ctx, int(n)))
+        self.ts += n
+
+    def sync(self):
+        _lib.check(self.lib.kh_sync(self.ctx))
+
+    def sources_active(self) -> bool:
+        v = C.c_int32()
+        _lib.check(self.lib.kh_sources_active(self.ctx, C.byref(v)))
+        return bool(v.value)
+
+    def timestep(self) -> int:
+        v = C.c_int64()
+        _lib.check(self.lib.kh_get_timestep(self.ctx, C.byref(v)))
+        return v.value
+
+    # -------------------------------------------------------------- readout
+    def field_shape(self, code: int):
+        cnt = component_voxel_count(*self.spec.N, component_from_code(code))
+        return tuple(c + 2 for c in cnt)
+
+    def read_field(self, code: int) -> np.ndarray:
+        shp = self.field_shape(code)
+        out = np.zeros(shp, dtype=self.R, order="F")
+        _lib.check(self.lib.kh_read_field(self.ctx, code, _ptr(out), out.nbytes))
+        return out
+
+    def write_field(self, code: int, arr: np.ndarray):
+        a = np.asfortranarray(arr, dtype=self.R)
+        assert a.shape == self.field_shape(code)
+        _lib.check(self.lib.kh_write_field(self.ctx, code, _ptr(a), a.nbytes))
+
+    def read_dft(self, i: int) -> np.ndarray:
+        md = self.spec.dfts[i]
+        shp = tuple(md.gv.dims) + (len(md.frequencies),)
+        buf = np.zeros(int(np.prod(shp)) * 2, dtype=np.float64)
+        _lib.check(self.lib.kh_read_dft(self.ctx, self.dft_ids[i], _ptr(buf), buf.nbytes))
+        z = buf[0::2] + 1j * buf[1::2]
+        return z.reshape(shp, order="F")
+
+    def flux(self, ids, normal_axis: int, dA: float) -> np.ndarray:
+        nf = len(self.spec.dfts[ids[0]].frequencies)
+        out = np.zeros(nf, dtype=np.float64)
+        e1, e2, h1, h2 = (self.dft_ids[i] for i in ids)
+        _lib.check(self.lib.kh_flux(self.ctx, e1, e2, h1, h2, int(normal_axis), float(dA), _ptr(out)))
+        return out
+
+    def dft_norms(self):
+        out = []
+        for did in self.dft_ids:
+            v = C.c_double()
+            _lib.check(self.lib.kh_dft_norm(self.ctx, did, C.byref(v)))
+            out.append(v.value)
+        return out
+
+    def reset(self):
+        _lib.check(self.lib.kh_reset(self.ctx))
+        self.ts = 0
+
+    # -------------------------------------------------------------- timing
+    def set_timing(self, on: bool):
+        _lib.check(self.lib.kh_set_timing(self.ctx, 1 if on else 0))
+
+    def get_timing(self):
+        out = np.zeros(6, dtype=np.float64)
+        _lib.check(self.lib.kh_get_timing(self.ctx, _ptr(out)))
+        return {"h_ms": out[0], "h_n": int(out[1]), "e_ms": out[2], "e_n": int(out[3]),
+                "dft_ms": out[4], "dft_n": int(out[5])}
+
+
+def _code(component) -> int:
+    from .grid import component_code
+    return component_code(component)

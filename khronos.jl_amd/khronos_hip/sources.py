@@ -1,0 +1,256 @@
+"""Time profiles and spatial current sources (host setup).
+
+* Gaussian pulse / CW / custom time profiles -- src/Sources/TimeSources.jl:53-190
+* UniformSource, amplitude assembly        -- src/Sources/SpatialSources.jl:99-129,
+                                              src/Sources/Sources.jl:44-131
+* ArraySource: caller-supplied complex amplitude on the source grid volume
+  (the role the reference's EquivalentSource/ModeSource profiles play:
+  a host-computed array the time step only scales and adds).
+
+The time step sees a source as (component, grid volume, complex amplitude
+array, waveform f(t) per half step); ``S += Re(f(t) * A)`` on the volume
+(src/Sources/Sources.jl:346-357).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from .grid import Field, Volume, grid_volume, component_origin, weight_1d
+
+
+# --------------------------------------------------------------------------
+# Time profiles
+# --------------------------------------------------------------------------
+class TimeSource:
+    def eval(self, t: float, R) -> complex:  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def cutoff(self) -> float:
+        raise NotImplementedError
+
+
+def _f32_exp(x):
+    return np.float32(math.exp(float(x)))
+
+
+@dataclass
+class ContinuousWaveData(TimeSource):
+    """src/Sources/TimeSources.jl:53-72."""
+
+    fcen: float
+
+    def eval(self, t: float, R) -> complex:
+        two_pi = R(2) * R(math.pi)
+        ph = (two_pi * R(self.fcen)) * R(t)
+        # exp(-i * ph): cos/sin evaluated in float64 and rounded to R
+        return complex(R(math.cos(float(ph))), -R(math.sin(float(ph))))
+
+    def cutoff(self) -> float:
+        raise ValueError("Cutoff not possible with CW source...")
+
+    @property
+    def f_max(self):
+        return float(self.fcen)
+
+
+@dataclass
+class GaussianPulseData(TimeSource):
+    """src/Sources/TimeSources.jl:78-138 (fields as stored by the reference)."""
+
+    fcen: float
+    fwidth: float
+    width: float
+    peak_time: float
+    cutoff_t: float
+
+    def eval(self, t: float, R) -> complex:
+        fcen, w, peak, cut = R(self.fcen), R(self.width), R(self.peak_time), R(self.cutoff_t)
+        tt = R(t) - peak
+        if tt > cut:
+            return complex(0.0, 0.0)
+        two_pi = R(2) * R(math.pi)
+        g = R(math.exp(float(((-tt) * tt) / ((R(2) * w) * w))))
+        b = ((-two_pi) * fcen) * tt  # imaginary part of -2*pi*i*fcen*tt
+        c = R(math.cos(float(b)))
+        s = R(math.sin(float(b)))
+        return complex(g * c, g * s)
+
+    def cutoff(self) -> float:
+        return float(self.cutoff_t)
+
+    @property
+    def f_max(self):
+        return float(self.fcen) + float(self.fwidth) / 2
+
+
+def _gaussian_bandwidth(width: float) -> float:
+    # src/Sources/TimeSources.jl:118-121
+    tol = 1e-7
+    return math.sqrt(-2.0 * math.log(tol)) / (width * math.pi)
+
+
+def GaussianPulseSource(*, fcen, fwidth, start_time=0.0, cutoff_scale=5.0, real=np.float64):
+    """src/Sources/TimeSources.jl:91-109 (peak snapped to a multiple of 1/fcen)."""
+    width = 1.0 / fwidth
+    cutoff = width * cutoff_scale + start_time
+    fw = _gaussian_bandwidth(width)
+    while math.exp(-cutoff * cutoff / (2 * width * width)) < 1e-100:
+        cutoff *= 0.9
+    peak_raw = cutoff / 2
+    period = 1.0 / fcen
+    peak = round(peak_raw / period) * period  # ties to even, like Julia's round
+    R = real
+    return GaussianPulseData(float(R(fcen)), float(R(fw)), float(R(width)), float(R(peak)), float(R(cutoff)))
+
+
+def ContinuousWaveSource(*, fcen, real=np.float64):
+    return ContinuousWaveData(float(real(fcen)))
+
+
+@dataclass
+class CustomSourceData(TimeSource):
+    """src/Sources/TimeSources.jl:157-190: arbitrary ``src_func(t)``."""
+
+    src_func: Callable[[float], complex]
+    fcen: float
+    fwidth: float
+    end_time: float
+
+    def eval(self, t: float, R) -> complex:
+        if R(t) > R(self.end_time):
+            return complex(0.0, 0.0)
+        v = complex(self.src_func(float(R(t))))
+        return complex(R(v.real), R(v.imag))
+
+    def cutoff(self) -> float:
+        return float(self.end_time)
+
+    @property
+    def f_max(self):
+        return float(self.fcen) + float(self.fwidth) / 2
+
+
+def CustomSource(*, src_func, fcen, fwidth, end_time):
+    return CustomSourceData(src_func, float(fcen), float(fwidth), float(end_time))
+
+
+# --------------------------------------------------------------------------
+# Spatial sources
+# --------------------------------------------------------------------------
+class Source:
+    time_profile: TimeSource
+    center: Sequence[float]
+    size: Sequence[float]
+    amplitude: complex
+
+    def components(self):
+        raise NotImplementedError
+
+    def profile(self, x, y, z, component) -> np.ndarray:
+        """Complex spatial profile evaluated on the broadcast grid (x, y, z)."""
+        return np.ones(np.broadcast(x, y, z).shape, dtype=np.complex128)
+
+
+@dataclass
+class UniformSource(Source):
+    """src/Sources/SpatialSources.jl:99-129."""
+
+    time_profile: TimeSource
+    component: Field
+    amplitude: complex = 1.0
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+    size: Sequence[float] = (0.0, 0.0, 0.0)
+
+    def components(self):
+        return [self.component]
+
+
+@dataclass
+class ArraySource(Source):
+    """A source whose complex profile is a host function of position.
+
+    ``profile_fn(x, y, z, component)`` receives broadcastable coordinate arrays
+    and returns the complex profile; the amplitude assembly multiplies it by
+    the interpolation weight and ``amplitude`` exactly as the reference does
+    for its non-uniform spatial sources (src/Sources/Sources.jl:101-131).
+    """
+
+    time_profile: TimeSource
+    component: Field
+    profile_fn: Callable = None
+    amplitude: complex = 1.0
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+    size: Sequence[float] = (0.0, 0.0, 0.0)
+
+    def components(self):
+        return [self.component]
+
+    def profile(self, x, y, z, component):
+        return np.asarray(self.profile_fn(x, y, z, component), dtype=np.complex128)
+
+
+def GaussianProfileSource(*, time_profile, component, center, size, waist, amplitude=1.0):
+    """Planar source with a Gaussian transverse envelope of 1/e field radius ``waist``."""
+    c = [float(v) for v in center]
+
+    def fn(x, y, z, comp):
+        r2 = np.zeros(np.broadcast(x, y, z).shape)
+        for a, coord in enumerate((x, y, z)):
+            if float(size[a]) != 0.0:
+                r2 = r2 + (coord - c[a]) ** 2
+        return np.exp(-r2 / (waist * waist)).astype(np.complex128)
+
+    return ArraySource(time_profile=time_profile, component=component, profile_fn=fn,
+                       amplitude=amplitude, center=center, size=size)
+
+
+@dataclass
+class SourceData:
+    """What the time step needs for one (source, component) pair."""
+
+    component: Field
+    gv: object
+    amplitude: np.ndarray  # complex, shape (Nx, Ny, Nz) of the grid volume
+    time_profile: TimeSource
+
+
+def assemble_source(sim, src: Source):
+    """Complex amplitude arrays for every component of ``src``
+    (src/Sources/Sources.jl:44-131): weight(point) * amplitude * profile(point).
+    """
+    vol = Volume(src.center, src.size)
+    mn, mx = vol.min_corner(), vol.max_corner()
+    out = []
+    d = [float(v) for v in sim.delta_n]
+    for comp in src.components():
+        gv = grid_volume(sim, vol, comp)
+        org = component_origin(sim.cell_center, sim.cell_size_n, sim.delta_n, comp)
+        dims = (gv.Nx, gv.Ny, max(1, gv.Nz))
+        coords = []
+        weights = []
+        for a in range(3):
+            pts = []
+            for i in range(1, dims[a] + 1):
+                p = org[a] + (i + gv.start_idx[a] - 2) * d[a]
+                if a == 2 and (math.isinf(p) or math.isnan(p)):
+                    p = float(sim.cell_center[2])
+                pts.append(p)
+            pts = np.array(pts)
+            coords.append(pts)
+            if a < sim.ndims:
+                weights.append(np.array([weight_1d(p, mn[a], mx[a], float(src.size[a]), d[a]) for p in pts]))
+            else:
+                weights.append(np.ones(len(pts)))
+        X = coords[0][:, None, None]
+        Y = coords[1][None, :, None]
+        Z = coords[2][None, None, :]
+        w = weights[0][:, None, None] * weights[1][None, :, None] * weights[2][None, None, :]
+        prof = src.profile(X, Y, Z, comp)
+        amp = (w * complex(src.amplitude)) * prof
+        amp = np.broadcast_to(amp, dims).astype(np.complex128)
+        out.append(SourceData(comp, gv, np.ascontiguousarray(amp), src.time_profile))
+    return out

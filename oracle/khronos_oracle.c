@@ -1,0 +1,492 @@
+/*
+ * khronos_oracle.c -- CPU restatement of the reference's time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+ * and never as the thing measured or shipped.
+ *
+ * It restates, on the reference's own array layout (column-major, x fastest,
+ * one ghost layer per side, staggered component extents src/utils.jl:26-37,
+ * src/Fields.jl:131-142), what the reference executes on its KernelAbstractions
+ * CPU() backend in a single-chunk simulation (num_chunks = 1):
+ *
+ *   step order ................ src/Kernels/Kernels.jl:20-88
+ *   curl (forward/backward) ... src/Kernels/Helpers.jl:283-298
+ *   sigma cascade dispatch .... src/Kernels/Helpers.jl:30-270 (generic_curl!)
+ *   W/material stage .......... src/Kernels/Helpers.jl:323-366
+ *   curl kernel, update kernel  src/Kernels/ReferenceKernels.jl:276-313, :468-512
+ *   source injection .......... src/Sources/Sources.jl:346-357
+ *   Gaussian / CW waveform .... src/Sources/TimeSources.jl:61-64, :123-132
+ *   DFT accumulation .......... src/Monitors/Monitors.jl:274-290, :361-379
+ *   decimation gate ........... src/Kernels/Kernels.jl:465-497
+ *   source shut-off ........... src/Kernels/Kernels.jl:27-35
+ *   time of a step ............ src/Simulation.jl:22
+ *
+ * Build twice: -DREAL=float (liboracle_f32.so) and -DREAL=double
+ * (liboracle_f64.so), always with -ffp-contract=off (Julia does not contract
+ * a*b+c into FMA).
+ *
+ * Auxiliary-array presence follows src/Fields.jl:266-305 for one chunk: with
+ * PML boundaries every component has U and W; C exists only with a material
+ * conductivity array.  A method combination the reference has no method for
+ * (e.g. C without U and a scalar sigma_next) is rejected with an error.
+ */
+#include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+
+typedef REAL real;
+
+#define NCOMP 6
+enum { EX = 0, EY, EZ, HX, HY, HZ };
+
+typedef struct {
+    int comp, start[3], dims[3];
+    real *amp; /* interleaved re, im; column-major dims */
+    int kind;  /* 0 gaussian, 1 cw, 2 table */
+    double p[4];
+    double *table; /* 2*n entries, values per step of the half step that uses it */
+    int64_t ntable;
+} orc_source;
+
+typedef struct {
+    int comp, start[3], dims[3], nf, decim, acc_f64;
+    double *freqs;
+    void *acc; /* complex, (dims..., nf), column-major; float pairs or double pairs */
+} orc_dft;
+
+typedef struct {
+    int nx, ny, nz;
+    real dx, dy, dz, dt;
+    int dim[NCOMP][3];   /* raw extents incl. ghosts */
+    int gdim[NCOMP][3];  /* geometry extents (component voxel count) */
+    real *F[NCOMP];      /* E/H */
+    real *T[NCOMP];      /* D/B */
+    real *C[NCOMP], *U[NCOMP], *W[NCOMP], *S[NCOMP];
+    real *sig[2][3];     /* [0]=B (H step), [1]=D (E step); length 2N+1 or NULL */
+    real *msig[NCOMP];   /* material conductivity sigma_D / sigma_B (geometry layout) */
+    real *minv[NCOMP];   /* per-voxel eps^-1 / mu^-1 (geometry layout) or NULL */
+    real minv_s[2];      /* scalar mu^-1 (H), eps^-1 (E) */
+    int has_bound;
+    double cutoff;
+    int sources_active;
+    int64_t timestep;
+    orc_source *src;
+    int nsrc;
+    orc_dft *dft;
+    int ndft;
+} orc_sim;
+
+static void comp_count(int nx, int ny, int nz, int c, int out[3]) {
+    /* src/utils.jl:26-37 */
+    int n[3] = {nx, ny, nz < 1 ? 1 : nz};
+    int a = c % 3;
+    if (c < 3) {
+        out[0] = n[0] + 1; out[1] = n[1] + 1; out[2] = n[2] + 1;
+        out[a] = n[a];
+    } else {
+        out[0] = n[0]; out[1] = n[1]; out[2] = n[2];
+        out[a] = n[a] + 1;
+    }
+}
+
+static inline size_t rix(const int *d, int i, int j, int k) {
+    /* 1-based raw index into a column-major array of extents d */
+    return (size_t)(i - 1) + (size_t)d[0] * ((size_t)(j - 1) + (size_t)d[1] * (size_t)(k - 1));
+}
+
+static real *zalloc(size_t n) { return (real *)calloc(n ? n : 1, sizeof(real)); }
+
+static size_t vol(const int *d) { return (size_t)d[0] * d[1] * d[2]; }
+
+orc_sim *orc_create(int nx, int ny, int nz, double dx, double dy, double dz, double dt,
+                    const real *sig_b0, const real *sig_b1, const real *sig_b2,
+                    const real *sig_d0, const real *sig_d1, const real *sig_d2,
+                    const real *minv0, const real *minv1, const real *minv2,
+                    const real *minv3, const real *minv4, const real *minv5,
+                    const real *msig0, const real *msig1, const real *msig2,
+                    const real *msig3, const real *msig4, const real *msig5,
+                    double eps_inv_s, double mu_inv_s, double cutoff) {
+    orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+    const real *sb[3] = {sig_b0, sig_b1, sig_b2};
+    const real *sd[3] = {sig_d0, sig_d1, sig_d2};
+    const real *mi[6] = {minv0, minv1, minv2, minv3, minv4, minv5};
+    const real *ms[6] = {msig0, msig1, msig2, msig3, msig4, msig5};
+    int N[3] = {nx, ny, nz};
+    s->nx = nx; s->ny = ny; s->nz = nz;
+    s->dx = (real)dx; s->dy = (real)dy; s->dz = (real)dz; s->dt = (real)dt;
+    s->minv_s[0] = (real)mu_inv_s;
+    s->minv_s[1] = (real)eps_inv_s;
+    s->cutoff = cutoff;
+    s->sources_active = 1;
+    s->has_bound = (sb[0] != NULL);
+    for (int a = 0; a < 3; a++) {
+        size_t n = 2 * (size_t)N[a] + 1;
+        if (sb[a]) { s->sig[0][a] = zalloc(n); memcpy(s->sig[0][a], sb[a], n * sizeof(real)); }
+        if (sd[a]) { s->sig[1][a] = zalloc(n); memcpy(s->sig[1][a], sd[a], n * sizeof(real)); }
+    }
+    for (int c = 0; c < NCOMP; c++) {
+        comp_count(nx, ny, nz, c, s->gdim[c]);
+        for (int a = 0; a < 3; a++) s->dim[c][a] = s->gdim[c][a] + 2;
+        size_t n = vol(s->dim[c]);
+        s->F[c] = zalloc(n);
+        s->T[c] = zalloc(n);
+        int e = c < 3;
+        int own = c % 3, nxt = (own + 1) % 3, prv = (own + 2) % 3;
+        real **sg = s->sig[e ? 1 : 0];
+        /* src/Fields.jl:266-305 */
+        if (sg[nxt]) s->U[c] = zalloc(n);
+        if (sg[own]) s->W[c] = zalloc(n);
+        if (ms[c]) {
+            size_t g = vol(s->gdim[c]);
+            s->msig[c] = zalloc(g);
+            memcpy(s->msig[c], ms[c], g * sizeof(real));
+            if (sg[nxt] || sg[prv]) s->C[c] = zalloc(n);
+        }
+        if (mi[c]) {
+            size_t g = vol(s->gdim[c]);
+            s->minv[c] = zalloc(g);
+            memcpy(s->minv[c], mi[c], g * sizeof(real));
+        }
+    }
+    return s;
+}
+
+void orc_destroy(orc_sim *s) {
+    if (!s) return;
+    for (int c = 0; c < NCOMP; c++) {
+        free(s->F[c]); free(s->T[c]); free(s->C[c]); free(s->U[c]); free(s->W[c]);
+        free(s->S[c]); free(s->msig[c]); free(s->minv[c]);
+    }
+    for (int e = 0; e < 2; e++)
+        for (int a = 0; a < 3; a++) free(s->sig[e][a]);
+    for (int i = 0; i < s->nsrc; i++) { free(s->src[i].amp); free(s->src[i].table); }
+    for (int i = 0; i < s->ndft; i++) { free(s->dft[i].freqs); free(s->dft[i].acc); }
+    free(s->src); free(s->dft); free(s);
+}
+
+/* kind 0: gaussian p = {fcen, width, peak_time, cutoff}; kind 1: cw p = {fcen};
+ * kind 2: explicit table (orc_set_table). */
+int orc_add_source(orc_sim *s, int comp, const int *start, const int *dims, const real *amp,
+                   int kind, const double *p) {
+    s->src = (orc_source *)realloc(s->src, (size_t)(s->nsrc + 1) * sizeof(orc_source));
+    orc_source *o = &s->src[s->nsrc];
+    memset(o, 0, sizeof(*o));
+    o->comp = comp;
+    for (int a = 0; a < 3; a++) { o->start[a] = start[a]; o->dims[a] = dims[a]; }
+    size_t n = vol(dims);
+    o->amp = zalloc(2 * n);
+    memcpy(o->amp, amp, 2 * n * sizeof(real));
+    o->kind = kind;
+    if (p) for (int i = 0; i < 4; i++) o->p[i] = p[i];
+    /* the reference allocates a full-size S array for every source component */
+    if (!s->S[comp]) s->S[comp] = zalloc(vol(s->dim[comp]));
+    return s->nsrc++;
+}
+
+int orc_set_table(orc_sim *s, int id, int64_t n, const double *values) {
+    orc_source *o = &s->src[id];
+    free(o->table);
+    o->table = (double *)malloc((size_t)(2 * n) * sizeof(double));
+    memcpy(o->table, values, (size_t)(2 * n) * sizeof(double));
+    o->ntable = n;
+    return 0;
+}
+
+int orc_add_dft(orc_sim *s, int comp, const int *start, const int *dims, int nf,
+                const double *freqs, int decim, int acc_f64) {
+    s->dft = (orc_dft *)realloc(s->dft, (size_t)(s->ndft + 1) * sizeof(orc_dft));
+    orc_dft *d = &s->dft[s->ndft];
+    memset(d, 0, sizeof(*d));
+    d->comp = comp; d->nf = nf; d->decim = decim; d->acc_f64 = acc_f64;
+    for (int a = 0; a < 3; a++) { d->start[a] = start[a]; d->dims[a] = dims[a]; }
+    d->freqs = (double *)malloc((size_t)nf * sizeof(double));
+    memcpy(d->freqs, freqs, (size_t)nf * sizeof(double));
+    size_t n = vol(dims) * (size_t)nf * 2;
+    d->acc = calloc(n, acc_f64 ? sizeof(double) : sizeof(real));
+    return s->ndft++;
+}
+
+/* ---------------------------------------------------------------- time */
+static double round_time(const orc_sim *s) {
+    /* Float64(timestep * dt): the product is formed in the backend precision */
+    return (double)((real)s->timestep * s->dt);
+}
+
+static void waveform(const orc_sim *s, const orc_source *o, double t, real *re, real *im) {
+    (void)s;
+    if (o->kind == 0) {
+        /* src/Sources/TimeSources.jl:123-132 */
+        real fcen = (real)o->p[0], w = (real)o->p[1], peak = (real)o->p[2], cut = (real)o->p[3];
+        real tt = (real)t - peak;
+        if (tt > cut) { *re = 0; *im = 0; return; }
+        real two_pi = (real)2 * (real)M_PI;
+        real g = (real)exp((double)(((-tt) * tt) / (((real)2 * w) * w)));
+        real b = ((-two_pi) * fcen) * tt;
+        real c = (real)cos((double)b), sn = (real)sin((double)b);
+        *re = g * c; *im = g * sn;
+    } else if (o->kind == 1) {
+        /* src/Sources/TimeSources.jl:61-64: exp(-i*2*pi*fcen*t) */
+        real two_pi = (real)2 * (real)M_PI;
+        real ph = (two_pi * (real)o->p[0]) * (real)t;
+        *re = (real)cos((double)ph);
+        *im = -(real)sin((double)ph);
+    } else {
+        int64_t n = s->timestep;
+        if (n < o->ntable) { *re = (real)o->table[2 * n]; *im = (real)o->table[2 * n + 1]; }
+        else { *re = 0; *im = 0; }
+    }
+}
+
+static void step_sources(orc_sim *s, int magnetic, double t) {
+    /* src/Sources/Sources.jl:330-357: S[off + i] += real(f(t) * A[i]) */
+    for (int q = 0; q < s->nsrc; q++) {
+        orc_source *o = &s->src[q];
+        if ((o->comp >= 3) != magnetic) continue;
+        real fr, fi;
+        waveform(s, o, t, &fr, &fi);
+        real *S = s->S[o->comp];
+        const int *d = s->dim[o->comp];
+        for (int iz = 1; iz <= o->dims[2]; iz++)
+            for (int iy = 1; iy <= o->dims[1]; iy++)
+                for (int ix = 1; ix <= o->dims[0]; ix++) {
+                    size_t ai = (size_t)(ix - 1) + (size_t)o->dims[0] * ((size_t)(iy - 1) + (size_t)o->dims[1] * (size_t)(iz - 1));
+                    real ar = o->amp[2 * ai], aim = o->amp[2 * ai + 1];
+                    real v = (fr * ar) - (fi * aim);
+                    S[rix(d, ix + o->start[0], iy + o->start[1], iz + o->start[2])] += v;
+                }
+    }
+}
+
+/* ------------------------------------------------------- curl + cascade */
+/* update_field_from_curl (src/Kernels/Helpers.jl:30-33); has_old / has_sig
+ * encode the Nothing dispatch. */
+static inline real ufc(real A, real B, int has_old, real Bold, int has_sig, real sig) {
+    if (has_sig) {
+        if (has_old) return (((real)1 - sig) * A + B - Bold) / ((real)1 + sig);
+        return (((real)1 - sig) * A + B) / ((real)1 + sig);
+    }
+    if (has_old) return A + B - Bold;
+    return A + B;
+}
+
+static int g_error = 0;
+
+/* generic_curl! (src/Kernels/Helpers.jl:39-270), method resolution spelled out.
+ * Updates T (and C, U) in place; returns 1 when the reference has no method
+ * for the combination, 0 otherwise. */
+static inline int generic_curl(real K, real *C, real *U, real *T, size_t f,
+                                int hsD, real sD, int hn, real sn, int hp, real sp) {
+    if (C && U) {
+        if (!hsD) { /* :72 */
+            real Uo = U[f];
+            U[f] = ufc(U[f], K, 0, 0, hn, sn);
+            T[f] = ufc(T[f], U[f], 1, Uo, hp, sp);
+            return 0;
+        }
+        /* :39 */
+        if (sD == (real)0) {
+            real Uo = U[f];
+            U[f] = ufc(U[f], K, 0, 0, hn, sn);
+            T[f] = ufc(T[f], U[f], 1, Uo, hp, sp);
+        } else {
+            real Co = C[f];
+            C[f] = ufc(C[f], K, 0, 0, 1, sD);
+            real Uo = U[f];
+            U[f] = ufc(U[f], C[f], 1, Co, hn, sn);
+            T[f] = ufc(T[f], U[f], 1, Uo, hp, sp);
+        }
+        return 0;
+    }
+    if (C && !U) {
+        if (hn) return 1; /* no method (only the never-taken array sigma_next one) */
+        if (!hsD) { T[f] = ufc(T[f], K, 0, 0, hp, sp); return 0; } /* :96 */
+        if (sD == (real)0) { T[f] = ufc(T[f], K, 0, 0, hp, sp); return 0; } /* :80 */
+        real Co = C[f];
+        C[f] = ufc(C[f], K, 0, 0, 1, sD);
+        T[f] = ufc(T[f], C[f], 1, Co, hp, sp);
+        return 0;
+    }
+    if (!C && U) {
+        if (hsD || !hn) return 1; /* no method */
+        /* :113 */
+        if (sn == (real)0) { T[f] = ufc(T[f], K, 0, 0, hp, sp); return 0; }
+        real Uo = U[f];
+        U[f] = ufc(U[f], K, 0, 0, 1, sn);
+        T[f] = ufc(T[f], U[f], 1, Uo, hp, sp);
+        return 0;
+    }
+    /* C and U both absent */
+    if (!hsD) { T[f] = T[f] + K; return 0; } /* :188, :208, :242, :257 */
+    if (!hn && !hp) return 1; /* :141 vs :225 ambiguous */
+    T[f] = ufc(T[f], K, 0, 0, 1, sD); /* :225 */
+    return 0;
+}
+
+/* sigma sample for cell i (1-based): sig[2i-1] (src/Kernels/Helpers.jl:277) */
+static inline real sig_at(const real *sg, int i) { return sg[2 * i - 2]; }
+
+static void half_step(orc_sim *s, int electric) {
+    /* step_curl! then update_field! (src/Kernels/Kernels.jl:260-283) */
+    int base = electric ? 0 : 3;     /* components being updated */
+    int src = electric ? 3 : 0;      /* components differentiated */
+    int idx = electric ? -1 : 1;     /* idx_curl */
+    real **sg = s->sig[electric ? 1 : 0];
+    real inv[3] = {(real)1 / s->dx, (real)1 / s->dy, (real)1 / s->dz};
+    int N[3] = {s->nx, s->ny, s->nz};
+    const real *A[3] = {s->F[src], s->F[src + 1], s->F[src + 2]};
+    const int *ad[3] = {s->dim[src], s->dim[src + 1], s->dim[src + 2]};
+    int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
+    for (int iz = 1; iz <= N[2]; iz++)
+        for (int iy = 1; iy <= N[1]; iy++)
+            for (int ix = 1; ix <= N[0]; ix++) {
+                int fx = ix + 1, fy = iy + 1, fz = iz + 1;
+                int gi[3] = {ix, iy, iz};
+                for (int a = 0; a < 3; a++) {
+                    int c = base + a;
+                    int n1 = (a + 1) % 3, n2 = (a + 2) % 3; /* next, prev */
+                    /* curl_a = d_n2(A_n1) - d_n1(A_n2)  (src/Kernels/Helpers.jl:293-298) */
+                    int o1[3] = {0, 0, 0}, o2[3] = {0, 0, 0};
+                    o1[n2] = idx;
+                    o2[n1] = idx;
+                    real t1 = inv[n2] * (A[n1][rix(ad[n1], fx + o1[0], fy + o1[1], fz + o1[2])] - A[n1][rix(ad[n1], fx, fy, fz)]);
+                    real t2 = inv[n1] * (A[n2][rix(ad[n2], fx + o2[0], fy + o2[1], fz + o2[2])] - A[n2][rix(ad[n2], fx, fy, fz)]);
+#ifdef ORC_PLANT_CURL_SIGN
+                    /* planted mistake for the test suite: wrong sign of one term */
+                    if (a == 2) t2 = -t2;
+#endif
+                    real K = s->dt * (t1 - t2);
+                    size_t f = rix(s->dim[c], fx, fy, fz);
+                    int hsD = s->msig[c] != NULL;
+                    real sD = 0;
+                    if (hsD) sD = (real)0.5 * (s->dt * s->msig[c][rix(s->gdim[c], ix, iy, iz)]);
+                    int hn = sg[n1] != NULL, hp = sg[n2] != NULL;
+                    real sn = hn ? sig_at(sg[n1], gi[n1]) : 0;
+                    real sp = hp ? sig_at(sg[n2], gi[n2]) : 0;
+                    err |= generic_curl(K, s->C[c], s->U[c], s->T[c], f, hsD, sD, hn, sn, hp, sp);
+                }
+            }
+    if (err) g_error = 1;
+#pragma omp parallel for schedule(static)
+    for (int iz = 1; iz <= N[2]; iz++)
+        for (int iy = 1; iy <= N[1]; iy++)
+            for (int ix = 1; ix <= N[0]; ix++) {
+                int gi[3] = {ix, iy, iz};
+                for (int a = 0; a < 3; a++) {
+                    int c = base + a;
+                    size_t f = rix(s->dim[c], ix + 1, iy + 1, iz + 1);
+                    real m = s->minv[c] ? s->minv[c][rix(s->gdim[c], ix, iy, iz)] : s->minv_s[electric];
+                    /* update_field_generic (src/Kernels/Helpers.jl:323-366) */
+                    real net = s->T[c][f];
+                    real *S = s->sources_active ? s->S[c] : NULL;
+                    if (S) net += S[f];
+                    if (S) S[f] = 0;
+                    int hs = sg[a] != NULL;
+                    real so = hs ? sig_at(sg[a], gi[a]) : 0;
+                    if (s->W[c] && hs && so != (real)0) {
+                        real Wo = s->W[c][f];
+                        s->W[c][f] = m * net;
+                        s->F[c][f] = s->F[c][f] + ((real)1 + so) * s->W[c][f] - ((real)1 - so) * Wo;
+                    } else {
+                        s->F[c][f] = m * net;
+                    }
+                }
+            }
+}
+
+static void update_dfts(orc_sim *s, int magnetic, double t) {
+    /* src/Kernels/Kernels.jl:465-497 gate, src/Monitors/Monitors.jl:361-379 */
+    for (int q = 0; q < s->ndft; q++) {
+        orc_dft *d = &s->dft[q];
+        if ((d->comp >= 3) != magnetic) continue;
+        if (d->decim > 1 && (s->timestep % d->decim) != 0) continue;
+        /* time_fac = complex_backend_number(im * 2 * pi * time) */
+        double theta = (double)(real)(6.283185307179586 * t);
+        double dts = (double)s->dt;
+        const real *F = s->F[d->comp];
+        const int *fd = s->dim[d->comp];
+        size_t ncell = vol(d->dims);
+        for (int k = 0; k < d->nf; k++) {
+            double arg = d->freqs[k] * theta;
+            double wr = dts * cos(arg), wi = dts * sin(arg);
+#pragma omp parallel for schedule(static)
+            for (int iz = 1; iz <= d->dims[2]; iz++)
+                for (int iy = 1; iy <= d->dims[1]; iy++)
+                    for (int ix = 1; ix <= d->dims[0]; ix++) {
+                        size_t ci = (size_t)(ix - 1) + (size_t)d->dims[0] * ((size_t)(iy - 1) + (size_t)d->dims[1] * (size_t)(iz - 1));
+                        size_t ai = ci + ncell * (size_t)k;
+                        double v = (double)F[rix(fd, ix + d->start[0], iy + d->start[1], iz + d->start[2])];
+                        if (d->acc_f64) {
+                            double *acc = (double *)d->acc;
+                            acc[2 * ai] = acc[2 * ai] + wr * v;
+                            acc[2 * ai + 1] = acc[2 * ai + 1] + wi * v;
+                        } else {
+                            real *acc = (real *)d->acc;
+                            acc[2 * ai] = (real)((double)acc[2 * ai] + wr * v);
+                            acc[2 * ai + 1] = (real)((double)acc[2 * ai + 1] + wi * v);
+                        }
+                    }
+        }
+    }
+}
+
+int orc_step(orc_sim *s, int64_t n) {
+    g_error = 0;
+    for (int64_t it = 0; it < n; it++) {
+        double t = round_time(s);
+        if (s->sources_active && t > s->cutoff) s->sources_active = 0;
+        double th = t + (double)(s->dt / (real)2);
+        if (s->sources_active) step_sources(s, 1, t);
+        half_step(s, 0);
+        update_dfts(s, 1, t);
+        if (s->sources_active) step_sources(s, 0, th);
+        half_step(s, 1);
+        update_dfts(s, 0, th);
+        s->timestep++;
+        if (g_error) return -1;
+    }
+    return 0;
+}
+
+int64_t orc_timestep(const orc_sim *s) { return s->timestep; }
+
+/* Copy a full raw array (reference layout, ghosts included). which: 0 field,
+ * 1 T (B/D), 2 U, 3 W, 4 C, 5 S. */
+int orc_read(const orc_sim *s, int which, int comp, real *out) {
+    real *src[6] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp]};
+    size_t n = vol(s->dim[comp]);
+    if (!src[which]) { memset(out, 0, n * sizeof(real)); return 1; }
+    memcpy(out, src[which], n * sizeof(real));
+    return 0;
+}
+
+int orc_write(orc_sim *s, int which, int comp, const real *in) {
+    real *dst[6] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp]};
+    if (!dst[which]) return 1;
+    memcpy(dst[which], in, vol(s->dim[comp]) * sizeof(real));
+    return 0;
+}
+
+void orc_dims(const orc_sim *s, int comp, int *out) {
+    for (int a = 0; a < 3; a++) out[a] = s->dim[comp][a];
+}
+
+/* DFT accumulator as complex128 (dims..., nf) */
+int orc_read_dft(const orc_sim *s, int id, double *out) {
+    const orc_dft *d = &s->dft[id];
+    size_t n = vol(d->dims) * (size_t)d->nf * 2;
+    for (size_t i = 0; i < n; i++)
+        out[i] = d->acc_f64 ? ((double *)d->acc)[i] : (double)((real *)d->acc)[i];
+    return 0;
+}
+
+int orc_real_size(void) { return (int)sizeof(real); }

@@ -1,0 +1,182 @@
+"""Test-side engine that steps a SimSpec with the CPU oracle (oracle/khronos_oracle.c).
+
+Same interface as khronos_hip.engine.HipEngine so that parity tests can run
+one simulation description through both.  The oracle evaluates the source
+waveforms itself (Gaussian / CW closed forms of src/Sources/TimeSources.jl),
+independently of the host code that feeds the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_LIBS = {}
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load(real):
+    key = "f32" if real is np.float32 else "f64"
+    if key in _LIBS:
+        return _LIBS[key]
+    path = os.path.join(ORACLE_DIR, "build", f"liboracle_{key}.so")
+    if not os.path.exists(path):
+        build_oracle()
+    _LIBS[key] = load_path(real, path)
+    return _LIBS[key]
+
+
+def load_path(real, path):
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.orc_create.restype = P
+    lib.orc_create.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [P] * 18 + [C.c_double] * 3
+    lib.orc_destroy.argtypes = [P]
+    lib.orc_add_source.restype = C.c_int
+    lib.orc_add_source.argtypes = [P, C.c_int, P, P, P, C.c_int, P]
+    lib.orc_set_table.argtypes = [P, C.c_int, C.c_int64, P]
+    lib.orc_add_dft.restype = C.c_int
+    lib.orc_add_dft.argtypes = [P, C.c_int, P, P, C.c_int, P, C.c_int, C.c_int]
+    lib.orc_step.restype = C.c_int
+    lib.orc_step.argtypes = [P, C.c_int64]
+    lib.orc_timestep.restype = C.c_int64
+    lib.orc_timestep.argtypes = [P]
+    lib.orc_read.argtypes = [P, C.c_int, C.c_int, P]
+    lib.orc_write.argtypes = [P, C.c_int, C.c_int, P]
+    lib.orc_dims.argtypes = [P, C.c_int, P]
+    lib.orc_read_dft.argtypes = [P, C.c_int, P]
+    lib.orc_real_size.restype = C.c_int
+    assert lib.orc_real_size() == np.dtype(real).itemsize
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleEngine:
+    """Steps a SimSpec on the CPU oracle."""
+
+    def __init__(self, spec, material_sigma=None):
+        from khronos_hip.sources import GaussianPulseData, ContinuousWaveData
+        self.spec = spec
+        R = spec.real
+        self.R = R
+        self.lib = load(R)
+        keep = []
+
+        def arr(a, order="C"):
+            if a is None:
+                return None
+            h = np.asarray(a, dtype=R)
+            h = np.asfortranarray(h) if order == "F" else np.ascontiguousarray(h)
+            keep.append(h)
+            return _p(h)
+
+        sig_b = [None] * 3
+        sig_d = [None] * 3
+        if spec.sigma is not None:
+            sig_b = [arr(spec.sigma["B"][a]) for a in range(3)]
+            sig_d = [arr(spec.sigma["D"][a]) for a in range(3)]
+        minv = [arr(spec.eps_inv[a], "F") for a in range(3)] + [arr(spec.mu_inv[a], "F") for a in range(3)]
+        msig = [None] * 6
+        if material_sigma is not None:
+            msig = [arr(m, "F") for m in material_sigma]
+        self.sim = self.lib.orc_create(*spec.N, *spec.delta, spec.dt, *sig_b, *sig_d, *minv, *msig,
+                                       spec.eps_inv_scalar, spec.mu_inv_scalar, spec.cutoff)
+        from khronos_hip.grid import component_code
+        self.src_ids = []
+        for sd in spec.sources:
+            amp = np.asfortranarray(sd.amplitude)
+            ri = np.empty(amp.size * 2, dtype=R)
+            flat = amp.ravel(order="F")
+            ri[0::2] = flat.real
+            ri[1::2] = flat.imag
+            st = np.array(sd.gv.start_idx, dtype=np.int32)
+            dm = np.array(amp.shape, dtype=np.int32)
+            tp = sd.time_profile
+            if isinstance(tp, GaussianPulseData):
+                kind, p = 0, np.array([tp.fcen, tp.width, tp.peak_time, tp.cutoff_t], dtype=np.float64)
+            elif isinstance(tp, ContinuousWaveData):
+                kind, p = 1, np.array([tp.fcen, 0, 0, 0], dtype=np.float64)
+            else:
+                kind, p = 2, np.zeros(4)
+            sid = self.lib.orc_add_source(self.sim, component_code(sd.component), _p(st), _p(dm), _p(ri), kind, _p(p))
+            self.src_ids.append((sid, kind, sd))
+        self.dft_ids = []
+        for md in spec.dfts:
+            st = np.array(md.gv.start_idx, dtype=np.int32)
+            dm = np.array(md.gv.dims, dtype=np.int32)
+            fr = np.ascontiguousarray(md.frequencies, dtype=np.float64)
+            did = self.lib.orc_add_dft(self.sim, component_code(md.component), _p(st), _p(dm), len(fr), _p(fr),
+                                       int(md.decimation), 1 if spec.dft_accum_f64 else 0)
+            self.dft_ids.append(did)
+        self.ts = 0
+
+    def __del__(self):
+        if getattr(self, "sim", None):
+            self.lib.orc_destroy(self.sim)
+            self.sim = None
+
+    def _tables(self, n_end):
+        R = self.R
+        for sid, kind, sd in self.src_ids:
+            if kind != 2:
+                continue
+            dt = R(self.spec.dt)
+            half = float(dt / R(2))
+            vals = np.empty(2 * n_end)
+            for n in range(n_end):
+                t = float(R(n) * dt) + (half if sd.component.electric else 0.0)
+                v = sd.time_profile.eval(t, R)
+                vals[2 * n], vals[2 * n + 1] = v.real, v.imag
+            self.lib.orc_set_table(self.sim, sid, n_end, _p(vals))
+
+    def step(self, n):
+        if n <= 0:
+            return
+        self._tables(self.ts + n)
+        rc = self.lib.orc_step(self.sim, int(n))
+        if rc != 0:
+            raise RuntimeError("oracle: no reference method for this auxiliary-array combination")
+        self.ts += n
+
+    def sync(self):
+        pass
+
+    def field_shape(self, code):
+        d = np.zeros(3, dtype=np.int32)
+        self.lib.orc_dims(self.sim, code, _p(d))
+        return tuple(int(v) for v in d)
+
+    def read(self, which, code):
+        out = np.zeros(self.field_shape(code), dtype=self.R, order="F")
+        self.lib.orc_read(self.sim, which, code, _p(out))
+        return out
+
+    def read_field(self, code):
+        return self.read(0, code)
+
+    def write(self, which, code, arr):
+        a = np.asfortranarray(arr, dtype=self.R)
+        return self.lib.orc_write(self.sim, which, code, _p(a))
+
+    def write_field(self, code, arr):
+        self.write(0, code, arr)
+
+    def read_dft(self, i):
+        md = self.spec.dfts[i]
+        shp = tuple(md.gv.dims) + (len(md.frequencies),)
+        buf = np.zeros(int(np.prod(shp)) * 2)
+        self.lib.orc_read_dft(self.sim, self.dft_ids[i], _p(buf))
+        return (buf[0::2] + 1j * buf[1::2]).reshape(shp, order="F")
+
+    def dft_norms(self):
+        return [float(np.sqrt(np.sum(np.abs(self.read_dft(i)) ** 2))) for i in range(len(self.dft_ids))]

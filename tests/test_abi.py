@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every symbol include/khronos_hip.h declares.
+
+CPU-only: calls only entry points that do no device work.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from khronos_hip import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "khronos_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(kh_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signature table out of sync with the header"
+
+
+def test_version_and_footprint_without_device():
+    lib = _lib.load()
+    assert lib.kh_version() == 1
+    d = _lib.KhSimDesc()
+    d.nx, d.ny, d.nz = 384, 192, 160
+    d.precision = _lib.KH_F32
+    d.dx = d.dy = d.dz = 0.05
+    d.dt = 0.025
+    n = C.c_size_t()
+    assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == 0
+    cells = 386 * 194 * 162
+    assert 24 * cells * 4 <= n.value <= 32 * cells * 4
+    d.nz = 0
+    assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == -1
+    assert b"extents" in lib.kh_last_error()
