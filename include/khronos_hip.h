@@ -165,10 +165,12 @@ int kh_reset(kh_ctx *ctx);
 /* Kernel timing for measurement: while enabled (enable > 0), every update
  * and DFT launch is bracketed by HIP events on the context's stream; `enable`
  * event pairs are created up front so that timing allocates nothing while it
- * runs.  kh_get_timing returns {total ms of H updates, count, total ms of E
- * updates, count, total ms of DFT launches, count} and clears the counters. */
+ * runs.  kh_get_timing writes {total ms, launch count} for each of the
+ * KH_TIMING_SLOTS kernel classes -- H inner box, H PML/full cascade, E inner
+ * box, E PML/full cascade, DFT -- and clears the counters. */
+#define KH_TIMING_SLOTS 5
 int kh_set_timing(kh_ctx *ctx, int32_t enable);
-int kh_get_timing(kh_ctx *ctx, double out[6]);
+int kh_get_timing(kh_ctx *ctx, double out[2 * KH_TIMING_SLOTS]);
 
 /* The hipStream_t the context launches on (for callers that order their own
  * work after it). */

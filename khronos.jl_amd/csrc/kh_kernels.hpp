@@ -4,21 +4,30 @@
 // (nx+2) x (ny+2) x (nz+2) points indexed by the reference's grid index g
 // (0 and N+1 are the always-zero boundary planes), x fastest, with the x pitch
 // rounded up so that grid index x = 1 of every row is 128-byte aligned.  All
-// twelve-plus arrays share one index, so a cell's E, H, D, B, U, W and
-// material values are at the same offset.
+// arrays share one index, so a cell's E, H, D, B, U, W and material values sit
+// at the same offset.
 //
-// Update kernel (one per half step, all three components):
-//   * a 64 x 4 tile of (x, y) columns per 256-thread block; each thread marches
-//     ZB cells along z, carrying the z-neighbour of the differentiated field in
-//     registers, so every array element streams from HBM once per half step;
-//   * x/y neighbours come from the same or an adjacent tile and hit L1/L2;
-//   * tiles are assigned so that consecutive tiles (adjacent in y) land on the
-//     same XCD and share its L2;
-//   * per cell and component the class is static: cells with a non-zero PML
-//     sigma on any axis, or inside a source box, run the reference's full
-//     cascade (src/Kernels/Helpers.jl:30-139, :323-340) keeping D/B, U, W;
-//     other cells advance E/H directly (FAST) -- or, in EXACT mode, every cell
-//     runs the cascade in the reference CPU path's operation order.
+// A half step (H or E) is split by cell class:
+//
+//   inner_*_kernel  -- the box of cells whose PML sigma is zero on all three
+//     axes (FAST mode).  E/H advance directly (E += eps^-1 * dt * curl H, the
+//     D/B-eliminated form of src/Kernels/CUDAKernels.jl:46-74).  Each lane owns
+//     two adjacent x cells (one 8/16-byte access per array), each thread
+//     marches ZB cells along z carrying the z-neighbour in registers, and the
+//     loads of plane z+1 are issued before the stores of plane z so that the
+//     memory pipeline never drains.  Every array element crosses HBM once.
+//
+//   full_*_kernel   -- every other cell (the PML shell, or the whole grid in
+//     EXACT mode): the reference's sigma cascade keeping D/B, U and W
+//     (src/Kernels/Helpers.jl:113-139, :323-340) in the CPU path's operation
+//     order, with current sources added to the material stage.
+//
+//   source_fix_kernel -- FAST mode, source cells inside the inner box: the
+//     reference computes E = eps^-1 (D + S_n); with D eliminated this is
+//     E += eps^-1 (S_n - S_{n-1}).
+//
+// Tiles are assigned so that consecutive logical tiles (adjacent in y) run on
+// the same XCD and share its L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,10 +35,14 @@
 
 namespace kh {
 
-constexpr int TX = 64;
-constexpr int TY = 4;
+constexpr int TX = 64;       // lanes per tile row
+constexpr int TY = 4;        // rows per tile
 constexpr int MAX_SRC = 8;
 constexpr int NXCD = 8;
+
+template <typename T> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
 
 template <typename T>
 struct SrcL {
@@ -55,19 +68,46 @@ struct HalfArgs {
     int n[3];
     int64_t sy, sz;       // element strides of y and z
     int zb;               // z cells per tile
-    int tiles[3];         // tile counts in x, y, z
+    int tiles[3];         // tile counts in x, y, z of the whole grid
+    const int *tile_list; // tiles to process (tile ids, x fastest); null = all
+    int ntile;
     int per_xcd;          // logical tiles per XCD slot
+    int skip_lo[3];       // cells inside [skip_lo, skip_hi] are left alone
+    int skip_hi[3];
     int nsrc;
     SrcL<T> src[MAX_SRC];
 };
 
 template <typename T>
-__device__ __forceinline__ T source_value(const HalfArgs<T> &p, int comp, int gx, int gy, int gz,
-                                          bool &inside) {
+struct InnerArgs {
+    T *A[3];
+    const T *F[3];
+    const T *minv[3];     // per-voxel inverse material or null (scalar minv_s)
+    T minv_s;
+    T dt;
+    T inv[3];
+    int lo[3], hi[3];     // the inner box, inclusive grid indices
+    int x0;               // grid x of the first tile's first cell (1 + 128k)
+    int64_t sy, sz;
+    int zb;
+    int tiles[3];
+    int per_xcd;
+};
+
+__device__ __forceinline__ bool xcd_tile(int per_xcd, int ntile, int &L) {
+    // Physical block b runs on XCD b % 8: give each XCD a contiguous range of
+    // logical tiles so that y-neighbouring tiles share an L2.
+    int b = blockIdx.x;
+    L = (b % NXCD) * per_xcd + b / NXCD;
+    return L < ntile;
+}
+
+// ----------------------------------------------------------- full cascade
+template <typename T>
+__device__ __forceinline__ T source_value(const HalfArgs<T> &p, int comp, int gx, int gy, int gz) {
     // S = sum over sources of Re(f * A) in registration order
     // (src/Sources/Sources.jl:346-357, accumulated into a zeroed S array).
     T s = T(0);
-    inside = false;
     for (int q = 0; q < p.nsrc; ++q) {
         const SrcL<T> &o = p.src[q];
         if (o.comp != comp) continue;
@@ -77,12 +117,10 @@ __device__ __forceinline__ T source_value(const HalfArgs<T> &p, int comp, int gx
         T ar = o.amp[2 * ai], aim = o.amp[2 * ai + 1];
         T v = (o.fr * ar) - (o.fi * aim);
         s = s + v;
-        inside = true;
     }
     return s;
 }
 
-// Cell class inside any source box of component `comp` (static over the run).
 template <typename T>
 __device__ __forceinline__ bool in_source_box(const HalfArgs<T> &p, int comp, int gx, int gy, int gz) {
     for (int q = 0; q < p.nsrc; ++q) {
@@ -94,17 +132,12 @@ __device__ __forceinline__ bool in_source_box(const HalfArgs<T> &p, int comp, in
     return false;
 }
 
-// One component of one cell.  K is dt * curl in the reference's order.
-template <typename T, bool EXACT>
-__device__ __forceinline__ void update_component(const HalfArgs<T> &p, int a, size_t idx, T K,
-                                                 T so, T sn, T sp, bool pml_cell, int gx, int gy,
-                                                 int gz, bool box_cell) {
+// One component of one cell, reference cascade.  K is dt * curl in the
+// reference's order; so, sn, sp are the own/next/prev-axis sigma samples.
+template <typename T>
+__device__ __forceinline__ void cascade_component(const HalfArgs<T> &p, int a, size_t idx, T K, T so,
+                                                  T sn, T sp, int gx, int gy, int gz) {
     T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
-    if (!EXACT && !pml_cell && !box_cell) {
-        // E/H advanced directly; D/B are not needed in this cell
-        p.A[a][idx] = p.A[a][idx] + m * K;
-        return;
-    }
     // curl stage: C absent, U present (src/Kernels/Helpers.jl:113-139)
     T tq = p.Tq[a][idx];
     T tn;
@@ -119,11 +152,7 @@ __device__ __forceinline__ void update_component(const HalfArgs<T> &p, int a, si
     p.Tq[a][idx] = tn;
     // material stage (src/Kernels/Helpers.jl:323-340)
     T net = tn;
-    if (box_cell && p.nsrc > 0) {
-        bool inside;
-        T s = source_value(p, a, gx, gy, gz, inside);
-        net = net + s;
-    }
+    if (p.nsrc > 0 && in_source_box(p, a, gx, gy, gz)) net = net + source_value(p, a, gx, gy, gz);
     if (so == T(0)) {
         p.A[a][idx] = m * net;
     } else {
@@ -134,26 +163,30 @@ __device__ __forceinline__ void update_component(const HalfArgs<T> &p, int a, si
     }
 }
 
-__device__ __forceinline__ bool logical_tile(int per_xcd, const int tiles[3], int &tx, int &ty, int &tz) {
-    // Physical block b runs on XCD b % 8; give each XCD a contiguous range of
-    // logical tiles so that y-neighbouring tiles share an L2.
-    int b = blockIdx.x;
-    int L = (b % NXCD) * per_xcd + b / NXCD;
-    int ntile = tiles[0] * tiles[1] * tiles[2];
-    if (L >= ntile) return false;
-    tx = L % tiles[0];
-    int r = L / tiles[0];
-    ty = r % tiles[1];
-    tz = r / tiles[1];
+template <typename T>
+__device__ __forceinline__ bool full_tile(const HalfArgs<T> &p, int &tx, int &ty, int &tz) {
+    int L;
+    if (!xcd_tile(p.per_xcd, p.ntile, L)) return false;
+    int id = p.tile_list ? p.tile_list[L] : L;
+    tx = id % p.tiles[0];
+    int r = id / p.tiles[0];
+    ty = r % p.tiles[1];
+    tz = r / p.tiles[1];
     return true;
 }
 
-// H half step: K = dt * (d_n2 E_n1 - d_n1 E_n2) with forward differences
-// (src/Kernels/Helpers.jl:286-298, idx_curl = +1).
-template <typename T, bool EXACT>
-__global__ __launch_bounds__(256) void step_h_kernel(HalfArgs<T> p) {
+template <typename T>
+__device__ __forceinline__ bool skipped(const HalfArgs<T> &p, int gx, int gy, int gz) {
+    return gx >= p.skip_lo[0] && gx <= p.skip_hi[0] && gy >= p.skip_lo[1] && gy <= p.skip_hi[1] &&
+           gz >= p.skip_lo[2] && gz <= p.skip_hi[2];
+}
+
+// H half step, full cascade: K = dt * (d_n2 E_n1 - d_n1 E_n2) with forward
+// differences (src/Kernels/Helpers.jl:286-298, idx_curl = +1).
+template <typename T>
+__global__ __launch_bounds__(256) void full_h_kernel(HalfArgs<T> p) {
     int tx, ty, tz;
-    if (!logical_tile(p.per_xcd, p.tiles, tx, ty, tz)) return;
+    if (!full_tile(p, tx, ty, tz)) return;
     const int gx = tx * TX + threadIdx.x + 1;
     const int gy = ty * TY + threadIdx.y + 1;
     if (gx > p.n[0] || gy > p.n[1]) return;
@@ -162,37 +195,31 @@ __global__ __launch_bounds__(256) void step_h_kernel(HalfArgs<T> p) {
     const int64_t sy = p.sy, sz = p.sz;
     const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
     const T sgx = p.sig[0][gx], sgy = p.sig[1][gy];
-    const bool px = sgx != T(0), py = sgy != T(0);
     size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)z0 * sz;
-    T ex = Ex[idx], ey = Ey[idx];
     for (int gz = z0; gz <= z1; ++gz, idx += sz) {
+        if (skipped(p, gx, gy, gz)) continue;
         const T sgz = p.sig[2][gz];
-        const bool pml = px || py || (sgz != T(0));
+        const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
         const T ex_z1 = Ex[idx + sz], ey_z1 = Ey[idx + sz];
-        const T ez = Ez[idx];
         const T ez_x1 = Ez[idx + 1], ez_y1 = Ez[idx + sy];
         const T ex_y1 = Ex[idx + sy], ey_x1 = Ey[idx + 1];
         // x: d_z Ey - d_y Ez ; y: d_x Ez - d_z Ex ; z: d_y Ex - d_x Ey
         const T Kx = p.dt * (p.inv[2] * (ey_z1 - ey) - p.inv[1] * (ez_y1 - ez));
         const T Ky = p.dt * (p.inv[0] * (ez_x1 - ez) - p.inv[2] * (ex_z1 - ex));
         const T Kz = p.dt * (p.inv[1] * (ex_y1 - ex) - p.inv[0] * (ey_x1 - ey));
-        const bool bx = p.nsrc ? in_source_box(p, 0, gx, gy, gz) : false;
-        const bool by = p.nsrc ? in_source_box(p, 1, gx, gy, gz) : false;
-        const bool bz = p.nsrc ? in_source_box(p, 2, gx, gy, gz) : false;
         // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y)
-        update_component<T, EXACT>(p, 0, idx, Kx, sgx, sgy, sgz, pml, gx, gy, gz, bx);
-        update_component<T, EXACT>(p, 1, idx, Ky, sgy, sgz, sgx, pml, gx, gy, gz, by);
-        update_component<T, EXACT>(p, 2, idx, Kz, sgz, sgx, sgy, pml, gx, gy, gz, bz);
-        ex = ex_z1;
-        ey = ey_z1;
+        cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
+        cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
+        cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
     }
 }
 
-// E half step: backward differences (idx_curl = -1): d(A) = inv * (A[i-1] - A[i]).
-template <typename T, bool EXACT>
-__global__ __launch_bounds__(256) void step_e_kernel(HalfArgs<T> p) {
+// E half step, full cascade: backward differences (idx_curl = -1),
+// d(A) = inv * (A[i-1] - A[i]).
+template <typename T>
+__global__ __launch_bounds__(256) void full_e_kernel(HalfArgs<T> p) {
     int tx, ty, tz;
-    if (!logical_tile(p.per_xcd, p.tiles, tx, ty, tz)) return;
+    if (!full_tile(p, tx, ty, tz)) return;
     const int gx = tx * TX + threadIdx.x + 1;
     const int gy = ty * TY + threadIdx.y + 1;
     if (gx > p.n[0] || gy > p.n[1]) return;
@@ -201,30 +228,214 @@ __global__ __launch_bounds__(256) void step_e_kernel(HalfArgs<T> p) {
     const int64_t sy = p.sy, sz = p.sz;
     const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
     const T sgx = p.sig[0][gx], sgy = p.sig[1][gy];
-    const bool px = sgx != T(0), py = sgy != T(0);
     size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)z0 * sz;
-    T hx_zm = Hx[idx - sz], hy_zm = Hy[idx - sz];
     for (int gz = z0; gz <= z1; ++gz, idx += sz) {
+        if (skipped(p, gx, gy, gz)) continue;
         const T sgz = p.sig[2][gz];
-        const bool pml = px || py || (sgz != T(0));
         const T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
+        const T hx_zm = Hx[idx - sz], hy_zm = Hy[idx - sz];
         const T hz_xm = Hz[idx - 1], hz_ym = Hz[idx - sy];
         const T hx_ym = Hx[idx - sy], hy_xm = Hy[idx - 1];
         // x: d_z Hy - d_y Hz ; y: d_x Hz - d_z Hx ; z: d_y Hx - d_x Hy
         const T Kx = p.dt * (p.inv[2] * (hy_zm - hy) - p.inv[1] * (hz_ym - hz));
         const T Ky = p.dt * (p.inv[0] * (hz_xm - hz) - p.inv[2] * (hx_zm - hx));
         const T Kz = p.dt * (p.inv[1] * (hx_ym - hx) - p.inv[0] * (hy_xm - hy));
-        const bool bx = p.nsrc ? in_source_box(p, 0, gx, gy, gz) : false;
-        const bool by = p.nsrc ? in_source_box(p, 1, gx, gy, gz) : false;
-        const bool bz = p.nsrc ? in_source_box(p, 2, gx, gy, gz) : false;
-        update_component<T, EXACT>(p, 0, idx, Kx, sgx, sgy, sgz, pml, gx, gy, gz, bx);
-        update_component<T, EXACT>(p, 1, idx, Ky, sgy, sgz, sgx, pml, gx, gy, gz, by);
-        update_component<T, EXACT>(p, 2, idx, Kz, sgz, sgx, sgy, pml, gx, gy, gz, bz);
-        hx_zm = hx;
-        hy_zm = hy;
+        cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
+        cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
+        cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
     }
 }
 
+// ----------------------------------------------------------- inner box
+template <typename T>
+__device__ __forceinline__ typename V2<T>::t ld2(const T *p, size_t i) {
+    return *reinterpret_cast<const typename V2<T>::t *>(p + i);
+}
+
+template <typename T>
+__device__ __forceinline__ void st2(T *p, size_t i, typename V2<T>::t v) {
+    *reinterpret_cast<typename V2<T>::t *>(p + i) = v;
+}
+
+template <typename T>
+__device__ __forceinline__ bool inner_lane(const InnerArgs<T> &p, int &x, int &y, int &z0, int &z1,
+                                           bool &v0, bool &v1) {
+    int L;
+    const int ntile = p.tiles[0] * p.tiles[1] * p.tiles[2];
+    if (!xcd_tile(p.per_xcd, ntile, L)) return false;
+    int tx = L % p.tiles[0];
+    int r = L / p.tiles[0];
+    int ty = r % p.tiles[1];
+    int tz = r / p.tiles[1];
+    x = p.x0 + tx * (2 * TX) + 2 * threadIdx.x;
+    y = p.lo[1] + ty * TY + threadIdx.y;
+    if (y > p.hi[1]) return false;
+    v0 = x >= p.lo[0] && x <= p.hi[0];
+    v1 = x + 1 >= p.lo[0] && x + 1 <= p.hi[0];
+    if (!v0 && !v1) return false;
+    z0 = p.lo[2] + tz * p.zb;
+    z1 = min(z0 + p.zb - 1, p.hi[2]);
+    return true;
+}
+
+// H half step on the inner box.  Lane owns cells (x, x+1).
+template <typename T, bool PERVOX>
+__global__ __launch_bounds__(256) void inner_h_kernel(InnerArgs<T> p) {
+    using V = typename V2<T>::t;
+    int x, y, z0, z1;
+    bool v0, v1;
+    if (!inner_lane(p, x, y, z0, z1, v0, v1)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
+    T *Hx = p.A[0], *Hy = p.A[1], *Hz = p.A[2];
+    const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
+    size_t idx = (size_t)x + (size_t)y * sy + (size_t)z0 * sz;
+
+    struct Plane {
+        V ex1, ey1, ez, ez_y1, ex_y1, hx, hy, hz, mx, my, mz;
+        T ez_x2, ey_x2;
+    };
+    auto load = [&](size_t i) {
+        Plane q;
+        q.ex1 = ld2(Ex, i + sz);
+        q.ey1 = ld2(Ey, i + sz);
+        q.ez = ld2(Ez, i);
+        q.ez_x2 = Ez[i + 2];
+        q.ey_x2 = Ey[i + 2];
+        q.ez_y1 = ld2(Ez, i + sy);
+        q.ex_y1 = ld2(Ex, i + sy);
+        q.hx = ld2((const T *)Hx, i);
+        q.hy = ld2((const T *)Hy, i);
+        q.hz = ld2((const T *)Hz, i);
+        if (PERVOX) {
+            q.mx = ld2(p.minv[0], i);
+            q.my = ld2(p.minv[1], i);
+            q.mz = ld2(p.minv[2], i);
+        }
+        return q;
+    };
+    V ex = ld2(Ex, idx), ey = ld2(Ey, idx);
+    Plane cur = load(idx);
+    for (int z = z0; z <= z1; ++z, idx += sz) {
+        Plane nxt;
+        if (z < z1) nxt = load(idx + sz);
+        const T m0x = PERVOX ? cur.mx.x : p.minv_s, m1x = PERVOX ? cur.mx.y : p.minv_s;
+        const T m0y = PERVOX ? cur.my.x : p.minv_s, m1y = PERVOX ? cur.my.y : p.minv_s;
+        const T m0z = PERVOX ? cur.mz.x : p.minv_s, m1z = PERVOX ? cur.mz.y : p.minv_s;
+        // cell x (element .x): x+1 neighbours are element .y; cell x+1 uses x+2
+        const T Kx0 = dt * (iz * (cur.ey1.x - ey.x) - iy * (cur.ez_y1.x - cur.ez.x));
+        const T Kx1 = dt * (iz * (cur.ey1.y - ey.y) - iy * (cur.ez_y1.y - cur.ez.y));
+        const T Ky0 = dt * (ix * (cur.ez.y - cur.ez.x) - iz * (cur.ex1.x - ex.x));
+        const T Ky1 = dt * (ix * (cur.ez_x2 - cur.ez.y) - iz * (cur.ex1.y - ex.y));
+        const T Kz0 = dt * (iy * (cur.ex_y1.x - ex.x) - ix * (ey.y - ey.x));
+        const T Kz1 = dt * (iy * (cur.ex_y1.y - ex.y) - ix * (cur.ey_x2 - ey.y));
+        V nhx, nhy, nhz;
+        nhx.x = v0 ? cur.hx.x + m0x * Kx0 : cur.hx.x;
+        nhx.y = v1 ? cur.hx.y + m1x * Kx1 : cur.hx.y;
+        nhy.x = v0 ? cur.hy.x + m0y * Ky0 : cur.hy.x;
+        nhy.y = v1 ? cur.hy.y + m1y * Ky1 : cur.hy.y;
+        nhz.x = v0 ? cur.hz.x + m0z * Kz0 : cur.hz.x;
+        nhz.y = v1 ? cur.hz.y + m1z * Kz1 : cur.hz.y;
+        st2(Hx, idx, nhx);
+        st2(Hy, idx, nhy);
+        st2(Hz, idx, nhz);
+        ex = cur.ex1;
+        ey = cur.ey1;
+        cur = nxt;
+    }
+}
+
+// E half step on the inner box.  Lane owns cells (x, x+1).
+template <typename T, bool PERVOX>
+__global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
+    using V = typename V2<T>::t;
+    int x, y, z0, z1;
+    bool v0, v1;
+    if (!inner_lane(p, x, y, z0, z1, v0, v1)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
+    T *Ex = p.A[0], *Ey = p.A[1], *Ez = p.A[2];
+    const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
+    size_t idx = (size_t)x + (size_t)y * sy + (size_t)z0 * sz;
+
+    struct Plane {
+        V hx, hy, hz, hz_ym, hx_ym, ex, ey, ez, mx, my, mz;
+        T hz_xm, hy_xm;
+    };
+    auto load = [&](size_t i) {
+        Plane q;
+        q.hx = ld2(Hx, i);
+        q.hy = ld2(Hy, i);
+        q.hz = ld2(Hz, i);
+        q.hz_xm = Hz[i - 1];
+        q.hy_xm = Hy[i - 1];
+        q.hz_ym = ld2(Hz, i - sy);
+        q.hx_ym = ld2(Hx, i - sy);
+        q.ex = ld2((const T *)Ex, i);
+        q.ey = ld2((const T *)Ey, i);
+        q.ez = ld2((const T *)Ez, i);
+        if (PERVOX) {
+            q.mx = ld2(p.minv[0], i);
+            q.my = ld2(p.minv[1], i);
+            q.mz = ld2(p.minv[2], i);
+        }
+        return q;
+    };
+    V hxm = ld2(Hx, idx - sz), hym = ld2(Hy, idx - sz);
+    Plane cur = load(idx);
+    for (int z = z0; z <= z1; ++z, idx += sz) {
+        Plane nxt;
+        if (z < z1) nxt = load(idx + sz);
+        const T m0x = PERVOX ? cur.mx.x : p.minv_s, m1x = PERVOX ? cur.mx.y : p.minv_s;
+        const T m0y = PERVOX ? cur.my.x : p.minv_s, m1y = PERVOX ? cur.my.y : p.minv_s;
+        const T m0z = PERVOX ? cur.mz.x : p.minv_s, m1z = PERVOX ? cur.mz.y : p.minv_s;
+        // cell x (element .x): x-1 neighbour is the scalar; cell x+1 uses element .x
+        const T Kx0 = dt * (iz * (hym.x - cur.hy.x) - iy * (cur.hz_ym.x - cur.hz.x));
+        const T Kx1 = dt * (iz * (hym.y - cur.hy.y) - iy * (cur.hz_ym.y - cur.hz.y));
+        const T Ky0 = dt * (ix * (cur.hz_xm - cur.hz.x) - iz * (hxm.x - cur.hx.x));
+        const T Ky1 = dt * (ix * (cur.hz.x - cur.hz.y) - iz * (hxm.y - cur.hx.y));
+        const T Kz0 = dt * (iy * (cur.hx_ym.x - cur.hx.x) - ix * (cur.hy_xm - cur.hy.x));
+        const T Kz1 = dt * (iy * (cur.hx_ym.y - cur.hx.y) - ix * (cur.hy.x - cur.hy.y));
+        V nex, ney, nez;
+        nex.x = v0 ? cur.ex.x + m0x * Kx0 : cur.ex.x;
+        nex.y = v1 ? cur.ex.y + m1x * Kx1 : cur.ex.y;
+        ney.x = v0 ? cur.ey.x + m0y * Ky0 : cur.ey.x;
+        ney.y = v1 ? cur.ey.y + m1y * Ky1 : cur.ey.y;
+        nez.x = v0 ? cur.ez.x + m0z * Kz0 : cur.ez.x;
+        nez.y = v1 ? cur.ez.y + m1z * Kz1 : cur.ez.y;
+        st2(Ex, idx, nex);
+        st2(Ey, idx, ney);
+        st2(Ez, idx, nez);
+        hxm = cur.hx;
+        hym = cur.hy;
+        cur = nxt;
+    }
+}
+
+// FAST mode, source cells inside the inner box: A += m * (S_n - S_{n-1}),
+// S = Re(f * amp) for one source (launched once per source, in order).
+template <typename T>
+__global__ __launch_bounds__(256) void source_fix_kernel(T *A, const T *minv, T minv_s, const T *amp, int s0,
+                                                         int s1, int s2, int d0, int d1, int d2, T fr, T fi,
+                                                         T pr, T pi, int lo0, int lo1, int lo2, int hi0,
+                                                         int hi1, int hi2, int64_t sy, int64_t sz) {
+    const int64_t n = (int64_t)d0 * d1 * d2;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int ix = (int)(c % d0);
+    const int64_t r = c / d0;
+    const int iy = (int)(r % d1), iz = (int)(r / d1);
+    const int gx = s0 + ix, gy = s1 + iy, gz = s2 + iz;
+    if (gx < lo0 || gx > hi0 || gy < lo1 || gy > hi1 || gz < lo2 || gz > hi2) return;
+    const T ar = amp[2 * c], ai = amp[2 * c + 1];
+    const T vn = (fr * ar) - (fi * ai);
+    const T vo = (pr * ar) - (pi * ai);
+    const size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const T m = minv ? minv[idx] : minv_s;
+    A[idx] = A[idx] + m * (vn - vo);
+}
+
+// ----------------------------------------------------------- monitors
 // DFT accumulation of one monitor (src/Monitors/Monitors.jl:361-379):
 // acc[cell, k] += (dt * exp(i * f_k * theta)) * field[cell].  The phase
 // weights are formed once per block in float64 and kept in LDS; each

@@ -7,7 +7,9 @@
 // then due E-component DFT launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -60,7 +62,8 @@ void require(bool c, const std::string &m, int code = KH_E_INVALID) {
     if (!c) throw KhError(code, m);
 }
 
-constexpr int ALIGN = 32;  // elements; x index 1 of every row is aligned
+constexpr int ALIGN = 32;
+constexpr int kh_timing_slots = KH_TIMING_SLOTS;  // elements; x index 1 of every row is aligned
 
 struct Grid {
     int n[3];
@@ -174,8 +177,8 @@ struct Ctx final : CtxBase {
         e.kind = kind;
         return e;
     }
-    double tsum[3] = {0, 0, 0};
-    double tcnt[3] = {0, 0, 0};
+    double tsum[kh_timing_slots] = {};
+    double tcnt[kh_timing_slots] = {};
 
     void *dalloc(size_t bytes) {
         void *p = nullptr;
@@ -306,68 +309,218 @@ struct Ctx final : CtxBase {
 
     double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22
 
-    kh::HalfArgs<T> base_args(bool electric) const {
+    // ------------------------------------------------------------ launch plans
+    // Per half step (0 = H, 1 = E): the inner box (sigma zero on all axes), the
+    // tile list of the full cascade kernel, and launch shapes.  Built once, on
+    // the first step.
+    struct Plan {
+        int lo[3], hi[3];
+        bool inner = false;
+        kh::InnerArgs<T> ia{};
+        int inner_blocks = 0;
+        int *tiles = nullptr;  // device tile list for the full kernel
+        int ntile = 0;
+        int zb = 16;
+        int gtiles[3];
+    };
+    Plan plan[2];
+    bool planned = false;
+    bool last_active = false;  // sources injected on the previous step
+
+    static int env_int(const char *name, int def) {
+        const char *v = getenv(name);
+        return v && *v ? atoi(v) : def;
+    }
+
+    static void zero_run(const std::vector<T> &s, int N, int &lo, int &hi) {
+        // longest run of zero samples among grid indices 1..N
+        lo = 1; hi = 0;
+        int best = 0;
+        for (int i = 1; i <= N;) {
+            if (s[i] != T(0)) { ++i; continue; }
+            int j = i;
+            while (j + 1 <= N && s[j + 1] == T(0)) ++j;
+            if (j - i + 1 > best) { best = j - i + 1; lo = i; hi = j; }
+            i = j + 1;
+        }
+    }
+
+    void make_plans() {
+        const int zb_full = std::max(1, std::min(env_int("KH_ZB_FULL", 16), g.n[2]));
+        const int zb_inner = std::max(1, env_int("KH_ZB_INNER", 8));
+        for (int e = 0; e < 2; ++e) {
+            Plan &P = plan[e];
+            // sampled sigma back on the host
+            bool any_empty = false;
+            for (int a = 0; a < 3; ++a) {
+                std::vector<T> h((size_t)g.n[a] + 2);
+                // all transfers go through the context stream: it is non-blocking, so
+                // the legacy null stream would not be ordered after its memsets
+                HIPCHK(hipMemcpyAsync(h.data(), e ? sigD[a] : sigB[a], h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                zero_run(h, g.n[a], P.lo[a], P.hi[a]);
+                any_empty |= P.lo[a] > P.hi[a];
+            }
+            P.inner = (mode == KH_MODE_FAST) && !any_empty;
+            if (!P.inner)
+                for (int a = 0; a < 3; ++a) { P.lo[a] = 1; P.hi[a] = 0; }
+            // full-kernel tiles: every tile with a cell outside the inner box
+            P.zb = zb_full;
+            P.gtiles[0] = (g.n[0] + kh::TX - 1) / kh::TX;
+            P.gtiles[1] = (g.n[1] + kh::TY - 1) / kh::TY;
+            P.gtiles[2] = (g.n[2] + P.zb - 1) / P.zb;
+            std::vector<int> list;
+            for (int tz = 0; tz < P.gtiles[2]; ++tz)
+                for (int ty = 0; ty < P.gtiles[1]; ++ty)
+                    for (int tx = 0; tx < P.gtiles[0]; ++tx) {
+                        int c0[3] = {tx * kh::TX + 1, ty * kh::TY + 1, tz * P.zb + 1};
+                        int c1[3] = {std::min(c0[0] + kh::TX - 1, g.n[0]), std::min(c0[1] + kh::TY - 1, g.n[1]),
+                                     std::min(c0[2] + P.zb - 1, g.n[2])};
+                        bool inside = P.inner;
+                        for (int a = 0; a < 3 && inside; ++a) inside = c0[a] >= P.lo[a] && c1[a] <= P.hi[a];
+                        if (!inside) list.push_back(tx + P.gtiles[0] * (ty + P.gtiles[1] * tz));
+                    }
+            P.ntile = (int)list.size();
+            if (P.ntile) {
+                P.tiles = (int *)dalloc(list.size() * sizeof(int));
+                HIPCHK(hipMemcpyAsync(P.tiles, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+            if (P.inner) {
+                kh::InnerArgs<T> &ia = P.ia;
+                for (int a = 0; a < 3; ++a) {
+                    ia.A[a] = e ? E[a] : H[a];
+                    ia.F[a] = e ? H[a] : E[a];
+                    ia.minv[a] = e ? epsi[a] : mui[a];
+                    ia.inv[a] = T(1) / dx[a];
+                    ia.lo[a] = P.lo[a];
+                    ia.hi[a] = P.hi[a];
+                }
+                ia.minv_s = e ? eps_s : mu_s;
+                ia.dt = dt;
+                ia.sy = g.sy();
+                ia.sz = g.sz();
+                ia.zb = std::min(zb_inner, P.hi[2] - P.lo[2] + 1);
+                int kx0 = (P.lo[0] - 1) / (2 * kh::TX), kx1 = (P.hi[0] - 1) / (2 * kh::TX);
+                ia.x0 = 1 + 2 * kh::TX * kx0;
+                ia.tiles[0] = kx1 - kx0 + 1;
+                ia.tiles[1] = (P.hi[1] - P.lo[1] + 1 + kh::TY - 1) / kh::TY;
+                ia.tiles[2] = (P.hi[2] - P.lo[2] + 1 + ia.zb - 1) / ia.zb;
+                int nt = ia.tiles[0] * ia.tiles[1] * ia.tiles[2];
+                ia.per_xcd = (nt + kh::NXCD - 1) / kh::NXCD;
+                P.inner_blocks = ia.per_xcd * kh::NXCD;
+                bool pervox = (e ? epsi[0] : mui[0]) != nullptr;
+                for (int a = 1; a < 3; ++a)
+                    require(((e ? epsi[a] : mui[a]) != nullptr) == pervox,
+                            "per-voxel materials must be given for all three components or none");
+            }
+        }
+        planned = true;
+    }
+
+    kh::HalfArgs<T> full_args(int e) const {
+        const Plan &P = plan[e];
         kh::HalfArgs<T> p{};
         for (int a = 0; a < 3; ++a) {
-            p.A[a] = electric ? E[a] : H[a];
-            p.F[a] = electric ? H[a] : E[a];
-            p.Tq[a] = electric ? D[a] : B[a];
-            p.U[a] = electric ? UD[a] : UB[a];
-            p.W[a] = electric ? WD[a] : WB[a];
-            p.sig[a] = electric ? sigD[a] : sigB[a];
-            p.minv[a] = electric ? epsi[a] : mui[a];
+            p.A[a] = e ? E[a] : H[a];
+            p.F[a] = e ? H[a] : E[a];
+            p.Tq[a] = e ? D[a] : B[a];
+            p.U[a] = e ? UD[a] : UB[a];
+            p.W[a] = e ? WD[a] : WB[a];
+            p.sig[a] = e ? sigD[a] : sigB[a];
+            p.minv[a] = e ? epsi[a] : mui[a];
             p.inv[a] = T(1) / dx[a];
             p.n[a] = g.n[a];
+            p.tiles[a] = P.gtiles[a];
+            p.skip_lo[a] = P.lo[a];
+            p.skip_hi[a] = P.hi[a];
         }
-        p.minv_s = electric ? eps_s : mu_s;
+        p.minv_s = e ? eps_s : mu_s;
         p.dt = dt;
         p.sy = g.sy();
         p.sz = g.sz();
-        p.zb = std::min(16, g.n[2]);
-        p.tiles[0] = (g.n[0] + kh::TX - 1) / kh::TX;
-        p.tiles[1] = (g.n[1] + kh::TY - 1) / kh::TY;
-        p.tiles[2] = (g.n[2] + p.zb - 1) / p.zb;
-        int nt = p.tiles[0] * p.tiles[1] * p.tiles[2];
-        p.per_xcd = (nt + kh::NXCD - 1) / kh::NXCD;
+        p.zb = P.zb;
+        p.tile_list = P.tiles;
+        p.ntile = P.ntile;
+        p.per_xcd = (P.ntile + kh::NXCD - 1) / kh::NXCD;
         return p;
     }
 
-    void launch_half(bool electric, double t) {
-        kh::HalfArgs<T> p = base_args(electric);
-        p.nsrc = 0;
-        // source boxes always define the cell class; values only while active
-        for (auto &o : srcs) {
-            if ((o.comp < 3) != electric) continue;
-            kh::SrcL<T> &l = p.src[p.nsrc++];
-            l.comp = o.comp % 3;
-            for (int a = 0; a < 3; ++a) { l.s[a] = o.s[a]; l.d[a] = o.d[a]; }
-            l.amp = o.amp;
-            l.fr = T(0);
-            l.fi = T(0);
-            if (active && ts < (int64_t)(o.wave.size() / 2)) {
-                l.fr = (T)o.wave[2 * ts];
-                l.fi = (T)o.wave[2 * ts + 1];
-            }
-        }
-        (void)t;
-        int nblk = p.per_xcd * kh::NXCD;
+    bool wave_at(const Source &o, int64_t n, T &fr, T &fi) const {
+        fr = T(0);
+        fi = T(0);
+        if (n < 0 || n >= (int64_t)(o.wave.size() / 2)) return false;
+        fr = (T)o.wave[2 * n];
+        fi = (T)o.wave[2 * n + 1];
+        return true;
+    }
+
+    void launch_half(int e) {
+        const Plan &P = plan[e];
+        const bool electric = e == 1;
         dim3 blk(kh::TX, kh::TY, 1);
         EvPair ev{};
-        if (timing) {
-            ev = take_event(electric ? 1 : 0);
-            HIPCHK(hipEventRecord(ev.a, st));
+        if (P.inner) {
+            if (timing) {
+                ev = take_event(2 * e);
+                HIPCHK(hipEventRecord(ev.a, st));
+            }
+            bool pervox = P.ia.minv[0] != nullptr;
+            if (electric) {
+                if (pervox) hipLaunchKernelGGL((kh::inner_e_kernel<T, true>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+                else hipLaunchKernelGGL((kh::inner_e_kernel<T, false>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+            } else {
+                if (pervox) hipLaunchKernelGGL((kh::inner_h_kernel<T, true>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+                else hipLaunchKernelGGL((kh::inner_h_kernel<T, false>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+            }
+            HIPCHK(hipGetLastError());
+            if (timing) {
+                HIPCHK(hipEventRecord(ev.b, st));
+                evs.push_back(ev);
+            }
         }
-        if (mode == KH_MODE_EXACT) {
-            if (electric) hipLaunchKernelGGL((kh::step_e_kernel<T, true>), dim3(nblk), blk, 0, st, p);
-            else hipLaunchKernelGGL((kh::step_h_kernel<T, true>), dim3(nblk), blk, 0, st, p);
-        } else {
-            if (electric) hipLaunchKernelGGL((kh::step_e_kernel<T, false>), dim3(nblk), blk, 0, st, p);
-            else hipLaunchKernelGGL((kh::step_h_kernel<T, false>), dim3(nblk), blk, 0, st, p);
+        if (P.ntile) {
+            if (timing) {
+                ev = take_event(2 * e + 1);
+                HIPCHK(hipEventRecord(ev.a, st));
+            }
+            kh::HalfArgs<T> p = full_args(e);
+            p.nsrc = 0;
+            for (auto &o : srcs) {
+                if ((o.comp < 3) != electric) continue;
+                kh::SrcL<T> &l = p.src[p.nsrc++];
+                l.comp = o.comp % 3;
+                for (int a = 0; a < 3; ++a) { l.s[a] = o.s[a]; l.d[a] = o.d[a]; }
+                l.amp = o.amp;
+                if (active) wave_at(o, ts, l.fr, l.fi);
+                else { l.fr = T(0); l.fi = T(0); }
+            }
+            if (electric) hipLaunchKernelGGL((kh::full_e_kernel<T>), dim3(p.per_xcd * kh::NXCD), blk, 0, st, p);
+            else hipLaunchKernelGGL((kh::full_h_kernel<T>), dim3(p.per_xcd * kh::NXCD), blk, 0, st, p);
+            HIPCHK(hipGetLastError());
+            if (timing) {
+                HIPCHK(hipEventRecord(ev.b, st));
+                evs.push_back(ev);
+            }
         }
-        HIPCHK(hipGetLastError());
-        if (timing) {
-            HIPCHK(hipEventRecord(ev.b, st));
-            evs.push_back(ev);
+        // FAST mode: sources inside the inner box, E += m (S_n - S_{n-1})
+        if (P.inner && (active || last_active)) {
+            for (auto &o : srcs) {
+                if ((o.comp < 3) != electric) continue;
+                T fr = 0, fi = 0, pr = 0, pi = 0;
+                if (active) wave_at(o, ts, fr, fi);
+                if (last_active) wave_at(o, ts - 1, pr, pi);
+                int a = o.comp % 3;
+                int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
+                unsigned nb = (unsigned)((n + 255) / 256);
+                T *A = electric ? E[a] : H[a];
+                const T *mi = electric ? epsi[a] : mui[a];
+                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, mi,
+                                   electric ? eps_s : mu_s, (const T *)o.amp, o.s[0], o.s[1], o.s[2], o.d[0],
+                                   o.d[1], o.d[2], fr, fi, pr, pi, P.lo[0], P.lo[1], P.lo[2], P.hi[0],
+                                   P.hi[1], P.hi[2], g.sy(), g.sz());
+                HIPCHK(hipGetLastError());
+            }
         }
     }
 
@@ -383,7 +536,7 @@ struct Ctx final : CtxBase {
             size_t shm = (size_t)m.nf * 2 * sizeof(double);
             EvPair ev{};
             if (timing) {
-                ev = take_event(2);
+                ev = take_event(4);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
             if (acc64)
@@ -404,15 +557,17 @@ struct Ctx final : CtxBase {
 
     void step(int64_t n) override {
         require(n >= 0, "n_steps must be >= 0");
+        if (!planned) make_plans();
         stepped = true;
         for (int64_t it = 0; it < n; ++it) {
             double t = round_time(ts);
             if (active && t > cutoff) active = false;  // src/Kernels/Kernels.jl:27-35
             double th = t + (double)(dt / T(2));
-            launch_half(false, t);
+            launch_half(0);
             launch_dfts(false, t);
-            launch_half(true, th);
+            launch_half(1);
             launch_dfts(true, th);
+            last_active = active;
             ++ts;
         }
     }
@@ -541,6 +696,7 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamSynchronize(st));
         ts = 0;
         active = true;
+        last_active = false;
     }
 
     void set_timing(int n) override {
@@ -563,7 +719,7 @@ struct Ctx final : CtxBase {
             pool.push_back(e);
         }
         evs.clear();
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < kh_timing_slots; ++k) {
             out[2 * k] = tsum[k];
             out[2 * k + 1] = tcnt[k];
             tsum[k] = 0;
@@ -745,7 +901,7 @@ int kh_set_timing(kh_ctx *ctx, int32_t enable) {
     });
 }
 
-int kh_get_timing(kh_ctx *ctx, double out[6]) {
+int kh_get_timing(kh_ctx *ctx, double out[2 * KH_TIMING_SLOTS]) {
     return guarded([&] {
         CTX_OK;
         require(out != nullptr, "null argument");

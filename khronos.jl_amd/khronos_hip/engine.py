@@ -188,11 +188,40 @@ class HipEngine:
     def set_timing(self, on: bool):
         _lib.check(self.lib.kh_set_timing(self.ctx, 1 if on else 0))
 
+    TIMING_SLOTS = ("h_inner", "h_full", "e_inner", "e_full", "dft")
+
     def get_timing(self):
-        out = np.zeros(6, dtype=np.float64)
+        """{slot: (total_ms, launches)} for the kernel classes of include/khronos_hip.h."""
+        out = np.zeros(2 * len(self.TIMING_SLOTS), dtype=np.float64)
         _lib.check(self.lib.kh_get_timing(self.ctx, _ptr(out)))
-        return {"h_ms": out[0], "h_n": int(out[1]), "e_ms": out[2], "e_n": int(out[3]),
-                "dft_ms": out[4], "dft_n": int(out[5])}
+        return {k: (float(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(self.TIMING_SLOTS)}
+
+    def plan_boxes(self):
+        """Inner boxes (sigma zero on all axes) of the H and E half steps, as the
+        library computes them: [(lo, hi) for H, (lo, hi) for E], grid indices."""
+        out = []
+        for key in ("B", "D"):
+            lohi = []
+            for a in range(3):
+                n = self.spec.N[a]
+                if self.spec.sigma is None:
+                    lohi.append((1, n))
+                    continue
+                z = np.asarray(self.spec.sigma[key][a])[0:2 * n:2] == 0
+                best, lo, hi, i = 0, 1, 0, 0
+                while i < n:
+                    if not z[i]:
+                        i += 1
+                        continue
+                    j = i
+                    while j + 1 < n and z[j + 1]:
+                        j += 1
+                    if j - i + 1 > best:
+                        best, lo, hi = j - i + 1, i + 1, j + 1
+                    i = j + 1
+                lohi.append((lo, hi))
+            out.append(([v[0] for v in lohi], [v[1] for v in lohi]))
+        return out
 
 
 def _code(component) -> int:

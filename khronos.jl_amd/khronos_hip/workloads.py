@@ -75,4 +75,6 @@ def algorithmic_bytes(spec) -> dict:
             acc += 3 * sum(counts)
         out[kind] = acc * s
         out[kind + "_counts"] = counts
+        # the inner-box kernel alone: p = 0 cells, 9 accesses (+3 per-voxel)
+        out[kind + "_inner"] = (9 + (3 if any(m is not None for m in minv) else 0)) * counts[0] * s
     return out

@@ -29,10 +29,23 @@ def _note(msg):
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def field_errors(sa, sb):
+    """rel-L2 of the whole field vector (all six components), and each
+    component's error normalised by that same global norm.  A component that is
+    zero by symmetry (Hz of a z-directed dipole) holds round-off on both sides,
+    so its own norm is no yardstick."""
+    fa = [sa.read_field(C()).astype(np.float64) for C in COMPS]
+    fb = [sb.read_field(C()).astype(np.float64) for C in COMPS]
+    nb = math.sqrt(sum(float(np.sum(b * b)) for b in fb))
+    tot = math.sqrt(sum(float(np.sum((a - b) ** 2)) for a, b in zip(fa, fb))) / nb
+    per = {C.__name__: float(np.linalg.norm(a - b)) / nb for C, a, b in zip(COMPS, fa, fb)}
+    return tot, per
 
 
 def _box_sim(precision, mode, engine=None, n=(24, 20, 18), pml=0.3, res=10, steps_monitor=True,
@@ -80,9 +93,8 @@ def test_exact_mode_matches_oracle_value_for_value(precision):
 @pytest.mark.parametrize("precision,tol,dtol", [("f64", 1e-12, 1e-10), ("f32", 1e-5, 1e-4)])
 def test_fast_mode_matches_oracle(precision, tol, dtol):
     g, o = _run_pair(precision, "fast", 150)
-    for C in COMPS:
-        e = rel_l2(g.read_field(C()), o.read_field(C()))
-        assert e < tol, (C.__name__, e)
+    tot, per = field_errors(g, o)
+    assert tot < tol and max(per.values()) < tol, (tot, per)
     for i, md in enumerate(g.spec.dfts):
         assert rel_l2(g.engine.read_dft(i), o.engine.read_dft(i)) < dtol
 
@@ -117,9 +129,8 @@ def test_config1_dipole_f64_500_steps():
     _note("[parity] config1 80^3 f64 500 steps")
     g.step(500)
     o.step(500)
-    for C in COMPS:
-        e = rel_l2(g.read_field(C()), o.read_field(C()))
-        assert e < 1e-12, (C.__name__, e)
+    tot, per = field_errors(g, o)
+    assert tot < 1e-12 and max(per.values()) < 1e-12, (tot, per)
     assert rel_l2(g.engine.read_dft(0), o.engine.read_dft(0)) < 1e-10
 
 
@@ -211,9 +222,12 @@ def test_config2_full_size_fast_vs_exact_and_linearity():
     c = _strip_sim("f32", "fast", amp=2.0)
     for s in (a, b, c):
         s.step(steps)
+    tot, per = field_errors(a, b)
+    assert tot < 1e-5 and max(per.values()) < 1e-5, (tot, per)
     for C in COMPS:
-        fa, fb, fc = a.read_field(C()), b.read_field(C()), c.read_field(C())
+        fa, fc = a.read_field(C()), c.read_field(C())
         assert np.all(np.isfinite(fa))
-        assert rel_l2(fa, fb) < 1e-5, C.__name__
-        assert rel_l2(fc, 2 * fa) < 1e-5, C.__name__
+        # scaling every input by 2 scales every rounding by 2, except where
+        # values fall into the subnormal range (wavefront edge, deep PML)
+        assert rel_l2(fc, 2 * fa) < 1e-6, C.__name__
     del a, b, c
