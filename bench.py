@@ -63,7 +63,18 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (counter calibration)")
     args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return 0
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU: the time step runs on a single MI355X "
                          "(multi-GPU runs are not part of this build); use --gpus 1\n")

@@ -12,22 +12,22 @@
 //   inner_*_kernel  -- the box of cells whose PML sigma is zero on all three
 //     axes (FAST mode).  E/H advance directly (E += eps^-1 * dt * curl H, the
 //     D/B-eliminated form of src/Kernels/CUDAKernels.jl:46-74).  Each lane owns
-//     two adjacent x cells (one 8/16-byte access per array), each thread
-//     marches ZB cells along z carrying the z-neighbour in registers, and the
-//     loads of plane z+1 are issued before the stores of plane z so that the
-//     memory pipeline never drains.  Every array element crosses HBM once.
+//     two adjacent x cells (one 8/16-byte access per array) of ZB planes; with
+//     ZB > 1 the loads of plane z+1 are issued before the stores of plane z.
+//     Streams touched once (updated field, material) use nontemporal access.
 //
-//   full_*_kernel   -- every other cell (the PML shell, or the whole grid in
-//     EXACT mode): the reference's sigma cascade keeping D/B, U and W
-//     (src/Kernels/Helpers.jl:113-139, :323-340) in the CPU path's operation
-//     order, with current sources added to the material stage.
+//   cascade_*_kernel -- every other cell (the six PML slabs around the inner
+//     box, or the whole grid in EXACT mode), one cell per thread: the
+//     reference's sigma cascade keeping D/B, U and W (src/Kernels/Helpers.jl:
+//     113-139, :323-340) in the CPU path's operation order, with current
+//     sources added to the material stage.
 //
 //   source_fix_kernel -- FAST mode, source cells inside the inner box: the
 //     reference computes E = eps^-1 (D + S_n); with D eliminated this is
 //     E += eps^-1 (S_n - S_{n-1}).
 //
-// Tiles are assigned so that consecutive logical tiles (adjacent in y) run on
-// the same XCD and share its L2.
+// Blocks are assigned so that consecutive logical blocks (neighbouring rows
+// and planes) run on the same XCD and share its L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,14 +35,20 @@
 
 namespace kh {
 
-constexpr int TX = 64;       // lanes per tile row
-constexpr int TY = 4;        // rows per tile
+constexpr int TX = 64;       // lanes per inner-tile row
+constexpr int TY = 4;        // rows per inner tile
 constexpr int MAX_SRC = 8;
+constexpr int MAX_SLAB = 6;
 constexpr int NXCD = 8;
+constexpr int CBLK = 256;    // threads per cascade block
 
 template <typename T> struct V2;
 template <> struct V2<float> { using t = float2; };
 template <> struct V2<double> { using t = double2; };
+
+template <typename T> struct EV2;
+template <> struct EV2<float> { typedef float t __attribute__((ext_vector_type(2))); };
+template <> struct EV2<double> { typedef double t __attribute__((ext_vector_type(2))); };
 
 template <typename T>
 struct SrcL {
@@ -53,8 +59,14 @@ struct SrcL {
     T fr, fi;       // waveform value for this half step
 };
 
+struct Slab {
+    int lo[3];      // first grid index
+    int n[3];       // extent
+    int first;      // first logical block of the slab
+};
+
 template <typename T>
-struct HalfArgs {
+struct CascadeArgs {
     T *A[3];              // field being updated (H or E)
     const T *F[3];        // differentiated field (E or H)
     T *Tq[3];             // B or D
@@ -65,15 +77,11 @@ struct HalfArgs {
     T minv_s;
     T dt;
     T inv[3];
-    int n[3];
     int64_t sy, sz;       // element strides of y and z
-    int zb;               // z cells per tile
-    int tiles[3];         // tile counts in x, y, z of the whole grid
-    const int *tile_list; // tiles to process (tile ids, x fastest); null = all
-    int ntile;
-    int per_xcd;          // logical tiles per XCD slot
-    int skip_lo[3];       // cells inside [skip_lo, skip_hi] are left alone
-    int skip_hi[3];
+    int nslab;
+    Slab slab[MAX_SLAB];
+    int nblock;           // logical blocks over all slabs
+    int per_xcd;
     int nsrc;
     SrcL<T> src[MAX_SRC];
 };
@@ -94,159 +102,15 @@ struct InnerArgs {
     int per_xcd;
 };
 
-__device__ __forceinline__ bool xcd_tile(int per_xcd, int ntile, int &L) {
+__device__ __forceinline__ bool xcd_block(int per_xcd, int nblock, int &L) {
     // Physical block b runs on XCD b % 8: give each XCD a contiguous range of
-    // logical tiles so that y-neighbouring tiles share an L2.
+    // logical blocks so that neighbouring rows/planes share an L2.
     int b = blockIdx.x;
     L = (b % NXCD) * per_xcd + b / NXCD;
-    return L < ntile;
+    return L < nblock;
 }
 
-// ----------------------------------------------------------- full cascade
-template <typename T>
-__device__ __forceinline__ T source_value(const HalfArgs<T> &p, int comp, int gx, int gy, int gz) {
-    // S = sum over sources of Re(f * A) in registration order
-    // (src/Sources/Sources.jl:346-357, accumulated into a zeroed S array).
-    T s = T(0);
-    for (int q = 0; q < p.nsrc; ++q) {
-        const SrcL<T> &o = p.src[q];
-        if (o.comp != comp) continue;
-        int ix = gx - o.s[0], iy = gy - o.s[1], iz = gz - o.s[2];
-        if (ix < 0 || iy < 0 || iz < 0 || ix >= o.d[0] || iy >= o.d[1] || iz >= o.d[2]) continue;
-        size_t ai = (size_t)ix + (size_t)o.d[0] * ((size_t)iy + (size_t)o.d[1] * (size_t)iz);
-        T ar = o.amp[2 * ai], aim = o.amp[2 * ai + 1];
-        T v = (o.fr * ar) - (o.fi * aim);
-        s = s + v;
-    }
-    return s;
-}
-
-template <typename T>
-__device__ __forceinline__ bool in_source_box(const HalfArgs<T> &p, int comp, int gx, int gy, int gz) {
-    for (int q = 0; q < p.nsrc; ++q) {
-        const SrcL<T> &o = p.src[q];
-        if (o.comp != comp) continue;
-        int ix = gx - o.s[0], iy = gy - o.s[1], iz = gz - o.s[2];
-        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < o.d[0] && iy < o.d[1] && iz < o.d[2]) return true;
-    }
-    return false;
-}
-
-// One component of one cell, reference cascade.  K is dt * curl in the
-// reference's order; so, sn, sp are the own/next/prev-axis sigma samples.
-template <typename T>
-__device__ __forceinline__ void cascade_component(const HalfArgs<T> &p, int a, size_t idx, T K, T so,
-                                                  T sn, T sp, int gx, int gy, int gz) {
-    T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
-    // curl stage: C absent, U present (src/Kernels/Helpers.jl:113-139)
-    T tq = p.Tq[a][idx];
-    T tn;
-    if (sn == T(0)) {
-        tn = ((T(1) - sp) * tq + K) / (T(1) + sp);
-    } else {
-        T uo = p.U[a][idx];
-        T un = ((T(1) - sn) * uo + K) / (T(1) + sn);
-        p.U[a][idx] = un;
-        tn = (((T(1) - sp) * tq + un) - uo) / (T(1) + sp);
-    }
-    p.Tq[a][idx] = tn;
-    // material stage (src/Kernels/Helpers.jl:323-340)
-    T net = tn;
-    if (p.nsrc > 0 && in_source_box(p, a, gx, gy, gz)) net = net + source_value(p, a, gx, gy, gz);
-    if (so == T(0)) {
-        p.A[a][idx] = m * net;
-    } else {
-        T wo = p.W[a][idx];
-        T wn = m * net;
-        p.W[a][idx] = wn;
-        p.A[a][idx] = (p.A[a][idx] + (T(1) + so) * wn) - (T(1) - so) * wo;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ bool full_tile(const HalfArgs<T> &p, int &tx, int &ty, int &tz) {
-    int L;
-    if (!xcd_tile(p.per_xcd, p.ntile, L)) return false;
-    int id = p.tile_list ? p.tile_list[L] : L;
-    tx = id % p.tiles[0];
-    int r = id / p.tiles[0];
-    ty = r % p.tiles[1];
-    tz = r / p.tiles[1];
-    return true;
-}
-
-template <typename T>
-__device__ __forceinline__ bool skipped(const HalfArgs<T> &p, int gx, int gy, int gz) {
-    return gx >= p.skip_lo[0] && gx <= p.skip_hi[0] && gy >= p.skip_lo[1] && gy <= p.skip_hi[1] &&
-           gz >= p.skip_lo[2] && gz <= p.skip_hi[2];
-}
-
-// H half step, full cascade: K = dt * (d_n2 E_n1 - d_n1 E_n2) with forward
-// differences (src/Kernels/Helpers.jl:286-298, idx_curl = +1).
-template <typename T>
-__global__ __launch_bounds__(256) void full_h_kernel(HalfArgs<T> p) {
-    int tx, ty, tz;
-    if (!full_tile(p, tx, ty, tz)) return;
-    const int gx = tx * TX + threadIdx.x + 1;
-    const int gy = ty * TY + threadIdx.y + 1;
-    if (gx > p.n[0] || gy > p.n[1]) return;
-    const int z0 = tz * p.zb + 1;
-    const int z1 = min(z0 + p.zb - 1, p.n[2]);
-    const int64_t sy = p.sy, sz = p.sz;
-    const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
-    const T sgx = p.sig[0][gx], sgy = p.sig[1][gy];
-    size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)z0 * sz;
-    for (int gz = z0; gz <= z1; ++gz, idx += sz) {
-        if (skipped(p, gx, gy, gz)) continue;
-        const T sgz = p.sig[2][gz];
-        const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
-        const T ex_z1 = Ex[idx + sz], ey_z1 = Ey[idx + sz];
-        const T ez_x1 = Ez[idx + 1], ez_y1 = Ez[idx + sy];
-        const T ex_y1 = Ex[idx + sy], ey_x1 = Ey[idx + 1];
-        // x: d_z Ey - d_y Ez ; y: d_x Ez - d_z Ex ; z: d_y Ex - d_x Ey
-        const T Kx = p.dt * (p.inv[2] * (ey_z1 - ey) - p.inv[1] * (ez_y1 - ez));
-        const T Ky = p.dt * (p.inv[0] * (ez_x1 - ez) - p.inv[2] * (ex_z1 - ex));
-        const T Kz = p.dt * (p.inv[1] * (ex_y1 - ex) - p.inv[0] * (ey_x1 - ey));
-        // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y)
-        cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
-        cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
-        cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
-    }
-}
-
-// E half step, full cascade: backward differences (idx_curl = -1),
-// d(A) = inv * (A[i-1] - A[i]).
-template <typename T>
-__global__ __launch_bounds__(256) void full_e_kernel(HalfArgs<T> p) {
-    int tx, ty, tz;
-    if (!full_tile(p, tx, ty, tz)) return;
-    const int gx = tx * TX + threadIdx.x + 1;
-    const int gy = ty * TY + threadIdx.y + 1;
-    if (gx > p.n[0] || gy > p.n[1]) return;
-    const int z0 = tz * p.zb + 1;
-    const int z1 = min(z0 + p.zb - 1, p.n[2]);
-    const int64_t sy = p.sy, sz = p.sz;
-    const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
-    const T sgx = p.sig[0][gx], sgy = p.sig[1][gy];
-    size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)z0 * sz;
-    for (int gz = z0; gz <= z1; ++gz, idx += sz) {
-        if (skipped(p, gx, gy, gz)) continue;
-        const T sgz = p.sig[2][gz];
-        const T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
-        const T hx_zm = Hx[idx - sz], hy_zm = Hy[idx - sz];
-        const T hz_xm = Hz[idx - 1], hz_ym = Hz[idx - sy];
-        const T hx_ym = Hx[idx - sy], hy_xm = Hy[idx - 1];
-        // x: d_z Hy - d_y Hz ; y: d_x Hz - d_z Hx ; z: d_y Hx - d_x Hy
-        const T Kx = p.dt * (p.inv[2] * (hy_zm - hy) - p.inv[1] * (hz_ym - hz));
-        const T Ky = p.dt * (p.inv[0] * (hz_xm - hz) - p.inv[2] * (hx_zm - hx));
-        const T Kz = p.dt * (p.inv[1] * (hx_ym - hx) - p.inv[0] * (hy_xm - hy));
-        cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
-        cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
-        cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
-    }
-}
-
-// ----------------------------------------------------------- inner box
+// ------------------------------------------------------- memory helpers
 template <typename T>
 __device__ __forceinline__ typename V2<T>::t ld2(const T *p, size_t i) {
     return *reinterpret_cast<const typename V2<T>::t *>(p + i);
@@ -257,12 +121,173 @@ __device__ __forceinline__ void st2(T *p, size_t i, typename V2<T>::t v) {
     *reinterpret_cast<typename V2<T>::t *>(p + i) = v;
 }
 
+// Streams touched once per half step may bypass cache retention (NT =
+// nontemporal); the differentiated field is re-read by neighbouring lanes,
+// rows and planes and keeps normal caching.
+template <typename T, bool NT>
+__device__ __forceinline__ typename V2<T>::t ldo(const T *p, size_t i) {
+    if constexpr (NT) {
+        typename EV2<T>::t v = __builtin_nontemporal_load(reinterpret_cast<const typename EV2<T>::t *>(p + i));
+        typename V2<T>::t r;
+        r.x = v.x;
+        r.y = v.y;
+        return r;
+    } else {
+        return ld2(p, i);
+    }
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void sto(T *p, size_t i, typename V2<T>::t v) {
+    if constexpr (NT) {
+        typename EV2<T>::t w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<typename EV2<T>::t *>(p + i));
+    } else {
+        st2(p, i, v);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T ld1(const T *p, size_t i) {
+    return __builtin_nontemporal_load(p + i);
+}
+
+template <typename T>
+__device__ __forceinline__ void st1(T *p, size_t i, T v) {
+    __builtin_nontemporal_store(v, p + i);
+}
+
+// ------------------------------------------------------- cascade kernels
+template <typename T>
+__device__ __forceinline__ T source_value(const CascadeArgs<T> &p, int comp, int gx, int gy, int gz,
+                                          bool &inside) {
+    // S = sum over sources of Re(f * A) in registration order
+    // (src/Sources/Sources.jl:346-357, accumulated into a zeroed S array).
+    T s = T(0);
+    inside = false;
+    for (int q = 0; q < p.nsrc; ++q) {
+        const SrcL<T> &o = p.src[q];
+        if (o.comp != comp) continue;
+        int ix = gx - o.s[0], iy = gy - o.s[1], iz = gz - o.s[2];
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= o.d[0] || iy >= o.d[1] || iz >= o.d[2]) continue;
+        size_t ai = (size_t)ix + (size_t)o.d[0] * ((size_t)iy + (size_t)o.d[1] * (size_t)iz);
+        T ar = o.amp[2 * ai], aim = o.amp[2 * ai + 1];
+        T v = (o.fr * ar) - (o.fi * aim);
+        s = s + v;
+        inside = true;
+    }
+    return s;
+}
+
+// One component of one cell, reference cascade.  K is dt * curl in the
+// reference's order; so, sn, sp are the own/next/prev-axis sigma samples.
+template <typename T>
+__device__ __forceinline__ void cascade_component(const CascadeArgs<T> &p, int a, size_t idx, T K, T so,
+                                                  T sn, T sp, int gx, int gy, int gz) {
+    const T m = p.minv[a] ? ld1(p.minv[a], idx) : p.minv_s;
+    // curl stage: C absent, U present (src/Kernels/Helpers.jl:113-139)
+    const T tq = ld1((const T *)p.Tq[a], idx);
+    T tn;
+    if (sn == T(0)) {
+        tn = ((T(1) - sp) * tq + K) / (T(1) + sp);
+    } else {
+        const T uo = ld1((const T *)p.U[a], idx);
+        const T un = ((T(1) - sn) * uo + K) / (T(1) + sn);
+        st1(p.U[a], idx, un);
+        tn = (((T(1) - sp) * tq + un) - uo) / (T(1) + sp);
+    }
+    st1(p.Tq[a], idx, tn);
+    // material stage (src/Kernels/Helpers.jl:323-340)
+    T net = tn;
+    if (p.nsrc > 0) {
+        bool inside;
+        const T s = source_value(p, a, gx, gy, gz, inside);
+        if (inside) net = net + s;
+    }
+    if (so == T(0)) {
+        st1(p.A[a], idx, m * net);
+    } else {
+        const T wo = ld1((const T *)p.W[a], idx);
+        const T ao = ld1((const T *)p.A[a], idx);
+        const T wn = m * net;
+        st1(p.W[a], idx, wn);
+        st1(p.A[a], idx, (ao + (T(1) + so) * wn) - (T(1) - so) * wo);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool cascade_cell(const CascadeArgs<T> &p, int &gx, int &gy, int &gz) {
+    int L;
+    if (!xcd_block(p.per_xcd, p.nblock, L)) return false;
+    int s = 0;
+    while (s + 1 < p.nslab && L >= p.slab[s + 1].first) ++s;
+    const Slab &b = p.slab[s];
+    const int64_t c = (int64_t)(L - b.first) * CBLK + threadIdx.x;
+    const int64_t ncell = (int64_t)b.n[0] * b.n[1] * b.n[2];
+    if (c >= ncell) return false;
+    gx = b.lo[0] + (int)(c % b.n[0]);
+    const int64_t r = c / b.n[0];
+    gy = b.lo[1] + (int)(r % b.n[1]);
+    gz = b.lo[2] + (int)(r / b.n[1]);
+    return true;
+}
+
+// H half step, cascade: K = dt * (d_n2 E_n1 - d_n1 E_n2) with forward
+// differences (src/Kernels/Helpers.jl:286-298, idx_curl = +1).
+template <typename T>
+__global__ __launch_bounds__(CBLK) void cascade_h_kernel(CascadeArgs<T> p) {
+    int gx, gy, gz;
+    if (!cascade_cell(p, gx, gy, gz)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const T sgx = p.sig[0][gx], sgy = p.sig[1][gy], sgz = p.sig[2][gz];
+    const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
+    const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+    const T ex_z1 = Ex[idx + sz], ey_z1 = Ey[idx + sz];
+    const T ez_x1 = Ez[idx + 1], ez_y1 = Ez[idx + sy];
+    const T ex_y1 = Ex[idx + sy], ey_x1 = Ey[idx + 1];
+    // x: d_z Ey - d_y Ez ; y: d_x Ez - d_z Ex ; z: d_y Ex - d_x Ey
+    const T Kx = p.dt * (p.inv[2] * (ey_z1 - ey) - p.inv[1] * (ez_y1 - ez));
+    const T Ky = p.dt * (p.inv[0] * (ez_x1 - ez) - p.inv[2] * (ex_z1 - ex));
+    const T Kz = p.dt * (p.inv[1] * (ex_y1 - ex) - p.inv[0] * (ey_x1 - ey));
+    // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y)
+    cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
+    cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
+    cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
+}
+
+// E half step, cascade: backward differences (idx_curl = -1),
+// d(A) = inv * (A[i-1] - A[i]).
+template <typename T>
+__global__ __launch_bounds__(CBLK) void cascade_e_kernel(CascadeArgs<T> p) {
+    int gx, gy, gz;
+    if (!cascade_cell(p, gx, gy, gz)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const T sgx = p.sig[0][gx], sgy = p.sig[1][gy], sgz = p.sig[2][gz];
+    const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
+    const T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
+    const T hx_zm = Hx[idx - sz], hy_zm = Hy[idx - sz];
+    const T hz_xm = Hz[idx - 1], hz_ym = Hz[idx - sy];
+    const T hx_ym = Hx[idx - sy], hy_xm = Hy[idx - 1];
+    // x: d_z Hy - d_y Hz ; y: d_x Hz - d_z Hx ; z: d_y Hx - d_x Hy
+    const T Kx = p.dt * (p.inv[2] * (hy_zm - hy) - p.inv[1] * (hz_ym - hz));
+    const T Ky = p.dt * (p.inv[0] * (hz_xm - hz) - p.inv[2] * (hx_zm - hx));
+    const T Kz = p.dt * (p.inv[1] * (hx_ym - hx) - p.inv[0] * (hy_xm - hy));
+    cascade_component(p, 0, idx, Kx, sgx, sgy, sgz, gx, gy, gz);
+    cascade_component(p, 1, idx, Ky, sgy, sgz, sgx, gx, gy, gz);
+    cascade_component(p, 2, idx, Kz, sgz, sgx, sgy, gx, gy, gz);
+}
+
+// ----------------------------------------------------------- inner box
 template <typename T>
 __device__ __forceinline__ bool inner_lane(const InnerArgs<T> &p, int &x, int &y, int &z0, int &z1,
                                            bool &v0, bool &v1) {
     int L;
     const int ntile = p.tiles[0] * p.tiles[1] * p.tiles[2];
-    if (!xcd_tile(p.per_xcd, ntile, L)) return false;
+    if (!xcd_block(p.per_xcd, ntile, L)) return false;
     int tx = L % p.tiles[0];
     int r = L / p.tiles[0];
     int ty = r % p.tiles[1];
@@ -279,7 +304,7 @@ __device__ __forceinline__ bool inner_lane(const InnerArgs<T> &p, int &x, int &y
 }
 
 // H half step on the inner box.  Lane owns cells (x, x+1).
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool NT>
 __global__ __launch_bounds__(256) void inner_h_kernel(InnerArgs<T> p) {
     using V = typename V2<T>::t;
     int x, y, z0, z1;
@@ -304,13 +329,13 @@ __global__ __launch_bounds__(256) void inner_h_kernel(InnerArgs<T> p) {
         q.ey_x2 = Ey[i + 2];
         q.ez_y1 = ld2(Ez, i + sy);
         q.ex_y1 = ld2(Ex, i + sy);
-        q.hx = ld2((const T *)Hx, i);
-        q.hy = ld2((const T *)Hy, i);
-        q.hz = ld2((const T *)Hz, i);
+        q.hx = ldo<T, NT>((const T *)Hx, i);
+        q.hy = ldo<T, NT>((const T *)Hy, i);
+        q.hz = ldo<T, NT>((const T *)Hz, i);
         if (PERVOX) {
-            q.mx = ld2(p.minv[0], i);
-            q.my = ld2(p.minv[1], i);
-            q.mz = ld2(p.minv[2], i);
+            q.mx = ldo<T, NT>(p.minv[0], i);
+            q.my = ldo<T, NT>(p.minv[1], i);
+            q.mz = ldo<T, NT>(p.minv[2], i);
         }
         return q;
     };
@@ -336,9 +361,9 @@ __global__ __launch_bounds__(256) void inner_h_kernel(InnerArgs<T> p) {
         nhy.y = v1 ? cur.hy.y + m1y * Ky1 : cur.hy.y;
         nhz.x = v0 ? cur.hz.x + m0z * Kz0 : cur.hz.x;
         nhz.y = v1 ? cur.hz.y + m1z * Kz1 : cur.hz.y;
-        st2(Hx, idx, nhx);
-        st2(Hy, idx, nhy);
-        st2(Hz, idx, nhz);
+        sto<T, NT>(Hx, idx, nhx);
+        sto<T, NT>(Hy, idx, nhy);
+        sto<T, NT>(Hz, idx, nhz);
         ex = cur.ex1;
         ey = cur.ey1;
         cur = nxt;
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(256) void inner_h_kernel(InnerArgs<T> p) {
 }
 
 // E half step on the inner box.  Lane owns cells (x, x+1).
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool NT>
 __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
     using V = typename V2<T>::t;
     int x, y, z0, z1;
@@ -371,13 +396,13 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
         q.hy_xm = Hy[i - 1];
         q.hz_ym = ld2(Hz, i - sy);
         q.hx_ym = ld2(Hx, i - sy);
-        q.ex = ld2((const T *)Ex, i);
-        q.ey = ld2((const T *)Ey, i);
-        q.ez = ld2((const T *)Ez, i);
+        q.ex = ldo<T, NT>((const T *)Ex, i);
+        q.ey = ldo<T, NT>((const T *)Ey, i);
+        q.ez = ldo<T, NT>((const T *)Ez, i);
         if (PERVOX) {
-            q.mx = ld2(p.minv[0], i);
-            q.my = ld2(p.minv[1], i);
-            q.mz = ld2(p.minv[2], i);
+            q.mx = ldo<T, NT>(p.minv[0], i);
+            q.my = ldo<T, NT>(p.minv[1], i);
+            q.mz = ldo<T, NT>(p.minv[2], i);
         }
         return q;
     };
@@ -403,9 +428,9 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
         ney.y = v1 ? cur.ey.y + m1y * Ky1 : cur.ey.y;
         nez.x = v0 ? cur.ez.x + m0z * Kz0 : cur.ez.x;
         nez.y = v1 ? cur.ez.y + m1z * Kz1 : cur.ez.y;
-        st2(Ex, idx, nex);
-        st2(Ey, idx, ney);
-        st2(Ez, idx, nez);
+        sto<T, NT>(Ex, idx, nex);
+        sto<T, NT>(Ey, idx, ney);
+        sto<T, NT>(Ez, idx, nez);
         hxm = cur.hx;
         hym = cur.hy;
         cur = nxt;

@@ -318,13 +318,13 @@ struct Ctx final : CtxBase {
         bool inner = false;
         kh::InnerArgs<T> ia{};
         int inner_blocks = 0;
-        int *tiles = nullptr;  // device tile list for the full kernel
-        int ntile = 0;
-        int zb = 16;
-        int gtiles[3];
+        int nslab = 0;          // cascade-kernel boxes (PML slabs, or the whole grid)
+        kh::Slab slab[kh::MAX_SLAB];
+        int cblocks = 0;        // logical blocks of the cascade kernel
     };
     Plan plan[2];
     bool planned = false;
+    bool nt = false;           // nontemporal access for single-use streams (inner kernels)
     bool last_active = false;  // sources injected on the previous step
 
     static int env_int(const char *name, int def) {
@@ -345,9 +345,19 @@ struct Ctx final : CtxBase {
         }
     }
 
+    void add_slab(Plan &P, int x0, int x1, int y0, int y1, int z0, int z1) {
+        if (x0 > x1 || y0 > y1 || z0 > z1) return;
+        kh::Slab &s = P.slab[P.nslab++];
+        s.lo[0] = x0; s.lo[1] = y0; s.lo[2] = z0;
+        s.n[0] = x1 - x0 + 1; s.n[1] = y1 - y0 + 1; s.n[2] = z1 - z0 + 1;
+        s.first = P.cblocks;
+        int64_t cells = (int64_t)s.n[0] * s.n[1] * s.n[2];
+        P.cblocks += (int)((cells + kh::CBLK - 1) / kh::CBLK);
+    }
+
     void make_plans() {
-        const int zb_full = std::max(1, std::min(env_int("KH_ZB_FULL", 16), g.n[2]));
-        const int zb_inner = std::max(1, env_int("KH_ZB_INNER", 8));
+        const int zb_inner = std::max(1, env_int("KH_ZB_INNER", 1));
+        nt = env_int("KH_NT", 1) != 0;
         for (int e = 0; e < 2; ++e) {
             Plan &P = plan[e];
             // sampled sigma back on the host
@@ -364,27 +374,21 @@ struct Ctx final : CtxBase {
             P.inner = (mode == KH_MODE_FAST) && !any_empty;
             if (!P.inner)
                 for (int a = 0; a < 3; ++a) { P.lo[a] = 1; P.hi[a] = 0; }
-            // full-kernel tiles: every tile with a cell outside the inner box
-            P.zb = zb_full;
-            P.gtiles[0] = (g.n[0] + kh::TX - 1) / kh::TX;
-            P.gtiles[1] = (g.n[1] + kh::TY - 1) / kh::TY;
-            P.gtiles[2] = (g.n[2] + P.zb - 1) / P.zb;
-            std::vector<int> list;
-            for (int tz = 0; tz < P.gtiles[2]; ++tz)
-                for (int ty = 0; ty < P.gtiles[1]; ++ty)
-                    for (int tx = 0; tx < P.gtiles[0]; ++tx) {
-                        int c0[3] = {tx * kh::TX + 1, ty * kh::TY + 1, tz * P.zb + 1};
-                        int c1[3] = {std::min(c0[0] + kh::TX - 1, g.n[0]), std::min(c0[1] + kh::TY - 1, g.n[1]),
-                                     std::min(c0[2] + P.zb - 1, g.n[2])};
-                        bool inside = P.inner;
-                        for (int a = 0; a < 3 && inside; ++a) inside = c0[a] >= P.lo[a] && c1[a] <= P.hi[a];
-                        if (!inside) list.push_back(tx + P.gtiles[0] * (ty + P.gtiles[1] * tz));
-                    }
-            P.ntile = (int)list.size();
-            if (P.ntile) {
-                P.tiles = (int *)dalloc(list.size() * sizeof(int));
-                HIPCHK(hipMemcpyAsync(P.tiles, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, st));
-                HIPCHK(hipStreamSynchronize(st));
+            // cascade boxes: the six disjoint slabs around the inner box
+            // (z slabs span the grid, y slabs the inner z range, x slabs the
+            // inner y and z ranges), or the whole grid without an inner box
+            P.nslab = 0;
+            P.cblocks = 0;
+            const int *N = g.n;
+            if (!P.inner) {
+                add_slab(P, 1, N[0], 1, N[1], 1, N[2]);
+            } else {
+                add_slab(P, 1, N[0], 1, N[1], 1, P.lo[2] - 1);
+                add_slab(P, 1, N[0], 1, N[1], P.hi[2] + 1, N[2]);
+                add_slab(P, 1, N[0], 1, P.lo[1] - 1, P.lo[2], P.hi[2]);
+                add_slab(P, 1, N[0], P.hi[1] + 1, N[1], P.lo[2], P.hi[2]);
+                add_slab(P, 1, P.lo[0] - 1, P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
+                add_slab(P, P.hi[0] + 1, N[0], P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
             }
             if (P.inner) {
                 kh::InnerArgs<T> &ia = P.ia;
@@ -418,9 +422,9 @@ struct Ctx final : CtxBase {
         planned = true;
     }
 
-    kh::HalfArgs<T> full_args(int e) const {
+    kh::CascadeArgs<T> cascade_args(int e) const {
         const Plan &P = plan[e];
-        kh::HalfArgs<T> p{};
+        kh::CascadeArgs<T> p{};
         for (int a = 0; a < 3; ++a) {
             p.A[a] = e ? E[a] : H[a];
             p.F[a] = e ? H[a] : E[a];
@@ -430,19 +434,15 @@ struct Ctx final : CtxBase {
             p.sig[a] = e ? sigD[a] : sigB[a];
             p.minv[a] = e ? epsi[a] : mui[a];
             p.inv[a] = T(1) / dx[a];
-            p.n[a] = g.n[a];
-            p.tiles[a] = P.gtiles[a];
-            p.skip_lo[a] = P.lo[a];
-            p.skip_hi[a] = P.hi[a];
         }
         p.minv_s = e ? eps_s : mu_s;
         p.dt = dt;
         p.sy = g.sy();
         p.sz = g.sz();
-        p.zb = P.zb;
-        p.tile_list = P.tiles;
-        p.ntile = P.ntile;
-        p.per_xcd = (P.ntile + kh::NXCD - 1) / kh::NXCD;
+        p.nslab = P.nslab;
+        for (int s = 0; s < P.nslab; ++s) p.slab[s] = P.slab[s];
+        p.nblock = P.cblocks;
+        p.per_xcd = (P.cblocks + kh::NXCD - 1) / kh::NXCD;
         return p;
     }
 
@@ -465,26 +465,29 @@ struct Ctx final : CtxBase {
                 ev = take_event(2 * e);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
-            bool pervox = P.ia.minv[0] != nullptr;
+            const bool pervox = P.ia.minv[0] != nullptr;
+            const dim3 grid(P.inner_blocks);
+#define KH_INNER(KER, PV, NTV) hipLaunchKernelGGL((kh::KER<T, PV, NTV>), grid, blk, 0, st, P.ia)
             if (electric) {
-                if (pervox) hipLaunchKernelGGL((kh::inner_e_kernel<T, true>), dim3(P.inner_blocks), blk, 0, st, P.ia);
-                else hipLaunchKernelGGL((kh::inner_e_kernel<T, false>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+                if (pervox) { if (nt) KH_INNER(inner_e_kernel, true, true); else KH_INNER(inner_e_kernel, true, false); }
+                else { if (nt) KH_INNER(inner_e_kernel, false, true); else KH_INNER(inner_e_kernel, false, false); }
             } else {
-                if (pervox) hipLaunchKernelGGL((kh::inner_h_kernel<T, true>), dim3(P.inner_blocks), blk, 0, st, P.ia);
-                else hipLaunchKernelGGL((kh::inner_h_kernel<T, false>), dim3(P.inner_blocks), blk, 0, st, P.ia);
+                if (pervox) { if (nt) KH_INNER(inner_h_kernel, true, true); else KH_INNER(inner_h_kernel, true, false); }
+                else { if (nt) KH_INNER(inner_h_kernel, false, true); else KH_INNER(inner_h_kernel, false, false); }
             }
+#undef KH_INNER
             HIPCHK(hipGetLastError());
             if (timing) {
                 HIPCHK(hipEventRecord(ev.b, st));
                 evs.push_back(ev);
             }
         }
-        if (P.ntile) {
+        if (P.cblocks) {
             if (timing) {
                 ev = take_event(2 * e + 1);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
-            kh::HalfArgs<T> p = full_args(e);
+            kh::CascadeArgs<T> p = cascade_args(e);
             p.nsrc = 0;
             for (auto &o : srcs) {
                 if ((o.comp < 3) != electric) continue;
@@ -495,8 +498,9 @@ struct Ctx final : CtxBase {
                 if (active) wave_at(o, ts, l.fr, l.fi);
                 else { l.fr = T(0); l.fi = T(0); }
             }
-            if (electric) hipLaunchKernelGGL((kh::full_e_kernel<T>), dim3(p.per_xcd * kh::NXCD), blk, 0, st, p);
-            else hipLaunchKernelGGL((kh::full_h_kernel<T>), dim3(p.per_xcd * kh::NXCD), blk, 0, st, p);
+            const dim3 cgrid(p.per_xcd * kh::NXCD), cblk(kh::CBLK);
+            if (electric) hipLaunchKernelGGL((kh::cascade_e_kernel<T>), cgrid, cblk, 0, st, p);
+            else hipLaunchKernelGGL((kh::cascade_h_kernel<T>), cgrid, cblk, 0, st, p);
             HIPCHK(hipGetLastError());
             if (timing) {
                 HIPCHK(hipEventRecord(ev.b, st));
