@@ -437,27 +437,201 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
     }
 }
 
-// FAST mode, source cells inside the inner box: A += m * (S_n - S_{n-1}),
-// S = Re(f * amp) for one source (launched once per source, in order).
+// ------------------------------------------------------- FAST-mode PML
+// D/B-eliminated cascade.  The reference keeps T (= D or B) and sets
+// A = m (T + S) where the own-axis sigma is zero, W = m (T + S) otherwise
+// (src/Kernels/Helpers.jl:323-340).  Both relations let the T stage run on
+// A (or W) directly:
+//   U' = ((1 - s_n) U + K) / (1 + s_n),  in = U' - U        (s_n != 0)
+//   s_o == 0:  A' = ((1 - s_p) A + m in) / (1 + s_p)
+//   s_o != 0:  W' = ((1 - s_p) W + m in) / (1 + s_p),
+//              A' = A + (1 + s_o) W' - (1 - s_o) W
+// Source terms are added afterwards by source_fix_kernel.  A lane owns the x
+// pair (x, x+1) like the inner kernels; slab x ranges are widened to whole
+// pairs and the extra element is written back unchanged.
+struct PSlab {
+    int lo[3];      // first cell: lo[0] odd
+    int n[3];       // extent: n[0] even
+    int xlo, xhi;   // cells actually updated along x
+    int first;      // first logical block
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void source_fix_kernel(T *A, const T *minv, T minv_s, const T *amp, int s0,
-                                                         int s1, int s2, int d0, int d1, int d2, T fr, T fi,
-                                                         T pr, T pi, int lo0, int lo1, int lo2, int hi0,
-                                                         int hi1, int hi2, int64_t sy, int64_t sz) {
+struct PmlArgs {
+    T *A[3];
+    const T *F[3];
+    T *U[3];
+    T *W[3];
+    const T *sig[3];
+    const T *minv[3];
+    T minv_s;
+    T dt;
+    T inv[3];
+    int64_t sy, sz;
+    int nslab;
+    PSlab slab[MAX_SLAB];
+    int nblock;
+    int per_xcd;
+};
+
+template <typename T>
+__device__ __forceinline__ bool pml_lane(const PmlArgs<T> &p, int &x, int &y, int &z, bool &v0, bool &v1) {
+    int L;
+    if (!xcd_block(p.per_xcd, p.nblock, L)) return false;
+    int s = 0;
+    while (s + 1 < p.nslab && L >= p.slab[s + 1].first) ++s;
+    const PSlab &b = p.slab[s];
+    const int npx = b.n[0] >> 1;
+    const int64_t c = (int64_t)(L - b.first) * CBLK + threadIdx.x;
+    if (c >= (int64_t)npx * b.n[1] * b.n[2]) return false;
+    x = b.lo[0] + 2 * (int)(c % npx);
+    const int64_t r = c / npx;
+    y = b.lo[1] + (int)(r % b.n[1]);
+    z = b.lo[2] + (int)(r / b.n[1]);
+    v0 = x >= b.xlo && x <= b.xhi;
+    v1 = x + 1 >= b.xlo && x + 1 <= b.xhi;
+    return v0 || v1;
+}
+
+// One component of the pair.  so/sn/sp: own/next/prev sigma per element.
+template <typename T, bool PERVOX>
+__device__ __forceinline__ void pml_component(const PmlArgs<T> &p, int a, size_t idx, T K0, T K1, T so0,
+                                              T so1, T sn0, T sn1, T sp0, T sp1, bool v0, bool v1) {
+    using V = typename V2<T>::t;
+    T m0 = p.minv_s, m1 = p.minv_s;
+    if (PERVOX) {
+        V m = ldo<T, true>(p.minv[a], idx);
+        m0 = m.x;
+        m1 = m.y;
+    }
+    const V ao = ldo<T, true>((const T *)p.A[a], idx);
+    T in0 = K0, in1 = K1;
+    if (sn0 != T(0) || sn1 != T(0)) {
+        const V uo = ldo<T, true>((const T *)p.U[a], idx);
+        V un = uo;
+        if (sn0 != T(0) && v0) {
+            un.x = ((T(1) - sn0) * uo.x + K0) / (T(1) + sn0);
+            in0 = un.x - uo.x;
+        }
+        if (sn1 != T(0) && v1) {
+            un.y = ((T(1) - sn1) * uo.y + K1) / (T(1) + sn1);
+            in1 = un.y - uo.y;
+        }
+        sto<T, true>(p.U[a], idx, un);
+    }
+    V an = ao;
+    if (so0 != T(0) || so1 != T(0)) {
+        const V wo = ldo<T, true>((const T *)p.W[a], idx);
+        V wn = wo;
+        if (v0) {
+            if (so0 != T(0)) {
+                wn.x = ((T(1) - sp0) * wo.x + m0 * in0) / (T(1) + sp0);
+                an.x = (ao.x + (T(1) + so0) * wn.x) - (T(1) - so0) * wo.x;
+            } else {
+                an.x = ((T(1) - sp0) * ao.x + m0 * in0) / (T(1) + sp0);
+            }
+        }
+        if (v1) {
+            if (so1 != T(0)) {
+                wn.y = ((T(1) - sp1) * wo.y + m1 * in1) / (T(1) + sp1);
+                an.y = (ao.y + (T(1) + so1) * wn.y) - (T(1) - so1) * wo.y;
+            } else {
+                an.y = ((T(1) - sp1) * ao.y + m1 * in1) / (T(1) + sp1);
+            }
+        }
+        sto<T, true>(p.W[a], idx, wn);
+    } else {
+        if (v0) an.x = ((T(1) - sp0) * ao.x + m0 * in0) / (T(1) + sp0);
+        if (v1) an.y = ((T(1) - sp1) * ao.y + m1 * in1) / (T(1) + sp1);
+    }
+    sto<T, true>(p.A[a], idx, an);
+}
+
+template <typename T, bool PERVOX>
+__global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
+    int x, y, z;
+    bool v0, v1;
+    if (!pml_lane(p, x, y, z, v0, v1)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const size_t idx = (size_t)x + (size_t)y * sy + (size_t)z * sz;
+    const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
+    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    const auto ex = ld2(Ex, idx), ey = ld2(Ey, idx), ez = ld2(Ez, idx);
+    const auto ex1 = ld2(Ex, idx + sz), ey1 = ld2(Ey, idx + sz);
+    const auto ez_y1 = ld2(Ez, idx + sy), ex_y1 = ld2(Ex, idx + sy);
+    const T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
+    const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
+    const T Kx0 = dt * (iz * (ey1.x - ey.x) - iy * (ez_y1.x - ez.x));
+    const T Kx1 = dt * (iz * (ey1.y - ey.y) - iy * (ez_y1.y - ez.y));
+    const T Ky0 = dt * (ix * (ez.y - ez.x) - iz * (ex1.x - ex.x));
+    const T Ky1 = dt * (ix * (ez_x2 - ez.y) - iz * (ex1.y - ex.y));
+    const T Kz0 = dt * (iy * (ex_y1.x - ex.x) - ix * (ey.y - ey.x));
+    const T Kz1 = dt * (iy * (ex_y1.y - ex.y) - ix * (ey_x2 - ey.y));
+    // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y)
+    pml_component<T, PERVOX>(p, 0, idx, Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_component<T, PERVOX>(p, 1, idx, Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_component<T, PERVOX>(p, 2, idx, Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+}
+
+template <typename T, bool PERVOX>
+__global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
+    int x, y, z;
+    bool v0, v1;
+    if (!pml_lane(p, x, y, z, v0, v1)) return;
+    const int64_t sy = p.sy, sz = p.sz;
+    const size_t idx = (size_t)x + (size_t)y * sy + (size_t)z * sz;
+    const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
+    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    const auto hx = ld2(Hx, idx), hy = ld2(Hy, idx), hz = ld2(Hz, idx);
+    const auto hxm = ld2(Hx, idx - sz), hym = ld2(Hy, idx - sz);
+    const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
+    const T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
+    const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
+    const T Kx0 = dt * (iz * (hym.x - hy.x) - iy * (hz_ym.x - hz.x));
+    const T Kx1 = dt * (iz * (hym.y - hy.y) - iy * (hz_ym.y - hz.y));
+    const T Ky0 = dt * (ix * (hz_xm - hz.x) - iz * (hxm.x - hx.x));
+    const T Ky1 = dt * (ix * (hz.x - hz.y) - iz * (hxm.y - hx.y));
+    const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
+    const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
+    pml_component<T, PERVOX>(p, 0, idx, Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_component<T, PERVOX>(p, 1, idx, Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_component<T, PERVOX>(p, 2, idx, Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+}
+
+// FAST mode sources.  With T eliminated the reference's E = m (T + S_n)
+// (or W = m (T + S_n)) becomes, for one source added after the update,
+//   dW = m (S_n - (1 - s_p) / (1 + s_p) S_{n-1});
+//   s_o == 0: A += dW;   s_o != 0: W += dW, A += (1 + s_o) dW.
+// In the inner box (all sigma zero) this is A += m (S_n - S_{n-1}).
+// S = Re(f * amp); launched once per source, in order.
+template <typename T>
+__global__ __launch_bounds__(256) void source_fix_kernel(T *A, T *W, const T *minv, T minv_s, const T *sig_own,
+                                                         const T *sig_prev, int own, int prev, const T *amp,
+                                                         int s0, int s1, int s2, int d0, int d1, int d2, T fr,
+                                                         T fi, T pr, T pi, int n0, int n1, int n2, int64_t sy,
+                                                         int64_t sz) {
     const int64_t n = (int64_t)d0 * d1 * d2;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int ix = (int)(c % d0);
     const int64_t r = c / d0;
     const int iy = (int)(r % d1), iz = (int)(r / d1);
-    const int gx = s0 + ix, gy = s1 + iy, gz = s2 + iz;
-    if (gx < lo0 || gx > hi0 || gy < lo1 || gy > hi1 || gz < lo2 || gz > hi2) return;
+    const int g[3] = {s0 + ix, s1 + iy, s2 + iz};
+    // only updated cells consume sources (grid index 1..N)
+    if (g[0] < 1 || g[0] > n0 || g[1] < 1 || g[1] > n1 || g[2] < 1 || g[2] > n2) return;
     const T ar = amp[2 * c], ai = amp[2 * c + 1];
     const T vn = (fr * ar) - (fi * ai);
     const T vo = (pr * ar) - (pi * ai);
-    const size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const size_t idx = (size_t)g[0] + (size_t)g[1] * sy + (size_t)g[2] * sz;
     const T m = minv ? minv[idx] : minv_s;
-    A[idx] = A[idx] + m * (vn - vo);
+    const T so = sig_own[g[own]], sp = sig_prev[g[prev]];
+    const T dw = m * (vn - ((T(1) - sp) / (T(1) + sp)) * vo);
+    if (so == T(0)) {
+        A[idx] = A[idx] + dw;
+    } else {
+        W[idx] = W[idx] + dw;
+        A[idx] = A[idx] + (T(1) + so) * dw;
+    }
 }
 
 // ----------------------------------------------------------- monitors

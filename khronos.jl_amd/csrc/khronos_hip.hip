@@ -204,7 +204,9 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (int a = 0; a < 3; ++a) {
             E[a] = field_alloc(); H[a] = field_alloc();
-            D[a] = field_alloc(); B[a] = field_alloc();
+            // D and B are state only in EXACT mode; FAST mode carries the
+            // cascade on E/H and W directly
+            if (mode == KH_MODE_EXACT) { D[a] = field_alloc(); B[a] = field_alloc(); }
             UD[a] = field_alloc(); UB[a] = field_alloc();
             WD[a] = field_alloc(); WB[a] = field_alloc();
         }
@@ -318,9 +320,12 @@ struct Ctx final : CtxBase {
         bool inner = false;
         kh::InnerArgs<T> ia{};
         int inner_blocks = 0;
-        int nslab = 0;          // cascade-kernel boxes (PML slabs, or the whole grid)
+        int nslab = 0;          // EXACT: cascade-kernel boxes (the whole grid)
         kh::Slab slab[kh::MAX_SLAB];
         int cblocks = 0;        // logical blocks of the cascade kernel
+        int npslab = 0;         // FAST: PML slabs around the inner box
+        kh::PSlab pslab[kh::MAX_SLAB];
+        int pblocks = 0;        // logical blocks of the PML kernel
     };
     Plan plan[2];
     bool planned = false;
@@ -343,6 +348,21 @@ struct Ctx final : CtxBase {
             if (j - i + 1 > best) { best = j - i + 1; lo = i; hi = j; }
             i = j + 1;
         }
+    }
+
+    void add_pslab(Plan &P, int x0, int x1, int y0, int y1, int z0, int z1) {
+        // x range widened to whole pairs starting at an odd index
+        if (x0 > x1 || y0 > y1 || z0 > z1) return;
+        kh::PSlab &s = P.pslab[P.npslab++];
+        int xs = (x0 % 2 == 1) ? x0 : x0 - 1;
+        int xe = ((x1 - xs + 1) % 2 == 0) ? x1 : x1 + 1;
+        s.lo[0] = xs; s.lo[1] = y0; s.lo[2] = z0;
+        s.n[0] = xe - xs + 1; s.n[1] = y1 - y0 + 1; s.n[2] = z1 - z0 + 1;
+        s.xlo = x0;
+        s.xhi = x1;
+        s.first = P.pblocks;
+        int64_t pairs = (int64_t)(s.n[0] / 2) * s.n[1] * s.n[2];
+        P.pblocks += (int)((pairs + kh::CBLK - 1) / kh::CBLK);
     }
 
     void add_slab(Plan &P, int x0, int x1, int y0, int y1, int z0, int z1) {
@@ -377,18 +397,19 @@ struct Ctx final : CtxBase {
             // cascade boxes: the six disjoint slabs around the inner box
             // (z slabs span the grid, y slabs the inner z range, x slabs the
             // inner y and z ranges), or the whole grid without an inner box
-            P.nslab = 0;
-            P.cblocks = 0;
+            P.nslab = P.cblocks = P.npslab = P.pblocks = 0;
             const int *N = g.n;
-            if (!P.inner) {
+            if (mode == KH_MODE_EXACT) {
                 add_slab(P, 1, N[0], 1, N[1], 1, N[2]);
+            } else if (!P.inner) {
+                add_pslab(P, 1, N[0], 1, N[1], 1, N[2]);
             } else {
-                add_slab(P, 1, N[0], 1, N[1], 1, P.lo[2] - 1);
-                add_slab(P, 1, N[0], 1, N[1], P.hi[2] + 1, N[2]);
-                add_slab(P, 1, N[0], 1, P.lo[1] - 1, P.lo[2], P.hi[2]);
-                add_slab(P, 1, N[0], P.hi[1] + 1, N[1], P.lo[2], P.hi[2]);
-                add_slab(P, 1, P.lo[0] - 1, P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
-                add_slab(P, P.hi[0] + 1, N[0], P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
+                add_pslab(P, 1, N[0], 1, N[1], 1, P.lo[2] - 1);
+                add_pslab(P, 1, N[0], 1, N[1], P.hi[2] + 1, N[2]);
+                add_pslab(P, 1, N[0], 1, P.lo[1] - 1, P.lo[2], P.hi[2]);
+                add_pslab(P, 1, N[0], P.hi[1] + 1, N[1], P.lo[2], P.hi[2]);
+                add_pslab(P, 1, P.lo[0] - 1, P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
+                add_pslab(P, P.hi[0] + 1, N[0], P.lo[1], P.hi[1], P.lo[2], P.hi[2]);
             }
             if (P.inner) {
                 kh::InnerArgs<T> &ia = P.ia;
@@ -507,22 +528,63 @@ struct Ctx final : CtxBase {
                 evs.push_back(ev);
             }
         }
-        // FAST mode: sources inside the inner box, E += m (S_n - S_{n-1})
-        if (P.inner && (active || last_active)) {
+        if (P.pblocks) {
+            if (timing) {
+                ev = take_event(2 * e + 1);
+                HIPCHK(hipEventRecord(ev.a, st));
+            }
+            kh::PmlArgs<T> q{};
+            for (int a = 0; a < 3; ++a) {
+                q.A[a] = electric ? E[a] : H[a];
+                q.F[a] = electric ? H[a] : E[a];
+                q.U[a] = electric ? UD[a] : UB[a];
+                q.W[a] = electric ? WD[a] : WB[a];
+                q.sig[a] = electric ? sigD[a] : sigB[a];
+                q.minv[a] = electric ? epsi[a] : mui[a];
+                q.inv[a] = T(1) / dx[a];
+            }
+            q.minv_s = electric ? eps_s : mu_s;
+            q.dt = dt;
+            q.sy = g.sy();
+            q.sz = g.sz();
+            q.nslab = P.npslab;
+            for (int s = 0; s < P.npslab; ++s) q.slab[s] = P.pslab[s];
+            q.nblock = P.pblocks;
+            q.per_xcd = (P.pblocks + kh::NXCD - 1) / kh::NXCD;
+            const bool pervox = q.minv[0] != nullptr;
+            const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
+            if (electric) {
+                if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true>), pgrid, pblk, 0, st, q);
+                else hipLaunchKernelGGL((kh::pml_e_kernel<T, false>), pgrid, pblk, 0, st, q);
+            } else {
+                if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true>), pgrid, pblk, 0, st, q);
+                else hipLaunchKernelGGL((kh::pml_h_kernel<T, false>), pgrid, pblk, 0, st, q);
+            }
+            HIPCHK(hipGetLastError());
+            if (timing) {
+                HIPCHK(hipEventRecord(ev.b, st));
+                evs.push_back(ev);
+            }
+        }
+        // FAST mode sources: added after the update, for every source cell
+        if (mode == KH_MODE_FAST && (active || last_active)) {
             for (auto &o : srcs) {
                 if ((o.comp < 3) != electric) continue;
                 T fr = 0, fi = 0, pr = 0, pi = 0;
                 if (active) wave_at(o, ts, fr, fi);
                 if (last_active) wave_at(o, ts - 1, pr, pi);
-                int a = o.comp % 3;
+                const int a = o.comp % 3, prv = (a + 2) % 3;
                 int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
                 unsigned nb = (unsigned)((n + 255) / 256);
                 T *A = electric ? E[a] : H[a];
+                T *W = electric ? WD[a] : WB[a];
                 const T *mi = electric ? epsi[a] : mui[a];
-                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, mi,
-                                   electric ? eps_s : mu_s, (const T *)o.amp, o.s[0], o.s[1], o.s[2], o.d[0],
-                                   o.d[1], o.d[2], fr, fi, pr, pi, P.lo[0], P.lo[1], P.lo[2], P.hi[0],
-                                   P.hi[1], P.hi[2], g.sy(), g.sz());
+                const T *so = electric ? sigD[a] : sigB[a];
+                const T *sp = electric ? sigD[prv] : sigB[prv];
+                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, W, mi,
+                                   electric ? eps_s : mu_s, so, sp, a, prv, (const T *)o.amp, o.s[0], o.s[1],
+                                   o.s[2], o.d[0], o.d[1], o.d[2], fr, fi, pr, pi, g.n[0], g.n[1], g.n[2],
+                                   g.sy(), g.sz());
                 HIPCHK(hipGetLastError());
             }
         }
@@ -693,7 +755,8 @@ struct Ctx final : CtxBase {
         size_t fb = (size_t)g.total * sizeof(T);
         for (int a = 0; a < 3; ++a) {
             T *arrs[8] = {E[a], H[a], D[a], B[a], UD[a], UB[a], WD[a], WB[a]};
-            for (T *p : arrs) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
+            for (T *p : arrs)
+                if (p) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
         }
         for (auto &m : dfts)
             HIPCHK(hipMemsetAsync(m.acc, 0, (size_t)m.ncell * m.nf * 2 * (acc64 ? sizeof(double) : sizeof(T)), st));
@@ -738,7 +801,8 @@ size_t footprint(const kh_sim_desc &d) {
     Grid g;
     g.init(d.nx, d.ny, d.nz);
     size_t es = d.precision == KH_F32 ? 4 : 8;
-    size_t arrays = 24;
+    // E, H, U and W per component; D and B only in EXACT mode
+    size_t arrays = d.mode == KH_MODE_EXACT ? 24 : 18;
     for (int a = 0; a < 3; ++a) arrays += (d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr);
     size_t sig = 0;
     for (int a = 0; a < 3; ++a) sig += 2 * ((size_t)g.n[a] + 2) * es;

@@ -39,7 +39,10 @@ def test_version_and_footprint_without_device():
     n = C.c_size_t()
     assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == 0
     cells = 386 * 194 * 162
-    assert 24 * cells * 4 <= n.value <= 32 * cells * 4
+    assert 18 * cells * 4 <= n.value <= 24 * cells * 4  # FAST: E, H, U, W x 3
+    d.mode = _lib.KH_MODE_EXACT
+    assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == 0
+    assert 24 * cells * 4 <= n.value <= 32 * cells * 4  # EXACT adds D and B
     d.nz = 0
     assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == -1
     assert b"extents" in lib.kh_last_error()
