@@ -56,6 +56,23 @@ def cpu_baseline(spec, cores: int, budget_s: float = 15.0):
     }
 
 
+def measured_traffic(kernel_prefix: str):
+    """HBM bytes per dispatch of a kernel from the committed counter summary
+    (profiles/pmc_traffic_strip_waveguide_fp32_fast.md: FETCH_SIZE and
+    WRITE_SIZE runs scaled by a 1 GiB copy's counter readings), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_strip_waveguide_fp32_fast.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0].startswith(kernel_prefix):
+            try:
+                return int(float(cells[5]))
+            except ValueError:
+                return None
+    return None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,7 +134,7 @@ def main(argv=None):
     # dominant kernel: the E update of the sigma-free inner box (about 70% of
     # the cells and of the step's bytes); EXACT mode runs only the full kernel
     if args.mode == "fast":
-        dom, dom_bytes, dom_name = avg["e_inner"], ab["E_inner"], "inner_e_kernel<float,true>"
+        dom, dom_bytes, dom_name = avg["e_inner"], ab["E_inner"], "inner_e_kernel<float,true,true>"
     else:
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "full_e_kernel<float>"
     achieved = dom_bytes / (dom * 1e-3) / 1e9 if dom > 0 else 0.0
@@ -153,7 +170,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("kh::inner_e_kernel<float, true, true>") if args.mode == "fast" else None,
             "kernel": dom_name,
             "alg_bytes_per_launch": dom_bytes,
             "kernel_ms": round(dom, 5),

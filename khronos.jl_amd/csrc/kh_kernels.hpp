@@ -493,58 +493,91 @@ __device__ __forceinline__ bool pml_lane(const PmlArgs<T> &p, int &x, int &y, in
     return v0 || v1;
 }
 
-// One component of the pair.  so/sn/sp: own/next/prev sigma per element.
+// State of one component of the pair, loaded before any store of the lane
+// (the memory counter is in order: a load issued after a store waits for it).
+template <typename T>
+struct PmlComp {
+    typename V2<T>::t m, ao, uo, wo;
+    bool nu, nw;  // U stage (next-axis sigma) / W stage (own-axis sigma) present
+};
+
 template <typename T, bool PERVOX>
-__device__ __forceinline__ void pml_component(const PmlArgs<T> &p, int a, size_t idx, T K0, T K1, T so0,
-                                              T so1, T sn0, T sn1, T sp0, T sp1, bool v0, bool v1) {
-    using V = typename V2<T>::t;
-    T m0 = p.minv_s, m1 = p.minv_s;
+__device__ __forceinline__ void pml_load_a(const PmlArgs<T> &p, int a, size_t idx, PmlComp<T> &c) {
     if (PERVOX) {
-        V m = ldo<T, true>(p.minv[a], idx);
-        m0 = m.x;
-        m1 = m.y;
+        c.m = ldo<T, true>(p.minv[a], idx);
+    } else {
+        c.m.x = p.minv_s;
+        c.m.y = p.minv_s;
     }
-    const V ao = ldo<T, true>((const T *)p.A[a], idx);
+    c.ao = ldo<T, true>((const T *)p.A[a], idx);
+}
+
+template <typename T>
+__device__ __forceinline__ void pml_load_uw(const PmlArgs<T> &p, int a, size_t idx, bool nu, bool nw,
+                                            PmlComp<T> &c) {
+    c.nu = nu;
+    c.nw = nw;
+    if (nu) c.uo = ldo<T, true>((const T *)p.U[a], idx);
+    if (nw) c.wo = ldo<T, true>((const T *)p.W[a], idx);
+}
+
+// Update and store one component.  so/sn/sp: own/next/prev sigma per element.
+template <typename T>
+__device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t idx, const PmlComp<T> &c, T K0,
+                                           T K1, T so0, T so1, T sn0, T sn1, T sp0, T sp1, bool v0, bool v1) {
+    using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
-    if (sn0 != T(0) || sn1 != T(0)) {
-        const V uo = ldo<T, true>((const T *)p.U[a], idx);
-        V un = uo;
+    if (c.nu) {
+        V un = c.uo;
         if (sn0 != T(0) && v0) {
-            un.x = ((T(1) - sn0) * uo.x + K0) / (T(1) + sn0);
-            in0 = un.x - uo.x;
+            un.x = ((T(1) - sn0) * c.uo.x + K0) / (T(1) + sn0);
+            in0 = un.x - c.uo.x;
         }
         if (sn1 != T(0) && v1) {
-            un.y = ((T(1) - sn1) * uo.y + K1) / (T(1) + sn1);
-            in1 = un.y - uo.y;
+            un.y = ((T(1) - sn1) * c.uo.y + K1) / (T(1) + sn1);
+            in1 = un.y - c.uo.y;
         }
         sto<T, true>(p.U[a], idx, un);
     }
-    V an = ao;
-    if (so0 != T(0) || so1 != T(0)) {
-        const V wo = ldo<T, true>((const T *)p.W[a], idx);
-        V wn = wo;
+    V an = c.ao;
+    if (c.nw) {
+        V wn = c.wo;
         if (v0) {
             if (so0 != T(0)) {
-                wn.x = ((T(1) - sp0) * wo.x + m0 * in0) / (T(1) + sp0);
-                an.x = (ao.x + (T(1) + so0) * wn.x) - (T(1) - so0) * wo.x;
+                wn.x = ((T(1) - sp0) * c.wo.x + c.m.x * in0) / (T(1) + sp0);
+                an.x = (c.ao.x + (T(1) + so0) * wn.x) - (T(1) - so0) * c.wo.x;
             } else {
-                an.x = ((T(1) - sp0) * ao.x + m0 * in0) / (T(1) + sp0);
+                an.x = ((T(1) - sp0) * c.ao.x + c.m.x * in0) / (T(1) + sp0);
             }
         }
         if (v1) {
             if (so1 != T(0)) {
-                wn.y = ((T(1) - sp1) * wo.y + m1 * in1) / (T(1) + sp1);
-                an.y = (ao.y + (T(1) + so1) * wn.y) - (T(1) - so1) * wo.y;
+                wn.y = ((T(1) - sp1) * c.wo.y + c.m.y * in1) / (T(1) + sp1);
+                an.y = (c.ao.y + (T(1) + so1) * wn.y) - (T(1) - so1) * c.wo.y;
             } else {
-                an.y = ((T(1) - sp1) * ao.y + m1 * in1) / (T(1) + sp1);
+                an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
             }
         }
         sto<T, true>(p.W[a], idx, wn);
     } else {
-        if (v0) an.x = ((T(1) - sp0) * ao.x + m0 * in0) / (T(1) + sp0);
-        if (v1) an.y = ((T(1) - sp1) * ao.y + m1 * in1) / (T(1) + sp1);
+        if (v0) an.x = ((T(1) - sp0) * c.ao.x + c.m.x * in0) / (T(1) + sp0);
+        if (v1) an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
     }
     sto<T, true>(p.A[a], idx, an);
+}
+
+// Loads of the updated field's state for the three components:
+// (own, next, prev) = (x,y,z), (y,z,x), (z,x,y).
+template <typename T, bool PERVOX>
+__device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T sx0, T sx1, T sgy, T sgz,
+                                             PmlComp<T> c[3]) {
+    pml_load_a<T, PERVOX>(p, 0, idx, c[0]);
+    pml_load_a<T, PERVOX>(p, 1, idx, c[1]);
+    pml_load_a<T, PERVOX>(p, 2, idx, c[2]);
+    const bool px = sx0 != T(0) || sx1 != T(0), py = sgy != T(0), pz = sgz != T(0);
+    pml_load_uw(p, 0, idx, py, px, c[0]);
+    pml_load_uw(p, 1, idx, pz, py, c[1]);
+    pml_load_uw(p, 2, idx, px, pz, c[2]);
 }
 
 template <typename T, bool PERVOX>
@@ -560,6 +593,8 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const auto ex1 = ld2(Ex, idx + sz), ey1 = ld2(Ey, idx + sz);
     const auto ez_y1 = ld2(Ez, idx + sy), ex_y1 = ld2(Ex, idx + sy);
     const T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
+    PmlComp<T> c[3];
+    pml_load_all<T, PERVOX>(p, idx, sx0, sx1, sgy, sgz, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (ey1.x - ey.x) - iy * (ez_y1.x - ez.x));
     const T Kx1 = dt * (iz * (ey1.y - ey.y) - iy * (ez_y1.y - ez.y));
@@ -567,10 +602,9 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const T Ky1 = dt * (ix * (ez_x2 - ez.y) - iz * (ex1.y - ex.y));
     const T Kz0 = dt * (iy * (ex_y1.x - ex.x) - ix * (ey.y - ey.x));
     const T Kz1 = dt * (iy * (ex_y1.y - ex.y) - ix * (ey_x2 - ey.y));
-    // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y)
-    pml_component<T, PERVOX>(p, 0, idx, Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_component<T, PERVOX>(p, 1, idx, Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_component<T, PERVOX>(p, 2, idx, Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    pml_update(p, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_update(p, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
 }
 
 template <typename T, bool PERVOX>
@@ -586,6 +620,8 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const auto hxm = ld2(Hx, idx - sz), hym = ld2(Hy, idx - sz);
     const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
     const T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
+    PmlComp<T> c[3];
+    pml_load_all<T, PERVOX>(p, idx, sx0, sx1, sgy, sgz, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (hym.x - hy.x) - iy * (hz_ym.x - hz.x));
     const T Kx1 = dt * (iz * (hym.y - hy.y) - iy * (hz_ym.y - hz.y));
@@ -593,9 +629,9 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const T Ky1 = dt * (ix * (hz.x - hz.y) - iz * (hxm.y - hx.y));
     const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
     const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
-    pml_component<T, PERVOX>(p, 0, idx, Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_component<T, PERVOX>(p, 1, idx, Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_component<T, PERVOX>(p, 2, idx, Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    pml_update(p, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_update(p, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
 }
 
 // FAST mode sources.  With T eliminated the reference's E = m (T + S_n)
