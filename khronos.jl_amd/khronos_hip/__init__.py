@@ -10,6 +10,8 @@ from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, U
                       ArraySource, GaussianProfileSource)
 from .monitors import DFTMonitor, FluxMonitor
 from .geometry import Material, Object, Box, Sphere, Cylinder
+from .susceptibility import (LorentzianSusceptibility, DrudeSusceptibility, compute_ade_coefficients,
+                             eval_susceptibility)
 from .simulation import (Simulation, SimSpec, run, run_until, stop_when_dft_decayed, run_benchmark,
                          reset_fields, get_flux, dft_fields_norm, prepare_simulation)
 
@@ -18,6 +20,7 @@ __all__ = [
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
     "GaussianProfileSource", "DFTMonitor", "FluxMonitor", "Material", "Object", "Box", "Sphere",
-    "Cylinder", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
+    "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",
+    "eval_susceptibility", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
     "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",
 ]

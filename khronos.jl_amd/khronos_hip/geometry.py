@@ -9,6 +9,11 @@ sample point lies inside it (boundaries inclusive).
 The reference tests membership with the GeometryPrimitives package (not
 vendored); the box / sphere / cylinder membership tests here are the closed
 forms of those shapes.  Subpixel smoothing is not applied here.
+
+Also here: the chi3 array (src/Geometry.jl:610-636) and the dispersive-pole
+setup of ``init_polarization!`` (src/Geometry.jl:1049-1355): unique poles,
+per-pole oscillator-strength arrays, their zeroing inside the PML, and the
+chi1 correction of the inverse permittivity.
 """
 from __future__ import annotations
 
@@ -17,7 +22,10 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .grid import Ex, Ey, Ez, Hx, Hy, Hz, full_grid_volume, gv_coords, component_voxel_count
+from .grid import (Center, Ex, Ey, Ez, Hx, Hy, Hz, build_coords, component_origin, full_grid_volume, gv_coords,
+                   component_voxel_count)
+from .susceptibility import (LorentzianSusceptibility, chi1_coefficient, compute_ade_coefficients,
+                             has_susceptibilities)
 
 
 @dataclass
@@ -30,6 +38,7 @@ class Material:
     εz: Optional[float] = None
     μ: Optional[float] = None
     chi3: Optional[float] = None
+    susceptibilities: List[LorentzianSusceptibility] = field(default_factory=list)
 
     def perm(self, electric: bool, axis: int) -> Optional[float]:
         if electric:
@@ -143,3 +152,158 @@ def rasterize(sim, geometry: List[Object]):
                 arr[np.ix_(ix, iy, iz)] = sub
             out[a] = arr
     return out_eps, out_mu
+
+
+def _index_range(coords, lo, hi):
+    """searchsortedfirst(coords, lo) .. searchsortedlast(coords, hi), 0-based."""
+    return np.nonzero((coords >= lo) & (coords <= hi))[0]
+
+
+def _paint(arr, xs, ys, zs, shape, value):
+    lo, hi = shape.bounds()
+    ix = _index_range(xs, lo[0], hi[0])
+    iy = _index_range(ys, lo[1], hi[1])
+    iz = _index_range(zs, lo[2], hi[2])
+    if len(ix) == 0 or len(iy) == 0 or len(iz) == 0:
+        return
+    inside = shape.contains(xs[ix][:, None, None], ys[iy][None, :, None], zs[iz][None, None, :])
+    sub = arr[np.ix_(ix, iy, iz)]
+    sub[inside] = value
+    arr[np.ix_(ix, iy, iz)] = sub
+
+
+def rasterize_chi3(sim, geometry: List[Object]) -> Optional[np.ndarray]:
+    """chi3 array (Nx, Ny, Nz) on the cell centres, or None (src/Geometry.jl:610-636).
+
+    Objects are painted last to first and only where their chi3 is non-zero,
+    so an object without chi3 does not clear one painted below it."""
+    R = sim.real
+    if not any(o.material.chi3 is not None for o in geometry):
+        return None
+    arr = np.zeros((sim.Nx, sim.Ny, sim.Nz), dtype=R)
+    xs, ys, zs = gv_coords(sim, full_grid_volume(sim, Center()))
+    xs, ys, zs = xs[:sim.Nx], ys[:sim.Ny], zs[:sim.Nz]
+    for obj in reversed(geometry):
+        c3 = R(0) if obj.material.chi3 is None else R(obj.material.chi3)
+        if c3 != 0:
+            _paint(arr, xs, ys, zs, obj.shape, c3)
+    return arr
+
+
+@dataclass
+class PoleData:
+    """One dispersive pole as the time step consumes it: ADE coefficients and
+    the oscillator strength per cell, (Nx, Ny, Nz) in the backend type."""
+
+    pole: LorentzianSusceptibility
+    coeffs: object
+    sigma: np.ndarray
+
+
+def collect_unique_poles(geometry: List[Object]) -> List[LorentzianSusceptibility]:
+    """Poles keyed by (omega_0, gamma), in first-seen order (src/Geometry.jl:1053-1073)."""
+    poles, seen = [], set()
+    for obj in geometry:
+        if not has_susceptibilities(obj.material):
+            continue
+        for s in obj.material.susceptibilities:
+            key = (s.omega_0, s.gamma)
+            if key not in seen:
+                seen.add(key)
+                poles.append(s)
+    return poles
+
+
+def init_polarization(sim, geometry: List[Object], eps_inv: list):
+    """Host part of ``init_polarization!`` (src/Geometry.jl:1136-1355) for one chunk.
+
+    Returns ``(poles, eps_inv)``: the per-pole data and the inverse
+    permittivity arrays after the chi1 correction eps_inv / (1 + eps_inv chi1).
+    The reference applies that correction with the linear index of the
+    (Nx, Ny, Nz) chi1 array into each component's array
+    (src/Geometry.jl:1334-1342).  With a scalar permittivity the arrays it
+    corrects are (Nx, Ny, Nz) too and the linear index is the cell; with
+    per-voxel arrays (component voxel count) the linear index lands on other
+    cells, and that is reproduced here as the reference executes it.
+    """
+    R = sim.real
+    poles = collect_unique_poles(geometry)
+    if not poles:
+        return [], eps_inv
+    # Ex sample points (src/Geometry.jl:1146-1167)
+    gv = full_grid_volume(sim, Ex())
+    org = component_origin(sim.cell_center, sim.cell_size_n, sim.delta_n, Ex())
+    cnt = component_voxel_count(sim.Nx, sim.Ny, sim.Nz, Ex())
+    xs = build_coords(sim.delta_n[0], org[0], gv.start_idx[0], gv.Nx)[:cnt[0]]
+    ys = build_coords(sim.delta_n[1], org[1], gv.start_idx[1], gv.Ny)[:cnt[1]]
+    zs = build_coords(sim.delta_n[2], org[2], gv.start_idx[2], gv.Nz)[:cnt[2]]
+    sigmas = []
+    for pole in poles:
+        arr = np.zeros((len(xs), len(ys), len(zs)), dtype=R)
+        key = (pole.omega_0, pole.gamma)
+        for obj in reversed(geometry):
+            if not has_susceptibilities(obj.material):
+                continue
+            val = None
+            for s in obj.material.susceptibilities:
+                if (s.omega_0, s.gamma) == key:
+                    val = R(s.sigma)
+                    break
+            if val is not None:
+                _paint(arr, xs, ys, zs, obj.shape, val)
+        sigmas.append(arr)
+    # zero the oscillator strength inside the PML (src/Geometry.jl:1180-1222)
+    if sim.boundaries is not None:
+        # cell_size and boundaries are stored in the backend type
+        # (src/DataStructures.jl:719, :723); the comparisons run in float64
+        half = [float(R(v) / R(2)) for v in sim.cell_size_n]
+        cc = [float(v) for v in sim.cell_center]
+        inpml = np.zeros((len(xs), len(ys), len(zs)), dtype=bool)
+        for a, co in enumerate((xs, ys, zs)):
+            if len(sim.boundaries) <= a:
+                continue
+            lo, hi = cc[a] - half[a], cc[a] + half[a]
+            left, right = float(R(sim.boundaries[a][0])), float(R(sim.boundaries[a][1]))
+            m = np.zeros(len(co), dtype=bool)
+            if left > 0:
+                m |= co < lo + left
+            if right > 0:
+                m |= co > hi - right
+            shp = [1, 1, 1]
+            shp[a] = len(co)
+            inpml |= m.reshape(shp)
+        for arr in sigmas:
+            arr[inpml] = R(0)
+    # chi1 = sum_k sigma_k c_k in the backend type (src/Geometry.jl:1224-1237)
+    chi1 = np.zeros((len(xs), len(ys), len(zs)), dtype=R)
+    coeffs = []
+    for pole, arr in zip(poles, sigmas):
+        c = compute_ade_coefficients(pole, sim.dt_n)
+        coeffs.append(c)
+        chi1 = chi1 + arr * R(chi1_coefficient(c))
+    N = (sim.Nx, sim.Ny, sim.Nz)
+    sl = (slice(0, N[0]), slice(0, N[1]), slice(0, N[2]))
+    out = [PoleData(p, c, np.ascontiguousarray(arr[sl])) for p, c, arr in zip(poles, coeffs, sigmas)]
+    chunk_chi1 = chi1[sl]
+    if np.max(np.abs(chunk_chi1)) > 0:
+        new_eps = []
+        flat_chi1 = chunk_chi1.ravel(order="F")
+        nz = np.nonzero(flat_chi1 != 0)[0]
+        for a in range(3):
+            if eps_inv[a] is None:
+                # scalar: the corrected array is per cell; cells 1..N hold the
+                # corrected values, the component's extra plane is never read
+                cnt_a = component_voxel_count(*N, (Ex(), Ey(), Ez())[a])
+                arr = np.ones(cnt_a, dtype=R)
+                e = np.ones(N, dtype=R)
+                f = e.ravel(order="F")
+                f[nz] = f[nz] / (R(1) + f[nz] * flat_chi1[nz])
+                arr[sl] = f.reshape(N, order="F")
+            else:
+                arr = np.array(eps_inv[a], dtype=R, order="F", copy=True)
+                f = arr.ravel(order="F")
+                f[nz] = f[nz] / (R(1) + f[nz] * flat_chi1[nz])
+                arr = f.reshape(arr.shape, order="F")
+            new_eps.append(arr)
+        eps_inv = new_eps
+    return out, eps_inv

@@ -246,8 +246,16 @@ def gv_coords(sim_like, gv: GridVolume):
     d = sim_like.delta_n
     xs = np.array([org[0] + (i + gv.start_idx[0] - 2) * d[0] for i in range(1, gv.Nx + 1)])
     ys = np.array([org[1] + (i + gv.start_idx[1] - 2) * d[1] for i in range(1, gv.Ny + 1)])
+    # z is accumulated from a base point, in that order (src/Geometry.jl:359-364)
+    pz_base = org[2] + (gv.start_idx[2] - 2) * d[2]
     zs = []
     for i in range(1, gv.Nz + 1):
-        v = org[2] + (i + gv.start_idx[2] - 2) * d[2]
+        v = pz_base + i * d[2]
         zs.append(float(sim_like.cell_center[2]) if (math.isinf(v) or math.isnan(v)) else v)
     return xs, ys, np.array(zs)
+
+
+def build_coords(delta: float, origin: float, gv_offset: int, n: int) -> np.ndarray:
+    """Uniform-grid coordinates origin + (i + gv_offset - 2) delta, i = 1..n
+    (src/Geometry.jl:369-376)."""
+    return np.array([origin + (i + gv_offset - 2) * delta for i in range(1, n + 1)])

@@ -24,7 +24,7 @@ import numpy as np
 
 from . import grid as G
 from .boundaries import init_boundaries
-from .geometry import rasterize
+from .geometry import init_polarization, rasterize, rasterize_chi3
 from .monitors import (DFTMonitor, FluxMonitor, FluxMonitorData, auto_decimate, expand_monitors,
                        init_dft_monitor, flux_from_dft)
 from .sources import assemble_source
@@ -48,6 +48,8 @@ class SimSpec:
     cutoff: float
     mode: str
     dft_accum_f64: bool
+    chi3: Optional[np.ndarray] = None   # (Nx, Ny, Nz) or None
+    poles: list = field(default_factory=list)  # geometry.PoleData
 
 
 class Simulation:
@@ -114,6 +116,8 @@ class Simulation:
         """Host setup of ``prepare_simulation!`` (src/Simulation.jl:92-283)."""
         sig = init_boundaries(self)
         eps_inv, mu_inv = rasterize(self, self.geometry)
+        chi3 = rasterize_chi3(self, self.geometry)
+        poles, eps_inv = init_polarization(self, self.geometry, eps_inv)
         srcs = []
         for s in self.sources:
             srcs.extend(assemble_source(self, s))
@@ -124,7 +128,7 @@ class Simulation:
             N=(self.Nx, self.Ny, self.Nz), real=self.real, delta=tuple(self.delta_n), dt=self.dt_n,
             sigma=sig, eps_inv=eps_inv, mu_inv=mu_inv, eps_inv_scalar=1.0, mu_inv_scalar=1.0,
             sources=srcs, dfts=dfts, cutoff=self.last_source_time(), mode=self.mode,
-            dft_accum_f64=self.dft_accum_f64)
+            dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles)
 
     def prepare(self):
         if self.is_prepared:

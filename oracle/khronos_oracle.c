@@ -21,6 +21,9 @@
  *   decimation gate ........... src/Kernels/Kernels.jl:465-497
  *   source shut-off ........... src/Kernels/Kernels.jl:27-35
  *   time of a step ............ src/Simulation.jl:22
+ *   chi3 Kerr correction ...... src/Kernels/Dispersive.jl:127-173
+ *   ADE poles, total P ........ src/Kernels/Dispersive.jl:25-117, :186-228
+ *   P subtracted in the update  src/Kernels/Helpers.jl:330-333
  *
  * Build twice: -DREAL=float (liboracle_f32.so) and -DREAL=double
  * (liboracle_f64.so), always with -ffp-contract=off (Julia does not contract
@@ -64,6 +67,14 @@ typedef struct {
 } orc_dft;
 
 typedef struct {
+    /* src/Susceptibility.jl:60-85, cast to the backend type (Dispersive.jl:214-219) */
+    real g1i, g1, w, sw, dc;
+    int drude;
+    real *sigma;         /* (nx, ny, nz) oscillator strength, shared by x/y/z */
+    real *P[3], *Pp[3];  /* (nx+2, ny+2, nz+2) */
+} orc_pole;
+
+typedef struct {
     int nx, ny, nz;
     real dx, dy, dz, dt;
     int dim[NCOMP][3];   /* raw extents incl. ghosts */
@@ -83,6 +94,11 @@ typedef struct {
     int nsrc;
     orc_dft *dft;
     int ndft;
+    int pdim[3];         /* (nx+2, ny+2, nz+2): polarization array extents */
+    real *chi3;          /* (nx, ny, nz) or NULL */
+    real *fPD[3];        /* total polarization, allocated with the first pole */
+    orc_pole *pole;
+    int npole;
 } orc_sim;
 
 static void comp_count(int nx, int ny, int nz, int c, int out[3]) {
@@ -128,6 +144,7 @@ orc_sim *orc_create(int nx, int ny, int nz, double dx, double dy, double dz, dou
     s->cutoff = cutoff;
     s->sources_active = 1;
     s->has_bound = (sb[0] != NULL);
+    s->pdim[0] = nx + 2; s->pdim[1] = ny + 2; s->pdim[2] = (nz < 1 ? 1 : nz) + 2;
     for (int a = 0; a < 3; a++) {
         size_t n = 2 * (size_t)N[a] + 1;
         if (sb[a]) { s->sig[0][a] = zalloc(n); memcpy(s->sig[0][a], sb[a], n * sizeof(real)); }
@@ -170,6 +187,12 @@ void orc_destroy(orc_sim *s) {
         for (int a = 0; a < 3; a++) free(s->sig[e][a]);
     for (int i = 0; i < s->nsrc; i++) { free(s->src[i].amp); free(s->src[i].table); }
     for (int i = 0; i < s->ndft; i++) { free(s->dft[i].freqs); free(s->dft[i].acc); }
+    for (int i = 0; i < s->npole; i++) {
+        free(s->pole[i].sigma);
+        for (int a = 0; a < 3; a++) { free(s->pole[i].P[a]); free(s->pole[i].Pp[a]); }
+    }
+    for (int a = 0; a < 3; a++) free(s->fPD[a]);
+    free(s->chi3); free(s->pole);
     free(s->src); free(s->dft); free(s);
 }
 
@@ -213,6 +236,36 @@ int orc_add_dft(orc_sim *s, int comp, const int *start, const int *dims, int nf,
     size_t n = vol(dims) * (size_t)nf * 2;
     d->acc = calloc(n, acc_f64 ? sizeof(double) : sizeof(real));
     return s->ndft++;
+}
+
+/* chi3 array (nx, ny, nz) painted on the cell centres (src/Geometry.jl:610-636) */
+int orc_set_chi3(orc_sim *s, const real *chi3) {
+    size_t n = (size_t)s->nx * s->ny * (s->nz < 1 ? 1 : s->nz);
+    if (!s->chi3) s->chi3 = zalloc(n);
+    memcpy(s->chi3, chi3, n * sizeof(real));
+    return 0;
+}
+
+/* One Lorentz/Drude pole: c = {gamma1_inv, gamma1, omega0_dt_sq,
+ * sigma_omega0_dt_sq, drude_coeff} as compute_ade_coefficients returns them
+ * (src/Susceptibility.jl:74-85), sigma (nx, ny, nz).  The first pole also
+ * allocates the total polarization fPD (src/Geometry.jl:1259-1267). */
+int orc_add_pole(orc_sim *s, const double *c, int is_drude, const real *sigma) {
+    s->pole = (orc_pole *)realloc(s->pole, (size_t)(s->npole + 1) * sizeof(orc_pole));
+    orc_pole *q = &s->pole[s->npole];
+    memset(q, 0, sizeof(*q));
+    q->g1i = (real)c[0]; q->g1 = (real)c[1]; q->w = (real)c[2]; q->sw = (real)c[3]; q->dc = (real)c[4];
+    q->drude = is_drude;
+    size_t n = (size_t)s->nx * s->ny * (s->nz < 1 ? 1 : s->nz);
+    q->sigma = zalloc(n);
+    memcpy(q->sigma, sigma, n * sizeof(real));
+    size_t np = vol(s->pdim);
+    for (int a = 0; a < 3; a++) {
+        q->P[a] = zalloc(np);
+        q->Pp[a] = zalloc(np);
+        if (!s->fPD[a]) s->fPD[a] = zalloc(np);
+    }
+    return s->npole++;
 }
 
 /* ---------------------------------------------------------------- time */
@@ -389,6 +442,7 @@ static void half_step(orc_sim *s, int electric) {
                     real net = s->T[c][f];
                     real *S = s->sources_active ? s->S[c] : NULL;
                     if (S) net += S[f];
+                    if (electric && s->fPD[a]) net -= s->fPD[a][rix(s->pdim, ix + 1, iy + 1, iz + 1)];
                     if (S) S[f] = 0;
                     int hs = sg[a] != NULL;
                     real so = hs ? sig_at(sg[a], gi[a]) : 0;
@@ -399,6 +453,61 @@ static void half_step(orc_sim *s, int electric) {
                     } else {
                         s->F[c][f] = m * net;
                     }
+                }
+            }
+}
+
+/* _chi3_correction_kernel! (src/Kernels/Dispersive.jl:137-157) */
+static void chi3_step(orc_sim *s) {
+    int N[3] = {s->nx, s->ny, s->nz < 1 ? 1 : s->nz};
+#pragma omp parallel for schedule(static)
+    for (int iz = 1; iz <= N[2]; iz++)
+        for (int iy = 1; iy <= N[1]; iy++)
+            for (int ix = 1; ix <= N[0]; ix++) {
+                real c3 = s->chi3[(size_t)(ix - 1) + (size_t)N[0] * ((size_t)(iy - 1) + (size_t)N[1] * (size_t)(iz - 1))];
+                if (c3 == (real)0) continue;
+                size_t f[3];
+                real e[3];
+                for (int a = 0; a < 3; a++) {
+                    f[a] = rix(s->dim[a], ix + 1, iy + 1, iz + 1);
+                    e[a] = s->F[a][f[a]];
+                }
+                real esq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                real corr = (real)1 / ((real)1 + c3 * esq);
+                for (int a = 0; a < 3; a++) s->F[a][f[a]] = e[a] * corr;
+            }
+}
+
+/* step_polarization! (src/Kernels/Dispersive.jl:186-228): fPD <- 0, then per
+ * pole the ADE update (:25-90) and fPD += P (:97-105). */
+static void polarization_step(orc_sim *s) {
+    int N[3] = {s->nx, s->ny, s->nz < 1 ? 1 : s->nz};
+#pragma omp parallel for schedule(static)
+    for (int iz = 1; iz <= N[2]; iz++)
+        for (int iy = 1; iy <= N[1]; iy++)
+            for (int ix = 1; ix <= N[0]; ix++) {
+                size_t g = (size_t)(ix - 1) + (size_t)N[0] * ((size_t)(iy - 1) + (size_t)N[1] * (size_t)(iz - 1));
+                size_t p = rix(s->pdim, ix + 1, iy + 1, iz + 1);
+                for (int a = 0; a < 3; a++) s->fPD[a][p] = (real)0;
+                for (int k = 0; k < s->npole; k++) {
+                    orc_pole *q = &s->pole[k];
+                    real sg = q->sigma[g];
+                    if (sg != (real)0) {
+                        for (int a = 0; a < 3; a++) {
+                            real e = s->F[a][rix(s->dim[a], ix + 1, iy + 1, iz + 1)];
+                            real pc = q->P[a][p];
+#ifdef ORC_PLANT_ADE_INDEX
+                            /* planted mistake for the test suite: P in place of P_prev */
+                            q->Pp[a][p] = pc;
+#endif
+                            if (q->drude)
+                                q->P[a][p] = q->g1i * (((real)2 * pc - q->g1 * q->Pp[a][p]) + (q->dc * sg) * e);
+                            else
+                                q->P[a][p] = q->g1i * ((((real)2 - q->w) * pc - q->g1 * q->Pp[a][p]) + (q->sw * sg) * e);
+                            q->Pp[a][p] = pc;
+                        }
+                    }
+                    for (int a = 0; a < 3; a++) s->fPD[a][p] += q->P[a][p];
                 }
             }
 }
@@ -450,6 +559,8 @@ int orc_step(orc_sim *s, int64_t n) {
         update_dfts(s, 1, t);
         if (s->sources_active) step_sources(s, 0, th);
         half_step(s, 1);
+        if (s->chi3) chi3_step(s);
+        if (s->npole) polarization_step(s);
         update_dfts(s, 0, th);
         s->timestep++;
         if (g_error) return -1;
@@ -473,6 +584,22 @@ int orc_write(orc_sim *s, int which, int comp, const real *in) {
     real *dst[6] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp]};
     if (!dst[which]) return 1;
     memcpy(dst[which], in, vol(s->dim[comp]) * sizeof(real));
+    return 0;
+}
+
+/* Polarization arrays of a pole, (nx+2, ny+2, nz+2): which 0 P, 1 P_prev;
+ * pole -1 reads the total fPD. */
+int orc_read_pole(const orc_sim *s, int pole, int which, int a, real *out) {
+    const real *src = pole < 0 ? s->fPD[a] : (which ? s->pole[pole].Pp[a] : s->pole[pole].P[a]);
+    if (!src) return 1;
+    memcpy(out, src, vol(s->pdim) * sizeof(real));
+    return 0;
+}
+
+int orc_write_pole(orc_sim *s, int pole, int which, int a, const real *in) {
+    real *dst = pole < 0 ? s->fPD[a] : (which ? s->pole[pole].Pp[a] : s->pole[pole].P[a]);
+    if (!dst) return 1;
+    memcpy(dst, in, vol(s->pdim) * sizeof(real));
     return 0;
 }
 

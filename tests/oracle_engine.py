@@ -52,6 +52,11 @@ def load_path(real, path):
     lib.orc_write.argtypes = [P, C.c_int, C.c_int, P]
     lib.orc_dims.argtypes = [P, C.c_int, P]
     lib.orc_read_dft.argtypes = [P, C.c_int, P]
+    lib.orc_set_chi3.argtypes = [P, P]
+    lib.orc_add_pole.restype = C.c_int
+    lib.orc_add_pole.argtypes = [P, P, C.c_int, P]
+    lib.orc_read_pole.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
+    lib.orc_write_pole.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
     lib.orc_real_size.restype = C.c_int
     assert lib.orc_real_size() == np.dtype(real).itemsize
     return lib
@@ -118,6 +123,13 @@ class OracleEngine:
             did = self.lib.orc_add_dft(self.sim, component_code(md.component), _p(st), _p(dm), len(fr), _p(fr),
                                        int(md.decimation), 1 if spec.dft_accum_f64 else 0)
             self.dft_ids.append(did)
+        if getattr(spec, "chi3", None) is not None:
+            self.lib.orc_set_chi3(self.sim, arr(spec.chi3, "F"))
+        self.pole_ids = []
+        for pd in getattr(spec, "poles", []) or []:
+            c = np.ascontiguousarray(pd.coeffs.as_array(), dtype=np.float64)
+            self.pole_ids.append(self.lib.orc_add_pole(self.sim, _p(c), 1 if pd.coeffs.is_drude else 0,
+                                                       arr(pd.sigma, "F")))
         self.ts = 0
 
     def __del__(self):
@@ -170,6 +182,20 @@ class OracleEngine:
 
     def write_field(self, code, arr):
         self.write(0, code, arr)
+
+    def pole_shape(self):
+        return tuple(n + 2 for n in self.spec.N)
+
+    def read_pole(self, pole, which, a):
+        """P (which 0) or P_prev (which 1) of a pole, or the total fPD (pole -1);
+        (Nx+2, Ny+2, Nz+2)."""
+        out = np.zeros(self.pole_shape(), dtype=self.R, order="F")
+        self.lib.orc_read_pole(self.sim, pole, which, a, _p(out))
+        return out
+
+    def write_pole(self, pole, which, a, arr):
+        h = np.asfortranarray(arr, dtype=self.R)
+        return self.lib.orc_write_pole(self.sim, pole, which, a, _p(h))
 
     def read_dft(self, i):
         md = self.spec.dfts[i]
