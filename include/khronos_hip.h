@@ -122,6 +122,38 @@ int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3],
                const int32_t dims[3], int32_t n_freq, const double *freqs,
                int32_t decimation, int32_t *dft_id);
 
+/* Kerr coefficient per cell, (nx, ny, nz) column-major, backend precision,
+ * painted on the cell centres (src/Geometry.jl:610-636).  After every E update
+ * each cell with chi3 != 0 gets E <- E / (1 + chi3 (Ex^2 + Ey^2 + Ez^2)) at the
+ * same grid index, before the polarization update and the E monitors.
+ * Before the first kh_step.  Replaces: _chi3_correction_kernel! /
+ * step_chi3_correction! (src/Kernels/Dispersive.jl:127-173). */
+int kh_set_chi3(kh_ctx *ctx, const void *chi3);
+
+/* One Lorentz or Drude pole.  coeffs = {gamma1_inv, gamma1, omega0_dt_sq,
+ * sigma_omega0_dt_sq, drude_coeff} as compute_ade_coefficients returns them
+ * (src/Susceptibility.jl:67-85), is_drude = (omega_0 == 0); sigma is the
+ * oscillator strength per cell, (nx, ny, nz) column-major, backend precision,
+ * already zeroed inside the PML (src/Geometry.jl:1180-1222).  The library
+ * keeps P and P_prev over the bounding box of sigma != 0; after every E update
+ * (and chi3) P <- gamma1_inv (c P - gamma1 P_prev + k sigma E), and the E
+ * update subtracts the sum of all poles' P.  The chi1 correction of the
+ * inverse permittivity is the caller's (src/Geometry.jl:1224-1350), as in the
+ * reference.  Before the first kh_step.  Replaces: init_polarization!
+ * uploads and step_polarization! (src/Geometry.jl:1136-1355,
+ * src/Kernels/Dispersive.jl:25-117, :186-228). */
+int kh_add_pole(kh_ctx *ctx, const double coeffs[5], int32_t is_drude, const void *sigma,
+                int32_t *pole_id);
+
+/* P (which = 0) or P_prev (which = 1) of component 0..2 of a pole, in the
+ * reference's polarization layout (nx+2, ny+2, nz+2), column-major, backend
+ * precision; zero outside the pole's box.  Synchronous.  Replaces:
+ * Array(pole.Px) etc. (src/DataStructures.jl:288-304). */
+int kh_read_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component, void *host,
+                 size_t bytes);
+int kh_write_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component,
+                  const void *host, size_t bytes);
+
 /* Advance n_steps time steps in the reference's sub-step order
  * (H sources, H update, H monitors, E sources, E update, E monitors).
  * Asynchronous with respect to the host.  Replaces: step! (src/Kernels/Kernels.jl:20-88). */
@@ -167,8 +199,9 @@ int kh_reset(kh_ctx *ctx);
  * event pairs are created up front so that timing allocates nothing while it
  * runs.  kh_get_timing writes {total ms, launch count} for each of the
  * KH_TIMING_SLOTS kernel classes -- H inner box, H PML/full cascade, E inner
- * box, E PML/full cascade, DFT -- and clears the counters. */
-#define KH_TIMING_SLOTS 5
+ * box, E PML/full cascade, DFT, material (chi3 + poles) -- and clears the
+ * counters. */
+#define KH_TIMING_SLOTS 6
 int kh_set_timing(kh_ctx *ctx, int32_t enable);
 int kh_get_timing(kh_ctx *ctx, double out[2 * KH_TIMING_SLOTS]);
 

@@ -84,6 +84,42 @@ struct CascadeArgs {
     int per_xcd;
     int nsrc;
     SrcL<T> src[MAX_SRC];
+    // total polarization subtracted in the material stage (E, EXACT mode):
+    // dense over the material box ulo..ulo+un-1, or null
+    const T *fpd[3];
+    int ulo[3], un[3];
+};
+
+// One dispersive pole on the device: coefficients in the backend type and
+// its arrays, dense over the pole's box (x fastest).
+template <typename T>
+struct PoleDev {
+    T g1i, g1;        // gamma1_inv, gamma1
+    T coefp;          // 2 - omega0_dt_sq (Lorentz) or 2 (Drude)
+    T drive;          // sigma_omega0_dt_sq (Lorentz) or drude_coeff (Drude)
+    int lo[3], n[3];  // box: first grid index and extent
+    const T *sigma;
+    T *P[3];
+    T *Pp[3];
+};
+
+// Material pass after the E update: chi3 and the poles, over the material box
+// (bounding box of chi3 != 0 and every pole's box), one cell per thread.
+template <typename T>
+struct MatArgs {
+    T *E[3];
+    T *W[3];              // E-side own-axis stage (FAST correction in the PML)
+    const T *minv[3];
+    T minv_s;
+    const T *sig[3];      // sampled D-side PML sigma per axis (N+2)
+    const T *chi3;        // dense over the box, or null
+    T *delta[3];          // FAST with chi3: linear minus stored E per cell, or null
+    T *fcur[3];           // total polarization the update used (null without poles)
+    T *fnext[3];          // FAST: previous total on entry; new total on exit.  EXACT: == fcur
+    int lo[3], n[3];
+    int64_t sy, sz;
+    int npole;
+    const PoleDev<T> *poles;
 };
 
 template <typename T>
@@ -205,6 +241,12 @@ __device__ __forceinline__ void cascade_component(const CascadeArgs<T> &p, int a
         bool inside;
         const T s = source_value(p, a, gx, gy, gz, inside);
         if (inside) net = net + s;
+    }
+    if (p.fpd[0]) {
+        // net -= fPD (src/Kernels/Helpers.jl:330-333); fPD is zero outside the box
+        const int ux = gx - p.ulo[0], uy = gy - p.ulo[1], uz = gz - p.ulo[2];
+        if (ux >= 0 && uy >= 0 && uz >= 0 && ux < p.un[0] && uy < p.un[1] && uz < p.un[2])
+            net = net - p.fpd[a][(size_t)ux + (size_t)p.un[0] * ((size_t)uy + (size_t)p.un[1] * (size_t)uz)];
     }
     if (so == T(0)) {
         st1(p.A[a], idx, m * net);
@@ -667,6 +709,96 @@ __global__ __launch_bounds__(256) void source_fix_kernel(T *A, T *W, const T *mi
     } else {
         W[idx] = W[idx] + dw;
         A[idx] = A[idx] + (T(1) + so) * dw;
+    }
+}
+
+// ----------------------------------------------------------- materials
+// After the E update, per cell of the material box (src/Kernels/Kernels.jl:
+// 76-83): [FAST: the terms the D-eliminated update left out], chi3
+// (src/Kernels/Dispersive.jl:137-157), then every pole's ADE update and the
+// new total polarization (:25-105).
+//
+// FAST mode carries D on E (or on W where the own-axis sigma is non-zero).
+// With fPD the reference computes E = m (D - fPD) (W likewise); advancing the
+// carried value leaves out  dw = m ((1 - s_p)/(1 + s_p) fPD_prev - fPD_cur),
+// added as A += dw (s_o == 0) or W += dw, A += (1 + s_o) dw -- the
+// source-correction form with S replaced by -fPD.  chi3 rescales the stored E
+// but not D: where E carries D (s_o == 0) the next step's carried value is
+// short by (1 - s_p)/(1 + s_p) delta, delta = E_linear - E_stored.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncell = (int64_t)p.n[0] * p.n[1] * p.n[2];
+    if (c >= ncell) return;
+    const int ux = (int)(c % p.n[0]);
+    const int64_t r = c / p.n[0];
+    const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
+    const int g[3] = {p.lo[0] + ux, p.lo[1] + uy, p.lo[2] + uz};
+    const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+    T e[3], so[3];
+    for (int a = 0; a < 3; ++a) e[a] = p.E[a][idx];
+    bool dirty = false;
+    if constexpr (FAST) {
+        for (int a = 0; a < 3; ++a) {
+            const int prv = (a + 2) % 3;
+            so[a] = p.sig[a][g[a]];
+            const T sp = p.sig[prv][g[prv]];
+            const T ratio = (T(1) - sp) / (T(1) + sp);
+            if (p.delta[0] && so[a] == T(0)) e[a] = e[a] + ratio * p.delta[a][c];
+            if (p.npole) {
+                const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
+                const T dw = m * (ratio * p.fnext[a][c] - p.fcur[a][c]);
+                if (so[a] == T(0)) {
+                    e[a] = e[a] + dw;
+                } else {
+                    p.W[a][idx] = p.W[a][idx] + dw;
+                    e[a] = e[a] + (T(1) + so[a]) * dw;
+                }
+            }
+        }
+        dirty = p.delta[0] || p.npole;
+    }
+    if (p.chi3) {
+        const T c3 = p.chi3[c];
+        if (c3 != T(0)) {
+            const T esq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+            const T corr = T(1) / (T(1) + c3 * esq);
+            for (int a = 0; a < 3; ++a) {
+                const T en = e[a] * corr;
+                if constexpr (FAST) {
+                    if (p.delta[0]) p.delta[a][c] = so[a] == T(0) ? e[a] - en : T(0);
+                }
+                e[a] = en;
+            }
+            dirty = true;
+        }
+    }
+    if (dirty)
+        for (int a = 0; a < 3; ++a) p.E[a][idx] = e[a];
+    if (p.npole) {
+        T sum[3] = {T(0), T(0), T(0)};
+        for (int k = 0; k < p.npole; ++k) {
+            const PoleDev<T> &q = p.poles[k];
+            const int kx = g[0] - q.lo[0], ky = g[1] - q.lo[1], kz = g[2] - q.lo[2];
+            if (kx < 0 || ky < 0 || kz < 0 || kx >= q.n[0] || ky >= q.n[1] || kz >= q.n[2]) {
+                // outside the pole's box its P is zero: fPD += 0
+                for (int a = 0; a < 3; ++a) sum[a] = sum[a] + T(0);
+                continue;
+            }
+            const size_t kc = (size_t)kx + (size_t)q.n[0] * ((size_t)ky + (size_t)q.n[1] * (size_t)kz);
+            const T s = q.sigma[kc];
+            for (int a = 0; a < 3; ++a) {
+                T pn = q.P[a][kc];
+                if (s != T(0)) {
+                    const T pc = pn;
+                    pn = q.g1i * ((q.coefp * pc - q.g1 * q.Pp[a][kc]) + (q.drive * s) * e[a]);
+                    q.P[a][kc] = pn;
+                    q.Pp[a][kc] = pc;
+                }
+                sum[a] = sum[a] + pn;
+            }
+        }
+        for (int a = 0; a < 3; ++a) p.fnext[a][c] = sum[a];
     }
 }
 

@@ -106,6 +106,10 @@ struct CtxBase {
     virtual void add_source(int comp, const int32_t *s, const int32_t *d, const void *amp, int32_t *id) = 0;
     virtual void set_waveform(int id, int64_t n, const double *v) = 0;
     virtual void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) = 0;
+    virtual void set_chi3(const void *chi3) = 0;
+    virtual void add_pole(const double *c, int is_drude, const void *sigma, int32_t *id) = 0;
+    virtual void read_pole(int id, int which, int comp, void *host, size_t bytes) = 0;
+    virtual void write_pole(int id, int which, int comp, const void *host, size_t bytes) = 0;
     virtual void step(int64_t n) = 0;
     virtual void sync() = 0;
     virtual int64_t timestep() const = 0;
@@ -156,6 +160,18 @@ struct Ctx final : CtxBase {
         int64_t ncell;
     };
     std::vector<Dft> dfts;
+
+    // chi3 and dispersive poles (material pass after the E update)
+    std::vector<T> chi3_host;                  // (nx, ny, nz) until the first step
+    std::vector<kh::PoleDev<T>> poles;
+    kh::PoleDev<T> *poles_dev = nullptr;
+    T *chi3_dev = nullptr;                     // dense over the material box
+    T *fpd[2][3] = {};                         // total polarization (FAST: two, alternating)
+    int fcur = 0;
+    T *delta[3] = {};                          // FAST with chi3
+    bool mat = false;
+    int mlo[3] = {1, 1, 1}, mn[3] = {0, 0, 0};
+    int64_t mcells = 0;
 
     double *scratch = nullptr;  // reduction partials
     static constexpr int NPART = 256;
@@ -309,6 +325,178 @@ struct Ctx final : CtxBase {
         *id = (int32_t)dfts.size() - 1;
     }
 
+    // bounding box (grid indices) of the non-zero entries of an (nx, ny, nz) array
+    bool nonzero_box(const T *a, int lo[3], int n[3]) const {
+        int hi[3] = {0, 0, 0};
+        lo[0] = lo[1] = lo[2] = 1 << 30;
+        for (int k = 1; k <= g.n[2]; ++k)
+            for (int j = 1; j <= g.n[1]; ++j)
+                for (int i = 1; i <= g.n[0]; ++i)
+                    if (a[(size_t)(i - 1) + (size_t)g.n[0] * ((size_t)(j - 1) + (size_t)g.n[1] * (k - 1))] != T(0)) {
+                        lo[0] = std::min(lo[0], i); hi[0] = std::max(hi[0], i);
+                        lo[1] = std::min(lo[1], j); hi[1] = std::max(hi[1], j);
+                        lo[2] = std::min(lo[2], k); hi[2] = std::max(hi[2], k);
+                    }
+        if (hi[0] == 0) { lo[0] = lo[1] = lo[2] = 1; n[0] = n[1] = n[2] = 0; return false; }
+        for (int a3 = 0; a3 < 3; ++a3) n[a3] = hi[a3] - lo[a3] + 1;
+        return true;
+    }
+
+    // copy box lo..lo+n-1 of an (nx, ny, nz) array to a dense device array
+    T *upload_box(const T *a, const int lo[3], const int n[3]) {
+        std::vector<T> h((size_t)n[0] * n[1] * n[2]);
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    h[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * k)] =
+                        a[(size_t)(lo[0] + i - 1) + (size_t)g.n[0] * ((size_t)(lo[1] + j - 1) + (size_t)g.n[1] * (lo[2] + k - 1))];
+        T *dv = (T *)dalloc(h.size() * sizeof(T));
+        HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return dv;
+    }
+
+    void set_chi3(const void *chi3) override {
+        require(!stepped, "chi3 must be set before the first step", KH_E_STATE);
+        require(chi3 != nullptr, "chi3 array is null");
+        const T *c = (const T *)chi3;
+        chi3_host.assign(c, c + (size_t)g.n[0] * g.n[1] * g.n[2]);
+    }
+
+    void add_pole(const double *c, int is_drude, const void *sigma, int32_t *id) override {
+        require(!stepped, "poles must be added before the first step", KH_E_STATE);
+        require(c != nullptr && sigma != nullptr, "pole coefficients or sigma array is null");
+        kh::PoleDev<T> q{};
+        q.g1i = (T)c[0];
+        q.g1 = (T)c[1];
+        // the reference's Lorentz factor is 2 - omega0_dt_sq in the backend
+        // type; its Drude factor is the literal 2 (src/Kernels/Dispersive.jl:40-76)
+        q.coefp = is_drude ? T(2) : T(2) - (T)c[2];
+        q.drive = is_drude ? (T)c[4] : (T)c[3];
+        const T *sg = (const T *)sigma;
+        if (nonzero_box(sg, q.lo, q.n)) {
+            q.sigma = upload_box(sg, q.lo, q.n);
+            const size_t nb = (size_t)q.n[0] * q.n[1] * q.n[2] * sizeof(T);
+            for (int a = 0; a < 3; ++a) {
+                q.P[a] = (T *)dalloc(nb);
+                q.Pp[a] = (T *)dalloc(nb);
+            }
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        poles.push_back(q);
+        *id = (int32_t)poles.size() - 1;
+    }
+
+    void pole_io(int id, int which, int comp, void *host, size_t bytes, bool rd) {
+        require(id >= 0 && id < (int)poles.size(), "bad pole id");
+        require(which == 0 || which == 1, "which must be 0 (P) or 1 (P_prev)");
+        require(comp >= 0 && comp < 3, "polarization component must be 0..2");
+        const int r[3] = {g.n[0] + 2, g.n[1] + 2, g.n[2] + 2};
+        require(bytes == (size_t)r[0] * r[1] * r[2] * sizeof(T), "polarization buffer size mismatch");
+        kh::PoleDev<T> &q = poles[id];
+        T *dv = which ? q.Pp[comp] : q.P[comp];
+        std::vector<T> h((size_t)q.n[0] * q.n[1] * q.n[2]);
+        if (rd) {
+            T *o = (T *)host;
+            std::fill(o, o + (size_t)r[0] * r[1] * r[2], T(0));
+            if (h.empty()) return;
+            HIPCHK(hipMemcpyAsync(h.data(), dv, h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        const T *in = (const T *)host;
+        for (int k = 0; k < q.n[2]; ++k)
+            for (int j = 0; j < q.n[1]; ++j)
+                for (int i = 0; i < q.n[0]; ++i) {
+                    const size_t hc = (size_t)(q.lo[0] + i) + (size_t)r[0] * ((size_t)(q.lo[1] + j) + (size_t)r[1] * (q.lo[2] + k));
+                    const size_t bc = (size_t)i + (size_t)q.n[0] * ((size_t)j + (size_t)q.n[1] * k);
+                    if (rd) ((T *)host)[hc] = h[bc];
+                    else h[bc] = in[hc];
+                }
+        if (!rd && !h.empty()) {
+            HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    }
+    void read_pole(int id, int which, int comp, void *host, size_t bytes) override {
+        pole_io(id, which, comp, host, bytes, true);
+    }
+    void write_pole(int id, int which, int comp, const void *host, size_t bytes) override {
+        pole_io(id, which, comp, const_cast<void *>(host), bytes, false);
+    }
+
+    // Material box = bounding box of chi3 != 0 and every pole's box; its
+    // arrays are allocated on the first step.
+    void plan_materials() {
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {0, 0, 0};
+        auto take = [&](const int l[3], const int n[3]) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], l[a]);
+                hi[a] = std::max(hi[a], l[a] + n[a] - 1);
+            }
+        };
+        int clo[3], cn[3];
+        bool has_chi3 = !chi3_host.empty() && nonzero_box(chi3_host.data(), clo, cn);
+        if (has_chi3) take(clo, cn);
+        bool has_pole = false;
+        for (auto &q : poles)
+            if (q.n[0] > 0) { take(q.lo, q.n); has_pole = true; }
+        mat = has_chi3 || has_pole;
+        if (!mat) return;
+        for (int a = 0; a < 3; ++a) { mlo[a] = lo[a]; mn[a] = hi[a] - lo[a] + 1; }
+        mcells = (int64_t)mn[0] * mn[1] * mn[2];
+        if (has_chi3) chi3_dev = upload_box(chi3_host.data(), mlo, mn);
+        chi3_host.clear();
+        chi3_host.shrink_to_fit();
+        const size_t nb = (size_t)mcells * sizeof(T);
+        if (has_pole) {
+            for (int a = 0; a < 3; ++a) {
+                fpd[0][a] = (T *)dalloc(nb);
+                if (mode == KH_MODE_FAST) fpd[1][a] = (T *)dalloc(nb);
+            }
+            poles_dev = (kh::PoleDev<T> *)dalloc(poles.size() * sizeof(kh::PoleDev<T>));
+            HIPCHK(hipMemcpyAsync(poles_dev, poles.data(), poles.size() * sizeof(kh::PoleDev<T>),
+                                  hipMemcpyHostToDevice, st));
+        }
+        if (has_chi3 && mode == KH_MODE_FAST)
+            for (int a = 0; a < 3; ++a) delta[a] = (T *)dalloc(nb);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void launch_material() {
+        kh::MatArgs<T> q{};
+        for (int a = 0; a < 3; ++a) {
+            q.E[a] = E[a];
+            q.W[a] = WD[a];
+            q.minv[a] = epsi[a];
+            q.sig[a] = sigD[a];
+            q.delta[a] = delta[a];
+            q.fcur[a] = fpd[fcur][a];
+            q.fnext[a] = mode == KH_MODE_FAST ? fpd[1 - fcur][a] : fpd[0][a];
+            q.lo[a] = mlo[a];
+            q.n[a] = mn[a];
+        }
+        q.minv_s = eps_s;
+        q.chi3 = chi3_dev;
+        q.sy = g.sy();
+        q.sz = g.sz();
+        q.npole = fpd[0][0] ? (int)poles.size() : 0;
+        q.poles = poles_dev;
+        EvPair ev{};
+        if (timing) {
+            ev = take_event(5);
+            HIPCHK(hipEventRecord(ev.a, st));
+        }
+        const dim3 grid((unsigned)((mcells + 255) / 256)), blk(256);
+        if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true>), grid, blk, 0, st, q);
+        else hipLaunchKernelGGL((kh::material_kernel<T, false>), grid, blk, 0, st, q);
+        HIPCHK(hipGetLastError());
+        if (timing) {
+            HIPCHK(hipEventRecord(ev.b, st));
+            evs.push_back(ev);
+        }
+        if (mode == KH_MODE_FAST && q.npole) fcur = 1 - fcur;
+    }
+
     double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22
 
     // ------------------------------------------------------------ launch plans
@@ -440,6 +628,7 @@ struct Ctx final : CtxBase {
                             "per-voxel materials must be given for all three components or none");
             }
         }
+        plan_materials();
         planned = true;
     }
 
@@ -464,6 +653,13 @@ struct Ctx final : CtxBase {
         for (int s = 0; s < P.nslab; ++s) p.slab[s] = P.slab[s];
         p.nblock = P.cblocks;
         p.per_xcd = (P.cblocks + kh::NXCD - 1) / kh::NXCD;
+        if (e && fpd[0][0]) {
+            for (int a = 0; a < 3; ++a) {
+                p.fpd[a] = fpd[0][a];
+                p.ulo[a] = mlo[a];
+                p.un[a] = mn[a];
+            }
+        }
         return p;
     }
 
@@ -632,6 +828,7 @@ struct Ctx final : CtxBase {
             launch_half(0);
             launch_dfts(false, t);
             launch_half(1);
+            if (mat) launch_material();
             launch_dfts(true, th);
             last_active = active;
             ++ts;
@@ -680,6 +877,8 @@ struct Ctx final : CtxBase {
                 for (int i = 1; i <= g.n[0]; ++i)
                     h[g.lin(i, j, k)] = in[(size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k)];
         HIPCHK(hipMemcpyAsync(field_ptr(comp), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        // a written E is taken as the linear field: no chi3 deficit carried over
+        if (comp < 3 && delta[comp]) HIPCHK(hipMemsetAsync(delta[comp], 0, (size_t)mcells * sizeof(T), st));
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -760,10 +959,29 @@ struct Ctx final : CtxBase {
         }
         for (auto &m : dfts)
             HIPCHK(hipMemsetAsync(m.acc, 0, (size_t)m.ncell * m.nf * 2 * (acc64 ? sizeof(double) : sizeof(T)), st));
+        for (auto &q : poles) {
+            const size_t nb = (size_t)q.n[0] * q.n[1] * q.n[2] * sizeof(T);
+            for (int a = 0; a < 3; ++a)
+                if (q.P[a]) {
+                    HIPCHK(hipMemsetAsync(q.P[a], 0, nb, st));
+                    HIPCHK(hipMemsetAsync(q.Pp[a], 0, nb, st));
+                }
+        }
+        clear_material_state();
         HIPCHK(hipStreamSynchronize(st));
         ts = 0;
         active = true;
         last_active = false;
+    }
+
+    void clear_material_state() {
+        const size_t nb = (size_t)mcells * sizeof(T);
+        for (int a = 0; a < 3; ++a) {
+            for (int b = 0; b < 2; ++b)
+                if (fpd[b][a]) HIPCHK(hipMemsetAsync(fpd[b][a], 0, nb, st));
+            if (delta[a]) HIPCHK(hipMemsetAsync(delta[a], 0, nb, st));
+        }
+        fcur = 0;
     }
 
     void set_timing(int n) override {
@@ -880,6 +1098,38 @@ int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3], const
         CTX_OK;
         require(start_idx && dims && freqs && dft_id, "null argument");
         ctx->impl->add_dft(component, start_idx, dims, n_freq, freqs, decimation, dft_id);
+    });
+}
+
+int kh_set_chi3(kh_ctx *ctx, const void *chi3) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->set_chi3(chi3);
+    });
+}
+
+int kh_add_pole(kh_ctx *ctx, const double coeffs[5], int32_t is_drude, const void *sigma, int32_t *pole_id) {
+    return guarded([&] {
+        CTX_OK;
+        require(pole_id != nullptr, "pole_id is null");
+        ctx->impl->add_pole(coeffs, is_drude, sigma, pole_id);
+    });
+}
+
+int kh_read_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component, void *host, size_t bytes) {
+    return guarded([&] {
+        CTX_OK;
+        require(host != nullptr, "host buffer is null");
+        ctx->impl->read_pole(pole_id, which, component, host, bytes);
+    });
+}
+
+int kh_write_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component, const void *host,
+                  size_t bytes) {
+    return guarded([&] {
+        CTX_OK;
+        require(host != nullptr, "host buffer is null");
+        ctx->impl->write_pole(pole_id, which, component, host, bytes);
     });
 }
 

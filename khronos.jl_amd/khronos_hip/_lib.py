@@ -40,6 +40,10 @@ SIGNATURES = {
     "kh_set_waveform": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p]),
     "kh_add_dft": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                              C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "kh_set_chi3": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kh_add_pole": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "kh_read_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),
+    "kh_write_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),
     "kh_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "kh_sync": (C.c_int, [C.c_void_p]),
     "kh_get_timestep": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),

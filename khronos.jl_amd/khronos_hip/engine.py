@@ -80,6 +80,18 @@ class HipEngine:
             _lib.check(self.lib.kh_add_dft(self.ctx, _code(md.component), st, dm, len(fr), _ptr(fr),
                                            int(md.decimation), C.byref(did)))
             self.dft_ids.append(did.value)
+        # Kerr coefficient and dispersive poles
+        if getattr(spec, "chi3", None) is not None:
+            c3 = np.asfortranarray(spec.chi3, dtype=R)
+            _lib.check(self.lib.kh_set_chi3(self.ctx, _ptr(c3)))
+        self.pole_ids = []
+        for pd in getattr(spec, "poles", []) or []:
+            cf = np.ascontiguousarray(pd.coeffs.as_array(), dtype=np.float64)
+            sg = np.asfortranarray(pd.sigma, dtype=R)
+            pid = C.c_int32()
+            _lib.check(self.lib.kh_add_pole(self.ctx, _ptr(cf), 1 if pd.coeffs.is_drude else 0, _ptr(sg),
+                                            C.byref(pid)))
+            self.pole_ids.append(pid.value)
         self.ts = 0
 
     def __del__(self):
@@ -157,6 +169,19 @@ class HipEngine:
         assert a.shape == self.field_shape(code)
         _lib.check(self.lib.kh_write_field(self.ctx, code, _ptr(a), a.nbytes))
 
+    def pole_shape(self):
+        return tuple(n + 2 for n in self.spec.N)
+
+    def read_pole(self, pole: int, which: int, a: int) -> np.ndarray:
+        """P (which 0) or P_prev (which 1) of a pole, (Nx+2, Ny+2, Nz+2)."""
+        out = np.zeros(self.pole_shape(), dtype=self.R, order="F")
+        _lib.check(self.lib.kh_read_pole(self.ctx, self.pole_ids[pole], which, a, _ptr(out), out.nbytes))
+        return out
+
+    def write_pole(self, pole: int, which: int, a: int, arr: np.ndarray):
+        h = np.asfortranarray(arr, dtype=self.R)
+        _lib.check(self.lib.kh_write_pole(self.ctx, self.pole_ids[pole], which, a, _ptr(h), h.nbytes))
+
     def read_dft(self, i: int) -> np.ndarray:
         md = self.spec.dfts[i]
         shp = tuple(md.gv.dims) + (len(md.frequencies),)
@@ -188,7 +213,7 @@ class HipEngine:
     def set_timing(self, on: bool):
         _lib.check(self.lib.kh_set_timing(self.ctx, 1 if on else 0))
 
-    TIMING_SLOTS = ("h_inner", "h_full", "e_inner", "e_full", "dft")
+    TIMING_SLOTS = ("h_inner", "h_full", "e_inner", "e_full", "dft", "material")
 
     def get_timing(self):
         """{slot: (total_ms, launches)} for the kernel classes of include/khronos_hip.h."""
