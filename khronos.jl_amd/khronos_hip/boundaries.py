@@ -18,6 +18,77 @@ import math
 import numpy as np
 
 
+class BoundaryCondition:
+    """Boundary condition of one side of one axis (src/DataStructures.jl:154-162).
+    ``zero_pml``: the side's PML thickness is overridden to 0
+    (src/Boundaries.jl:100-109)."""
+
+    zero_pml = False
+
+    def __repr__(self):
+        return type(self).__name__ + "()"
+
+
+class PML(BoundaryCondition):
+    pass
+
+
+class Periodic(BoundaryCondition):
+    zero_pml = True
+
+
+class Bloch(BoundaryCondition):
+    """Bloch-periodic side with wave number ``k`` (the field at x = L is the
+    field at x = 0 times exp(i k L)); any Bloch side makes the fields complex
+    (src/Fields.jl:145-159)."""
+
+    zero_pml = True
+
+    def __init__(self, k: float = 0.0):
+        self.k = float(k)
+
+    def __repr__(self):
+        return f"Bloch(k={self.k})"
+
+
+class PECBoundary(BoundaryCondition):
+    zero_pml = True
+
+
+class PMCBoundary(BoundaryCondition):
+    zero_pml = True
+
+
+def needs_complex_fields(boundary_conditions) -> bool:
+    """Any Bloch side (src/Fields.jl:149-159)."""
+    if boundary_conditions is None:
+        return False
+    return any(isinstance(bc, Bloch) for bcs in boundary_conditions for bc in bcs)
+
+
+def periodic_wrap(sim):
+    """Per axis: (wraps, exp(+i k L), exp(-i k L)) of the single-domain
+    wraparound (src/Chunking.jl:1725-1768).  An axis wraps when both sides are
+    Periodic or Bloch; k is the first Bloch side's; L is the backend-precision
+    cell size; the phases are float64 complex."""
+    out = []
+    bcs_all = sim.boundary_conditions
+    for axis in range(3):
+        bcs = bcs_all[axis] if bcs_all is not None and axis < len(bcs_all) else None
+        if bcs is None or len(bcs) < 2 or not all(isinstance(b, (Periodic, Bloch)) for b in bcs):
+            out.append((False, 1 + 0j, 1 + 0j))
+            continue
+        k = 0.0
+        if isinstance(bcs[0], Bloch):
+            k = bcs[0].k
+        elif isinstance(bcs[1], Bloch):
+            k = bcs[1].k
+        L = float(sim.real(sim.cell_size_n[axis]))
+        x = k * L
+        out.append((True, complex(math.cos(x), math.sin(x)), complex(math.cos(-x), math.sin(-x))))
+    return out
+
+
 def sigma_value(idx: int, n: int, delta, dt, left, right, R=np.float64) -> float:
     """Sample ``idx`` (1-based) of a profile of length ``n``, as float64.
 

@@ -23,7 +23,7 @@ from typing import Callable, List, Optional, Sequence
 import numpy as np
 
 from . import grid as G
-from .boundaries import init_boundaries
+from .boundaries import init_boundaries, needs_complex_fields, periodic_wrap
 from .geometry import init_polarization, rasterize, rasterize_chi3
 from .monitors import (DFTMonitor, FluxMonitor, FluxMonitorData, auto_decimate, expand_monitors,
                        init_dft_monitor, flux_from_dft)
@@ -50,6 +50,10 @@ class SimSpec:
     dft_accum_f64: bool
     chi3: Optional[np.ndarray] = None   # (Nx, Ny, Nz) or None
     poles: list = field(default_factory=list)  # geometry.PoleData
+    complex_fields: bool = False        # any Bloch side
+    periodic: tuple = (False, False, False)
+    bloch_fwd: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(+i k L): lower interior -> upper ghost
+    bloch_rev: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(-i k L): upper interior -> lower ghost
 
 
 class Simulation:
@@ -118,6 +122,7 @@ class Simulation:
         eps_inv, mu_inv = rasterize(self, self.geometry)
         chi3 = rasterize_chi3(self, self.geometry)
         poles, eps_inv = init_polarization(self, self.geometry, eps_inv)
+        wrap = periodic_wrap(self)
         srcs = []
         for s in self.sources:
             srcs.extend(assemble_source(self, s))
@@ -128,7 +133,10 @@ class Simulation:
             N=(self.Nx, self.Ny, self.Nz), real=self.real, delta=tuple(self.delta_n), dt=self.dt_n,
             sigma=sig, eps_inv=eps_inv, mu_inv=mu_inv, eps_inv_scalar=1.0, mu_inv_scalar=1.0,
             sources=srcs, dfts=dfts, cutoff=self.last_source_time(), mode=self.mode,
-            dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles)
+            dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles,
+            complex_fields=needs_complex_fields(self.boundary_conditions),
+            periodic=tuple(w[0] for w in wrap), bloch_fwd=tuple(w[1] for w in wrap),
+            bloch_rev=tuple(w[2] for w in wrap))
 
     def prepare(self):
         if self.is_prepared:

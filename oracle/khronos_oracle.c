@@ -24,6 +24,12 @@
  *   chi3 Kerr correction ...... src/Kernels/Dispersive.jl:127-173
  *   ADE poles, total P ........ src/Kernels/Dispersive.jl:25-117, :186-228
  *   P subtracted in the update  src/Kernels/Helpers.jl:330-333
+ *   periodic / Bloch wrap ..... src/Chunking.jl:1725-1768 (single chunk),
+ *                               :1825-1852, :2138-2214 (copy + phase)
+ *   complex fields (Bloch) .... src/Fields.jl:140-159: every field array is
+ *                               complex; held here as real and imaginary parts
+ *                               (*i arrays), updated by the same real-coefficient
+ *                               operations part by part
  *
  * Build twice: -DREAL=float (liboracle_f32.so) and -DREAL=double
  * (liboracle_f64.so), always with -ffp-contract=off (Julia does not contract
@@ -72,6 +78,8 @@ typedef struct {
     int drude;
     real *sigma;         /* (nx, ny, nz) oscillator strength, shared by x/y/z */
     real *P[3], *Pp[3];  /* (nx+2, ny+2, nz+2) */
+    real *Pi[3], *Ppi[3];/* imaginary parts with complex fields (P complex: the
+                            reference's real P cannot hold a complex E) */
 } orc_pole;
 
 typedef struct {
@@ -99,6 +107,12 @@ typedef struct {
     real *fPD[3];        /* total polarization, allocated with the first pole */
     orc_pole *pole;
     int npole;
+    int cplx;            /* complex fields: imaginary parts below */
+    real *Fi[NCOMP], *Ti[NCOMP], *Ci[NCOMP], *Ui[NCOMP], *Wi[NCOMP], *Si[NCOMP];
+    real *fPDi[3];
+    int periodic[3];     /* axis wraps (Periodic or Bloch on both sides) */
+    double ph_fwd[3][2]; /* lower interior -> upper ghost: exp(+i k L) */
+    double ph_rev[3][2]; /* upper interior -> lower ghost: exp(-i k L) */
 } orc_sim;
 
 static void comp_count(int nx, int ny, int nz, int c, int out[3]) {
@@ -191,7 +205,12 @@ void orc_destroy(orc_sim *s) {
         free(s->pole[i].sigma);
         for (int a = 0; a < 3; a++) { free(s->pole[i].P[a]); free(s->pole[i].Pp[a]); }
     }
-    for (int a = 0; a < 3; a++) free(s->fPD[a]);
+    for (int a = 0; a < 3; a++) { free(s->fPD[a]); free(s->fPDi[a]); }
+    for (int c = 0; c < NCOMP; c++) {
+        free(s->Fi[c]); free(s->Ti[c]); free(s->Ci[c]); free(s->Ui[c]); free(s->Wi[c]); free(s->Si[c]);
+    }
+    for (int i = 0; i < s->npole; i++)
+        for (int a = 0; a < 3; a++) { free(s->pole[i].Pi[a]); free(s->pole[i].Ppi[a]); }
     free(s->chi3); free(s->pole);
     free(s->src); free(s->dft); free(s);
 }
@@ -212,6 +231,7 @@ int orc_add_source(orc_sim *s, int comp, const int *start, const int *dims, cons
     if (p) for (int i = 0; i < 4; i++) o->p[i] = p[i];
     /* the reference allocates a full-size S array for every source component */
     if (!s->S[comp]) s->S[comp] = zalloc(vol(s->dim[comp]));
+    if (s->cplx && !s->Si[comp]) s->Si[comp] = zalloc(vol(s->dim[comp]));
     return s->nsrc++;
 }
 
@@ -236,6 +256,29 @@ int orc_add_dft(orc_sim *s, int comp, const int *start, const int *dims, int nf,
     size_t n = vol(dims) * (size_t)nf * 2;
     d->acc = calloc(n, acc_f64 ? sizeof(double) : sizeof(real));
     return s->ndft++;
+}
+
+/* Complex fields and periodic axes; call right after orc_create.  fwd/rev
+ * hold (re, im) of exp(+i k L) and exp(-i k L) per axis
+ * (src/Chunking.jl:1745-1747). */
+int orc_set_periodic(orc_sim *s, int cplx, const int *periodic, const double *fwd, const double *rev) {
+    if (s->nsrc || s->npole) return 1;
+    s->cplx = cplx;
+    for (int a = 0; a < 3; a++) {
+        s->periodic[a] = periodic[a];
+        s->ph_fwd[a][0] = fwd[2 * a]; s->ph_fwd[a][1] = fwd[2 * a + 1];
+        s->ph_rev[a][0] = rev[2 * a]; s->ph_rev[a][1] = rev[2 * a + 1];
+    }
+    if (cplx)
+        for (int c = 0; c < NCOMP; c++) {
+            size_t n = vol(s->dim[c]);
+            s->Fi[c] = zalloc(n);
+            s->Ti[c] = zalloc(n);
+            if (s->C[c]) s->Ci[c] = zalloc(n);
+            if (s->U[c]) s->Ui[c] = zalloc(n);
+            if (s->W[c]) s->Wi[c] = zalloc(n);
+        }
+    return 0;
 }
 
 /* chi3 array (nx, ny, nz) painted on the cell centres (src/Geometry.jl:610-636) */
@@ -264,6 +307,11 @@ int orc_add_pole(orc_sim *s, const double *c, int is_drude, const real *sigma) {
         q->P[a] = zalloc(np);
         q->Pp[a] = zalloc(np);
         if (!s->fPD[a]) s->fPD[a] = zalloc(np);
+        if (s->cplx) {
+            q->Pi[a] = zalloc(np);
+            q->Ppi[a] = zalloc(np);
+            if (!s->fPDi[a]) s->fPDi[a] = zalloc(np);
+        }
     }
     return s->npole++;
 }
@@ -457,7 +505,8 @@ static void half_step(orc_sim *s, int electric) {
             }
 }
 
-/* _chi3_correction_kernel! (src/Kernels/Dispersive.jl:137-157) */
+/* _chi3_correction_kernel! (src/Kernels/Dispersive.jl:137-157).  With complex
+ * fields real(e * conj(e)) = re*re + im*im exactly, summed x, y, z in order. */
 static void chi3_step(orc_sim *s) {
     int N[3] = {s->nx, s->ny, s->nz < 1 ? 1 : s->nz};
 #pragma omp parallel for schedule(static)
@@ -467,15 +516,34 @@ static void chi3_step(orc_sim *s) {
                 real c3 = s->chi3[(size_t)(ix - 1) + (size_t)N[0] * ((size_t)(iy - 1) + (size_t)N[1] * (size_t)(iz - 1))];
                 if (c3 == (real)0) continue;
                 size_t f[3];
-                real e[3];
+                real e[3], ei[3] = {0, 0, 0}, m[3];
                 for (int a = 0; a < 3; a++) {
                     f[a] = rix(s->dim[a], ix + 1, iy + 1, iz + 1);
                     e[a] = s->F[a][f[a]];
+                    if (s->cplx) ei[a] = s->Fi[a][f[a]];
+                    m[a] = s->cplx ? e[a] * e[a] + ei[a] * ei[a] : e[a] * e[a];
                 }
-                real esq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                real esq = m[0] + m[1] + m[2];
                 real corr = (real)1 / ((real)1 + c3 * esq);
-                for (int a = 0; a < 3; a++) s->F[a][f[a]] = e[a] * corr;
+                for (int a = 0; a < 3; a++) {
+                    s->F[a][f[a]] = e[a] * corr;
+                    if (s->cplx) s->Fi[a][f[a]] = ei[a] * corr;
+                }
             }
+}
+
+/* One ADE update of one part (real or imaginary) of one component. */
+static inline void ade(const orc_pole *q, real sg, real e, real *P, real *Pp, size_t p) {
+    real pc = P[p];
+#ifdef ORC_PLANT_ADE_INDEX
+    /* planted mistake for the test suite: P in place of P_prev */
+    Pp[p] = pc;
+#endif
+    if (q->drude)
+        P[p] = q->g1i * (((real)2 * pc - q->g1 * Pp[p]) + (q->dc * sg) * e);
+    else
+        P[p] = q->g1i * ((((real)2 - q->w) * pc - q->g1 * Pp[p]) + (q->sw * sg) * e);
+    Pp[p] = pc;
 }
 
 /* step_polarization! (src/Kernels/Dispersive.jl:186-228): fPD <- 0, then per
@@ -488,28 +556,85 @@ static void polarization_step(orc_sim *s) {
             for (int ix = 1; ix <= N[0]; ix++) {
                 size_t g = (size_t)(ix - 1) + (size_t)N[0] * ((size_t)(iy - 1) + (size_t)N[1] * (size_t)(iz - 1));
                 size_t p = rix(s->pdim, ix + 1, iy + 1, iz + 1);
-                for (int a = 0; a < 3; a++) s->fPD[a][p] = (real)0;
+                for (int a = 0; a < 3; a++) {
+                    s->fPD[a][p] = (real)0;
+                    if (s->cplx) s->fPDi[a][p] = (real)0;
+                }
                 for (int k = 0; k < s->npole; k++) {
                     orc_pole *q = &s->pole[k];
                     real sg = q->sigma[g];
                     if (sg != (real)0) {
                         for (int a = 0; a < 3; a++) {
-                            real e = s->F[a][rix(s->dim[a], ix + 1, iy + 1, iz + 1)];
-                            real pc = q->P[a][p];
-#ifdef ORC_PLANT_ADE_INDEX
-                            /* planted mistake for the test suite: P in place of P_prev */
-                            q->Pp[a][p] = pc;
-#endif
-                            if (q->drude)
-                                q->P[a][p] = q->g1i * (((real)2 * pc - q->g1 * q->Pp[a][p]) + (q->dc * sg) * e);
-                            else
-                                q->P[a][p] = q->g1i * ((((real)2 - q->w) * pc - q->g1 * q->Pp[a][p]) + (q->sw * sg) * e);
-                            q->Pp[a][p] = pc;
+                            size_t f = rix(s->dim[a], ix + 1, iy + 1, iz + 1);
+                            ade(q, sg, s->F[a][f], q->P[a], q->Pp[a], p);
+                            if (s->cplx) ade(q, sg, s->Fi[a][f], q->Pi[a], q->Ppi[a], p);
                         }
                     }
-                    for (int a = 0; a < 3; a++) s->fPD[a][p] += q->P[a][p];
+                    for (int a = 0; a < 3; a++) {
+                        s->fPD[a][p] += q->P[a][p];
+                        if (s->cplx) s->fPDi[a][p] += q->Pi[a][p];
+                    }
                 }
             }
+}
+
+/* Exchange the real and imaginary array sets (the update is part-wise). */
+static void swap_parts(orc_sim *s) {
+    real *t;
+#define ORC_SWAP(A, B) do { t = (A); (A) = (B); (B) = t; } while (0)
+    for (int c = 0; c < NCOMP; c++) {
+        ORC_SWAP(s->F[c], s->Fi[c]);
+        ORC_SWAP(s->T[c], s->Ti[c]);
+        ORC_SWAP(s->C[c], s->Ci[c]);
+        ORC_SWAP(s->U[c], s->Ui[c]);
+        ORC_SWAP(s->W[c], s->Wi[c]);
+        ORC_SWAP(s->S[c], s->Si[c]);
+    }
+    for (int a = 0; a < 3; a++) ORC_SWAP(s->fPD[a], s->fPDi[a]);
+#undef ORC_SWAP
+}
+
+/* Periodic / Bloch wrap of one field group after its half step
+ * (_exchange_halos_local!, src/Chunking.jl:2138-2172, with the single-chunk
+ * connections of :1755-1768): ghost 0 <- plane N * exp(-ikL), ghost N+1 <-
+ * plane 1 * exp(+ikL), transverse indices 1..N.  The phase is applied in
+ * complex float64 and rounded to the field type; a unit phase is a plain copy. */
+static void wrap_group(orc_sim *s, int electric) {
+    int N[3] = {s->nx, s->ny, s->nz < 1 ? 1 : s->nz};
+    int base = electric ? 0 : 3;
+    for (int ax = 0; ax < 3; ax++) {
+        if (!s->periodic[ax]) continue;
+        int t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+        for (int side = 0; side < 2; side++) {
+            /* side 0: upper interior -> lower ghost (phase_rev); side 1: lower -> upper (phase_fwd) */
+            int src_g = side == 0 ? N[ax] : 1;
+            int dst_g = side == 0 ? 0 : N[ax] + 1;
+            const double *ph = side == 0 ? s->ph_rev[ax] : s->ph_fwd[ax];
+            int unit = ph[0] == 1.0 && ph[1] == 0.0;
+            for (int c = base; c < base + 3; c++) {
+                const int *d = s->dim[c];
+                for (int j = 1; j <= N[t2]; j++)
+                    for (int i = 1; i <= N[t1]; i++) {
+                        int gs[3], gd[3];
+                        gs[ax] = src_g; gd[ax] = dst_g;
+                        gs[t1] = gd[t1] = i;
+                        gs[t2] = gd[t2] = j;
+                        size_t fs = rix(d, gs[0] + 1, gs[1] + 1, gs[2] + 1);
+                        size_t fd = rix(d, gd[0] + 1, gd[1] + 1, gd[2] + 1);
+                        if (!s->cplx) {
+                            s->F[c][fd] = s->F[c][fs];
+                        } else if (unit) {
+                            s->F[c][fd] = s->F[c][fs];
+                            s->Fi[c][fd] = s->Fi[c][fs];
+                        } else {
+                            double re = (double)s->F[c][fs], im = (double)s->Fi[c][fs];
+                            s->F[c][fd] = (real)(re * ph[0] - im * ph[1]);
+                            s->Fi[c][fd] = (real)(re * ph[1] + im * ph[0]);
+                        }
+                    }
+            }
+        }
+    }
 }
 
 static void update_dfts(orc_sim *s, int magnetic, double t) {
@@ -533,15 +658,23 @@ static void update_dfts(orc_sim *s, int magnetic, double t) {
                     for (int ix = 1; ix <= d->dims[0]; ix++) {
                         size_t ci = (size_t)(ix - 1) + (size_t)d->dims[0] * ((size_t)(iy - 1) + (size_t)d->dims[1] * (size_t)(iz - 1));
                         size_t ai = ci + ncell * (size_t)k;
-                        double v = (double)F[rix(fd, ix + d->start[0], iy + d->start[1], iz + d->start[2])];
+                        size_t fi = rix(fd, ix + d->start[0], iy + d->start[1], iz + d->start[2]);
+                        double v = (double)F[fi];
+                        /* (w_r + i w_i)(v + i u): complex fields add the u terms */
+                        double re = wr * v, im = wi * v;
+                        if (s->cplx) {
+                            double u = (double)s->Fi[d->comp][fi];
+                            re = wr * v - wi * u;
+                            im = wr * u + wi * v;
+                        }
                         if (d->acc_f64) {
                             double *acc = (double *)d->acc;
-                            acc[2 * ai] = acc[2 * ai] + wr * v;
-                            acc[2 * ai + 1] = acc[2 * ai + 1] + wi * v;
+                            acc[2 * ai] = acc[2 * ai] + re;
+                            acc[2 * ai + 1] = acc[2 * ai + 1] + im;
                         } else {
                             real *acc = (real *)d->acc;
-                            acc[2 * ai] = (real)((double)acc[2 * ai] + wr * v);
-                            acc[2 * ai + 1] = (real)((double)acc[2 * ai + 1] + wi * v);
+                            acc[2 * ai] = (real)((double)acc[2 * ai] + re);
+                            acc[2 * ai + 1] = (real)((double)acc[2 * ai + 1] + im);
                         }
                     }
         }
@@ -556,9 +689,14 @@ int orc_step(orc_sim *s, int64_t n) {
         double th = t + (double)(s->dt / (real)2);
         if (s->sources_active) step_sources(s, 1, t);
         half_step(s, 0);
+        if (s->cplx) { swap_parts(s); half_step(s, 0); swap_parts(s); }
+        wrap_group(s, 0);
         update_dfts(s, 1, t);
         if (s->sources_active) step_sources(s, 0, th);
         half_step(s, 1);
+        if (s->cplx) { swap_parts(s); half_step(s, 1); swap_parts(s); }
+        /* the E halo exchange closes step_E_fused!, before chi3 and P (src/Kernels/Kernels.jl:76-83, :452) */
+        wrap_group(s, 1);
         if (s->chi3) chi3_step(s);
         if (s->npole) polarization_step(s);
         update_dfts(s, 0, th);
@@ -571,9 +709,10 @@ int orc_step(orc_sim *s, int64_t n) {
 int64_t orc_timestep(const orc_sim *s) { return s->timestep; }
 
 /* Copy a full raw array (reference layout, ghosts included). which: 0 field,
- * 1 T (B/D), 2 U, 3 W, 4 C, 5 S. */
+ * 1 T (B/D), 2 U, 3 W, 4 C, 5 S; 6..11 the imaginary parts of the same. */
 int orc_read(const orc_sim *s, int which, int comp, real *out) {
-    real *src[6] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp]};
+    real *src[12] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp],
+                     s->Fi[comp], s->Ti[comp], s->Ui[comp], s->Wi[comp], s->Ci[comp], s->Si[comp]};
     size_t n = vol(s->dim[comp]);
     if (!src[which]) { memset(out, 0, n * sizeof(real)); return 1; }
     memcpy(out, src[which], n * sizeof(real));
@@ -581,7 +720,8 @@ int orc_read(const orc_sim *s, int which, int comp, real *out) {
 }
 
 int orc_write(orc_sim *s, int which, int comp, const real *in) {
-    real *dst[6] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp]};
+    real *dst[12] = {s->F[comp], s->T[comp], s->U[comp], s->W[comp], s->C[comp], s->S[comp],
+                     s->Fi[comp], s->Ti[comp], s->Ui[comp], s->Wi[comp], s->Ci[comp], s->Si[comp]};
     if (!dst[which]) return 1;
     memcpy(dst[which], in, vol(s->dim[comp]) * sizeof(real));
     return 0;
@@ -590,14 +730,27 @@ int orc_write(orc_sim *s, int which, int comp, const real *in) {
 /* Polarization arrays of a pole, (nx+2, ny+2, nz+2): which 0 P, 1 P_prev;
  * pole -1 reads the total fPD. */
 int orc_read_pole(const orc_sim *s, int pole, int which, int a, real *out) {
-    const real *src = pole < 0 ? s->fPD[a] : (which ? s->pole[pole].Pp[a] : s->pole[pole].P[a]);
+    /* which 2 / 3: imaginary parts of P / P_prev (fPD with pole -1) */
+    const real *src;
+    if (pole < 0) src = which >= 2 ? s->fPDi[a] : s->fPD[a];
+    else {
+        const orc_pole *q = &s->pole[pole];
+        const real *opts[4] = {q->P[a], q->Pp[a], q->Pi[a], q->Ppi[a]};
+        src = opts[which];
+    }
     if (!src) return 1;
     memcpy(out, src, vol(s->pdim) * sizeof(real));
     return 0;
 }
 
 int orc_write_pole(orc_sim *s, int pole, int which, int a, const real *in) {
-    real *dst = pole < 0 ? s->fPD[a] : (which ? s->pole[pole].Pp[a] : s->pole[pole].P[a]);
+    real *dst;
+    if (pole < 0) dst = which >= 2 ? s->fPDi[a] : s->fPD[a];
+    else {
+        orc_pole *q = &s->pole[pole];
+        real *opts[4] = {q->P[a], q->Pp[a], q->Pi[a], q->Ppi[a]};
+        dst = opts[which];
+    }
     if (!dst) return 1;
     memcpy(dst, in, vol(s->pdim) * sizeof(real));
     return 0;

@@ -52,6 +52,8 @@ def load_path(real, path):
     lib.orc_write.argtypes = [P, C.c_int, C.c_int, P]
     lib.orc_dims.argtypes = [P, C.c_int, P]
     lib.orc_read_dft.argtypes = [P, C.c_int, P]
+    lib.orc_set_periodic.restype = C.c_int
+    lib.orc_set_periodic.argtypes = [P, C.c_int, P, P, P]
     lib.orc_set_chi3.argtypes = [P, P]
     lib.orc_add_pole.restype = C.c_int
     lib.orc_add_pole.argtypes = [P, P, C.c_int, P]
@@ -97,6 +99,14 @@ class OracleEngine:
         self.sim = self.lib.orc_create(*spec.N, *spec.delta, spec.dt, *sig_b, *sig_d, *minv, *msig,
                                        spec.eps_inv_scalar, spec.mu_inv_scalar, spec.cutoff)
         from khronos_hip.grid import component_code
+        self.cplx = bool(getattr(spec, "complex_fields", False))
+        per = tuple(getattr(spec, "periodic", (False, False, False)))
+        if self.cplx or any(per):
+            pa = np.array([1 if p else 0 for p in per], dtype=np.int32)
+            fw = np.array([v for z in spec.bloch_fwd for v in (z.real, z.imag)], dtype=np.float64)
+            rv = np.array([v for z in spec.bloch_rev for v in (z.real, z.imag)], dtype=np.float64)
+            rc = self.lib.orc_set_periodic(self.sim, 1 if self.cplx else 0, _p(pa), _p(fw), _p(rv))
+            assert rc == 0
         self.src_ids = []
         for sd in spec.sources:
             amp = np.asfortranarray(sd.amplitude)
@@ -174,6 +184,8 @@ class OracleEngine:
         return out
 
     def read_field(self, code):
+        if self.cplx:
+            return self.read(0, code) + 1j * self.read(6, code)
         return self.read(0, code)
 
     def write(self, which, code, arr):
@@ -181,7 +193,26 @@ class OracleEngine:
         return self.lib.orc_write(self.sim, which, code, _p(a))
 
     def write_field(self, code, arr):
-        self.write(0, code, arr)
+        if self.cplx:
+            arr = np.asarray(arr)
+            self.write(0, code, arr.real)
+            self.write(6, code, arr.imag)
+        else:
+            self.write(0, code, arr)
+
+    def write_state(self, which, code, arr):
+        """Write field (which 0) or D/B (which 1), complex-aware."""
+        if self.cplx:
+            arr = np.asarray(arr)
+            self.write(which, code, np.real(arr))
+            self.write(which + 6, code, np.imag(arr))
+        else:
+            self.write(which, code, arr)
+
+    def read_state(self, which, code):
+        if self.cplx:
+            return self.read(which, code) + 1j * self.read(which + 6, code)
+        return self.read(which, code)
 
     def pole_shape(self):
         return tuple(n + 2 for n in self.spec.N)
@@ -191,9 +222,17 @@ class OracleEngine:
         (Nx+2, Ny+2, Nz+2)."""
         out = np.zeros(self.pole_shape(), dtype=self.R, order="F")
         self.lib.orc_read_pole(self.sim, pole, which, a, _p(out))
+        if self.cplx and which < 2:
+            im = np.zeros(self.pole_shape(), dtype=self.R, order="F")
+            self.lib.orc_read_pole(self.sim, pole, which + 2, a, _p(im))
+            return out + 1j * im
         return out
 
     def write_pole(self, pole, which, a, arr):
+        if self.cplx and which < 2:
+            arr = np.asarray(arr)
+            self.lib.orc_write_pole(self.sim, pole, which + 2, a, _p(np.asfortranarray(np.imag(arr), dtype=self.R)))
+            arr = np.real(arr)
         h = np.asfortranarray(arr, dtype=self.R)
         return self.lib.orc_write_pole(self.sim, pole, which, a, _p(h))
 
