@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 1
+#define KH_ABI_VERSION 2
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -78,6 +78,18 @@ typedef struct kh_sim_desc {
     double mu_inv_scalar;
     double source_cutoff_time; /* last_source_time (src/Sources/TimeSources.jl:46);
                                   +inf for sources without cutoff */
+    /* Periodic / Bloch boundaries (src/Chunking.jl:1725-1768).  complex_fields
+     * = 1 when any side is Bloch (src/Fields.jl:145-159): every field is then
+     * complex; kh_read_field / kh_write_field / kh_read_pole / kh_write_pole
+     * exchange interleaved (re, im) pairs and current sources still add only
+     * Re(f * amplitude) (src/Sources/Sources.jl:346-357).  periodic[a] = 1 when
+     * axis a wraps: after each half step the updated group gets
+     * ghost 0 <- plane N * bloch_rev[a] and ghost N+1 <- plane 1 * bloch_fwd[a]
+     * (complex (re, im) factors exp(-i k L) and exp(+i k L); 1 for Periodic). */
+    int32_t complex_fields;
+    int32_t periodic[3];
+    double bloch_fwd[3][2];
+    double bloch_rev[3][2];
 } kh_sim_desc;
 
 /* Message describing the last failure on this thread ("" if none). */

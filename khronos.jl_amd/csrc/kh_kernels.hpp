@@ -91,7 +91,8 @@ struct CascadeArgs {
 };
 
 // One dispersive pole on the device: coefficients in the backend type and
-// its arrays, dense over the pole's box (x fastest).
+// its arrays, dense over the pole's box (x fastest); [part][component], the
+// imaginary part only with complex fields.
 template <typename T>
 struct PoleDev {
     T g1i, g1;        // gamma1_inv, gamma1
@@ -99,27 +100,33 @@ struct PoleDev {
     T drive;          // sigma_omega0_dt_sq (Lorentz) or drude_coeff (Drude)
     int lo[3], n[3];  // box: first grid index and extent
     const T *sigma;
-    T *P[3];
-    T *Pp[3];
+    T *P[2][3];
+    T *Pp[2][3];
 };
 
 // Material pass after the E update: chi3 and the poles, over the material box
 // (bounding box of chi3 != 0 and every pole's box), one cell per thread.
+// Arrays are [part][component] (part 1: imaginary, complex fields only).
 template <typename T>
 struct MatArgs {
-    T *E[3];
-    T *W[3];              // E-side own-axis stage (FAST correction in the PML)
+    T *E[2][3];
+    T *W[2][3];           // E-side own-axis stage (FAST correction in the PML)
     const T *minv[3];
     T minv_s;
     const T *sig[3];      // sampled D-side PML sigma per axis (N+2)
     const T *chi3;        // dense over the box, or null
-    T *delta[3];          // FAST with chi3: linear minus stored E per cell, or null
-    T *fcur[3];           // total polarization the update used (null without poles)
-    T *fnext[3];          // FAST: previous total on entry; new total on exit.  EXACT: == fcur
+    T *delta[2][3];       // FAST with chi3: linear minus stored E per cell, or null
+    T *fcur[2][3];        // total polarization the update used (null without poles)
+    T *fnext[2][3];       // FAST: previous total on entry; new total on exit.  EXACT: == fcur
     int lo[3], n[3];
     int64_t sy, sz;
     int npole;
     const PoleDev<T> *poles;
+    // FAST with a periodic axis: the wrapped ghost of a corrected boundary
+    // cell is rewritten from the corrected linear field
+    int N[3];
+    int per[3];
+    double fwd[3][2], rev[3][2];
 };
 
 template <typename T>
@@ -725,7 +732,7 @@ __global__ __launch_bounds__(256) void source_fix_kernel(T *A, T *W, const T *mi
 // source-correction form with S replaced by -fPD.  chi3 rescales the stored E
 // but not D: where E carries D (s_o == 0) the next step's carried value is
 // short by (1 - s_p)/(1 + s_p) delta, delta = E_linear - E_stored.
-template <typename T, bool FAST>
+template <typename T, bool FAST, int NP>
 __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ncell = (int64_t)p.n[0] * p.n[1] * p.n[2];
@@ -735,8 +742,9 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
     const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
     const int g[3] = {p.lo[0] + ux, p.lo[1] + uy, p.lo[2] + uz};
     const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
-    T e[3], so[3];
-    for (int a = 0; a < 3; ++a) e[a] = p.E[a][idx];
+    T e[NP][3], so[3];
+    for (int q = 0; q < NP; ++q)
+        for (int a = 0; a < 3; ++a) e[q][a] = p.E[q][a][idx];
     bool dirty = false;
     if constexpr (FAST) {
         for (int a = 0; a < 3; ++a) {
@@ -744,61 +752,143 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
             so[a] = p.sig[a][g[a]];
             const T sp = p.sig[prv][g[prv]];
             const T ratio = (T(1) - sp) / (T(1) + sp);
-            if (p.delta[0] && so[a] == T(0)) e[a] = e[a] + ratio * p.delta[a][c];
-            if (p.npole) {
-                const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
-                const T dw = m * (ratio * p.fnext[a][c] - p.fcur[a][c]);
-                if (so[a] == T(0)) {
-                    e[a] = e[a] + dw;
-                } else {
-                    p.W[a][idx] = p.W[a][idx] + dw;
-                    e[a] = e[a] + (T(1) + so[a]) * dw;
+            const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
+            for (int q = 0; q < NP; ++q) {
+                if (p.delta[0][0] && so[a] == T(0)) e[q][a] = e[q][a] + ratio * p.delta[q][a][c];
+                if (p.npole) {
+                    const T dw = m * (ratio * p.fnext[q][a][c] - p.fcur[q][a][c]);
+                    if (so[a] == T(0)) {
+                        e[q][a] = e[q][a] + dw;
+                    } else {
+                        p.W[q][a][idx] = p.W[q][a][idx] + dw;
+                        e[q][a] = e[q][a] + (T(1) + so[a]) * dw;
+                    }
                 }
             }
         }
-        dirty = p.delta[0] || p.npole;
+        dirty = p.delta[0][0] || p.npole;
+        if (dirty) {
+            // the wrap ran before this pass: refresh the ghost copies of a
+            // corrected boundary cell (linear field, as the reference wraps
+            // before chi3 and P)
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!p.per[ax]) continue;
+                for (int side = 0; side < 2; ++side) {
+                    if (g[ax] != (side == 0 ? p.N[ax] : 1)) continue;
+                    int gd[3] = {g[0], g[1], g[2]};
+                    gd[ax] = side == 0 ? 0 : p.N[ax] + 1;
+                    const size_t di = (size_t)gd[0] + (size_t)gd[1] * p.sy + (size_t)gd[2] * p.sz;
+                    const double *ph = side == 0 ? p.rev[ax] : p.fwd[ax];
+                    for (int a = 0; a < 3; ++a) {
+                        if constexpr (NP == 2) {
+                            const double re = (double)e[0][a], im = (double)e[1][a];
+                            p.E[0][a][di] = (T)(re * ph[0] - im * ph[1]);
+                            p.E[1][a][di] = (T)(re * ph[1] + im * ph[0]);
+                        } else {
+                            p.E[0][a][di] = e[0][a];
+                        }
+                    }
+                }
+            }
+        }
     }
     if (p.chi3) {
         const T c3 = p.chi3[c];
         if (c3 != T(0)) {
-            const T esq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-            const T corr = T(1) / (T(1) + c3 * esq);
+            // real(e conj(e)) per component = re*re + im*im, summed x, y, z
+            T m2[3];
             for (int a = 0; a < 3; ++a) {
-                const T en = e[a] * corr;
-                if constexpr (FAST) {
-                    if (p.delta[0]) p.delta[a][c] = so[a] == T(0) ? e[a] - en : T(0);
-                }
-                e[a] = en;
+                m2[a] = e[0][a] * e[0][a];
+                if constexpr (NP == 2) m2[a] = m2[a] + e[1][a] * e[1][a];
             }
+            const T esq = m2[0] + m2[1] + m2[2];
+            const T corr = T(1) / (T(1) + c3 * esq);
+            for (int q = 0; q < NP; ++q)
+                for (int a = 0; a < 3; ++a) {
+                    const T en = e[q][a] * corr;
+                    if constexpr (FAST) {
+                        if (p.delta[0][0]) p.delta[q][a][c] = so[a] == T(0) ? e[q][a] - en : T(0);
+                    }
+                    e[q][a] = en;
+                }
             dirty = true;
         }
     }
     if (dirty)
-        for (int a = 0; a < 3; ++a) p.E[a][idx] = e[a];
+        for (int q = 0; q < NP; ++q)
+            for (int a = 0; a < 3; ++a) p.E[q][a][idx] = e[q][a];
     if (p.npole) {
-        T sum[3] = {T(0), T(0), T(0)};
+        T sum[NP][3];
+        for (int q = 0; q < NP; ++q)
+            for (int a = 0; a < 3; ++a) sum[q][a] = T(0);
         for (int k = 0; k < p.npole; ++k) {
-            const PoleDev<T> &q = p.poles[k];
-            const int kx = g[0] - q.lo[0], ky = g[1] - q.lo[1], kz = g[2] - q.lo[2];
-            if (kx < 0 || ky < 0 || kz < 0 || kx >= q.n[0] || ky >= q.n[1] || kz >= q.n[2]) {
+            const PoleDev<T> &pl = p.poles[k];
+            const int kx = g[0] - pl.lo[0], ky = g[1] - pl.lo[1], kz = g[2] - pl.lo[2];
+            if (kx < 0 || ky < 0 || kz < 0 || kx >= pl.n[0] || ky >= pl.n[1] || kz >= pl.n[2]) {
                 // outside the pole's box its P is zero: fPD += 0
-                for (int a = 0; a < 3; ++a) sum[a] = sum[a] + T(0);
+                for (int q = 0; q < NP; ++q)
+                    for (int a = 0; a < 3; ++a) sum[q][a] = sum[q][a] + T(0);
                 continue;
             }
-            const size_t kc = (size_t)kx + (size_t)q.n[0] * ((size_t)ky + (size_t)q.n[1] * (size_t)kz);
-            const T s = q.sigma[kc];
-            for (int a = 0; a < 3; ++a) {
-                T pn = q.P[a][kc];
-                if (s != T(0)) {
-                    const T pc = pn;
-                    pn = q.g1i * ((q.coefp * pc - q.g1 * q.Pp[a][kc]) + (q.drive * s) * e[a]);
-                    q.P[a][kc] = pn;
-                    q.Pp[a][kc] = pc;
+            const size_t kc = (size_t)kx + (size_t)pl.n[0] * ((size_t)ky + (size_t)pl.n[1] * (size_t)kz);
+            const T s = pl.sigma[kc];
+            for (int q = 0; q < NP; ++q)
+                for (int a = 0; a < 3; ++a) {
+                    T pn = pl.P[q][a][kc];
+                    if (s != T(0)) {
+                        const T pc = pn;
+                        pn = pl.g1i * ((pl.coefp * pc - pl.g1 * pl.Pp[q][a][kc]) + (pl.drive * s) * e[q][a]);
+                        pl.P[q][a][kc] = pn;
+                        pl.Pp[q][a][kc] = pc;
+                    }
+                    sum[q][a] = sum[q][a] + pn;
                 }
-                sum[a] = sum[a] + pn;
-            }
         }
-        for (int a = 0; a < 3; ++a) p.fnext[a][c] = sum[a];
+        for (int q = 0; q < NP; ++q)
+            for (int a = 0; a < 3; ++a) p.fnext[q][a][c] = sum[q][a];
+    }
+}
+
+// Periodic / Bloch wrap of one field group along one axis
+// (src/Chunking.jl:1755-1768, :2138-2172): ghost 0 <- plane N * rev,
+// ghost N+1 <- plane 1 * fwd, transverse grid indices 1..N.  The phase is
+// applied in complex float64 and rounded to T; a unit phase copies.
+template <typename T>
+struct WrapArgs {
+    T *F[2][3];
+    int nparts;
+    int ax;
+    int N[3];
+    int64_t sy, sz;
+    double fwd[2], rev[2];
+    int unit;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void wrap_kernel(WrapArgs<T> w) {
+    const int ax = w.ax, t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+    const int64_t n1 = w.N[t1], n2 = w.N[t2];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n1 * n2 * 6) return;
+    const int i = (int)(c % n1);
+    int64_t r = c / n1;
+    const int j = (int)(r % n2);
+    r /= n2;
+    const int comp = (int)(r % 3), side = (int)(r / 3);
+    int gs[3], gd[3];
+    gs[ax] = side == 0 ? w.N[ax] : 1;
+    gd[ax] = side == 0 ? 0 : w.N[ax] + 1;
+    gs[t1] = gd[t1] = i + 1;
+    gs[t2] = gd[t2] = j + 1;
+    const size_t si = (size_t)gs[0] + (size_t)gs[1] * w.sy + (size_t)gs[2] * w.sz;
+    const size_t di = (size_t)gd[0] + (size_t)gd[1] * w.sy + (size_t)gd[2] * w.sz;
+    if (w.nparts == 1 || w.unit) {
+        for (int q = 0; q < w.nparts; ++q) w.F[q][comp][di] = w.F[q][comp][si];
+    } else {
+        const double *ph = side == 0 ? w.rev : w.fwd;
+        const double re = (double)w.F[0][comp][si], im = (double)w.F[1][comp][si];
+        w.F[0][comp][di] = (T)(re * ph[0] - im * ph[1]);
+        w.F[1][comp][di] = (T)(re * ph[1] + im * ph[0]);
     }
 }
 
@@ -809,8 +899,9 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
 // accumulator update is formed in float64 and stored in ACC.
 template <typename T, typename ACC>
 __global__ __launch_bounds__(256) void dft_kernel(ACC *__restrict__ acc, const T *__restrict__ field,
+                                                  const T *__restrict__ field_im,
                                                   int d0, int d1, int d2, int s0, int s1, int s2,
-                                                  int64_t sy, int64_t sz, int nf,
+                                                  int n0, int n1, int n2, int64_t sy, int64_t sz, int nf,
                                                   const double *__restrict__ freqs, double theta,
                                                   double dts) {
     extern __shared__ double wsh[];
@@ -829,8 +920,24 @@ __global__ __launch_bounds__(256) void dft_kernel(ACC *__restrict__ acc, const T
     int64_t r = cell / d0;
     int iy = (int)(r % d1);
     int iz = (int)(r / d1);
-    const double v = (double)field[(size_t)(s0 + ix) + (size_t)(s1 + iy) * sy + (size_t)(s2 + iz) * sz];
+    const int gx = s0 + ix, gy = s1 + iy, gz = s2 + iz;
+    // past grid index N+1 the reference's array holds an always-zero ghost
+    const bool in = gx <= n0 + 1 && gy <= n1 + 1 && gz <= n2 + 1;
+    const size_t fi = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const double v = in ? (double)field[fi] : 0.0;
     ACC *a = acc + 2 * cell;
+    if (field_im) {
+        // complex field: (w_r + i w_i)(v + i u)
+        const double u = in ? (double)field_im[fi] : 0.0;
+        for (int k = 0; k < nf; ++k, a += 2 * ncell) {
+            const double wr = wsh[2 * k], wi = wsh[2 * k + 1];
+            double re = (double)a[0] + (wr * v - wi * u);
+            double im = (double)a[1] + (wr * u + wi * v);
+            a[0] = (ACC)re;
+            a[1] = (ACC)im;
+        }
+        return;
+    }
     for (int k = 0; k < nf; ++k, a += 2 * ncell) {
         double re = (double)a[0] + wsh[2 * k] * v;
         double im = (double)a[1] + wsh[2 * k + 1] * v;

@@ -134,8 +134,17 @@ struct Ctx final : CtxBase {
     double cutoff;
     hipStream_t st = nullptr;
     std::vector<DevBuf> owned;
-    // device arrays (pre-offset by g.front)
-    T *E[3]{}, *H[3]{}, *D[3]{}, *B[3]{}, *UD[3]{}, *UB[3]{}, *WD[3]{}, *WB[3]{};
+    // device arrays (pre-offset by g.front).  Complex fields are held as two
+    // real parts with identical layouts: every real-coefficient operation of
+    // the time step runs part by part; the wrap, chi3, the DFT and the
+    // readout combine them.
+    struct Part {
+        T *E[3]{}, *H[3]{}, *D[3]{}, *B[3]{}, *UD[3]{}, *UB[3]{}, *WD[3]{}, *WB[3]{};
+    };
+    Part fp[2];
+    int nparts = 1;
+    bool periodic[3] = {false, false, false};
+    double ph_fwd[3][2] = {}, ph_rev[3][2] = {};
     T *epsi[3]{}, *mui[3]{};
     T *sigB[3]{}, *sigD[3]{};
     T eps_s, mu_s;
@@ -166,9 +175,9 @@ struct Ctx final : CtxBase {
     std::vector<kh::PoleDev<T>> poles;
     kh::PoleDev<T> *poles_dev = nullptr;
     T *chi3_dev = nullptr;                     // dense over the material box
-    T *fpd[2][3] = {};                         // total polarization (FAST: two, alternating)
+    T *fpd[2][2][3] = {};                      // total polarization [buffer][part][comp] (FAST: two buffers, alternating)
     int fcur = 0;
-    T *delta[3] = {};                          // FAST with chi3
+    T *delta[2][3] = {};                       // FAST with chi3, [part][comp]
     bool mat = false;
     int mlo[3] = {1, 1, 1}, mn[3] = {0, 0, 0};
     int64_t mcells = 0;
@@ -217,14 +226,28 @@ struct Ctx final : CtxBase {
         cutoff = d.source_cutoff_time;
         eps_s = (T)d.eps_inv_scalar;
         mu_s = (T)d.mu_inv_scalar;
-        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        nparts = d.complex_fields ? 2 : 1;
         for (int a = 0; a < 3; ++a) {
-            E[a] = field_alloc(); H[a] = field_alloc();
-            // D and B are state only in EXACT mode; FAST mode carries the
-            // cascade on E/H and W directly
-            if (mode == KH_MODE_EXACT) { D[a] = field_alloc(); B[a] = field_alloc(); }
-            UD[a] = field_alloc(); UB[a] = field_alloc();
-            WD[a] = field_alloc(); WB[a] = field_alloc();
+            periodic[a] = d.periodic[a] != 0;
+            for (int q = 0; q < 2; ++q) {
+                ph_fwd[a][q] = periodic[a] ? d.bloch_fwd[a][q] : (q == 0 ? 1.0 : 0.0);
+                ph_rev[a][q] = periodic[a] ? d.bloch_rev[a][q] : (q == 0 ? 1.0 : 0.0);
+            }
+            require(nparts == 2 || (ph_fwd[a][0] == 1.0 && ph_fwd[a][1] == 0.0 && ph_rev[a][0] == 1.0 &&
+                                    ph_rev[a][1] == 0.0),
+                    "a Bloch phase other than 1 needs complex_fields = 1");
+        }
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (int p = 0; p < nparts; ++p) {
+            Part &f = fp[p];
+            for (int a = 0; a < 3; ++a) {
+                f.E[a] = field_alloc(); f.H[a] = field_alloc();
+                // D and B are state only in EXACT mode; FAST mode carries the
+                // cascade on E/H and W directly
+                if (mode == KH_MODE_EXACT) { f.D[a] = field_alloc(); f.B[a] = field_alloc(); }
+                f.UD[a] = field_alloc(); f.UB[a] = field_alloc();
+                f.WD[a] = field_alloc(); f.WB[a] = field_alloc();
+            }
         }
         // sampled sigma: s[g] = profile[2g - 1] (1-based) for g = 1..N, 0 at the boundaries
         for (int a = 0; a < 3; ++a) {
@@ -307,9 +330,12 @@ struct Ctx final : CtxBase {
         require(dec >= 1, "decimation must be >= 1");
         int cnt[3];
         comp_count(g.n, comp, cnt);
+        // the reference's array of the component spans indices 0..count+1; the
+        // device grid holds 0..N+1, and an index past it (the ghost beyond a
+        // staggered component's extra plane, never written) reads as zero
         for (int a = 0; a < 3; ++a) {
             require(d[a] >= 1, "monitor extent must be positive");
-            require(s[a] >= 0 && s[a] + d[a] - 1 <= g.n[a] + 1, "monitor grid volume outside the grid");
+            require(s[a] >= 0 && s[a] + d[a] - 1 <= cnt[a] + 1, "monitor grid volume outside the grid");
         }
         Dft m;
         m.comp = comp;
@@ -377,44 +403,51 @@ struct Ctx final : CtxBase {
         if (nonzero_box(sg, q.lo, q.n)) {
             q.sigma = upload_box(sg, q.lo, q.n);
             const size_t nb = (size_t)q.n[0] * q.n[1] * q.n[2] * sizeof(T);
-            for (int a = 0; a < 3; ++a) {
-                q.P[a] = (T *)dalloc(nb);
-                q.Pp[a] = (T *)dalloc(nb);
-            }
+            for (int pt = 0; pt < nparts; ++pt)
+                for (int a = 0; a < 3; ++a) {
+                    q.P[pt][a] = (T *)dalloc(nb);
+                    q.Pp[pt][a] = (T *)dalloc(nb);
+                }
             HIPCHK(hipStreamSynchronize(st));
         }
         poles.push_back(q);
         *id = (int32_t)poles.size() - 1;
     }
 
+    // host layout: (nx+2, ny+2, nz+2), backend precision; complex fields
+    // interleave (re, im)
     void pole_io(int id, int which, int comp, void *host, size_t bytes, bool rd) {
         require(id >= 0 && id < (int)poles.size(), "bad pole id");
         require(which == 0 || which == 1, "which must be 0 (P) or 1 (P_prev)");
         require(comp >= 0 && comp < 3, "polarization component must be 0..2");
         const int r[3] = {g.n[0] + 2, g.n[1] + 2, g.n[2] + 2};
-        require(bytes == (size_t)r[0] * r[1] * r[2] * sizeof(T), "polarization buffer size mismatch");
+        const size_t nh = (size_t)r[0] * r[1] * r[2];
+        require(bytes == nh * nparts * sizeof(T), "polarization buffer size mismatch");
         kh::PoleDev<T> &q = poles[id];
-        T *dv = which ? q.Pp[comp] : q.P[comp];
-        std::vector<T> h((size_t)q.n[0] * q.n[1] * q.n[2]);
-        if (rd) {
-            T *o = (T *)host;
-            std::fill(o, o + (size_t)r[0] * r[1] * r[2], T(0));
-            if (h.empty()) return;
-            HIPCHK(hipMemcpyAsync(h.data(), dv, h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-        }
-        const T *in = (const T *)host;
-        for (int k = 0; k < q.n[2]; ++k)
-            for (int j = 0; j < q.n[1]; ++j)
-                for (int i = 0; i < q.n[0]; ++i) {
-                    const size_t hc = (size_t)(q.lo[0] + i) + (size_t)r[0] * ((size_t)(q.lo[1] + j) + (size_t)r[1] * (q.lo[2] + k));
-                    const size_t bc = (size_t)i + (size_t)q.n[0] * ((size_t)j + (size_t)q.n[1] * k);
-                    if (rd) ((T *)host)[hc] = h[bc];
-                    else h[bc] = in[hc];
-                }
-        if (!rd && !h.empty()) {
-            HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));
+        const size_t nbx = (size_t)q.n[0] * q.n[1] * q.n[2];
+        T *hp = (T *)host;
+        if (rd) std::fill(hp, hp + nh * nparts, T(0));
+        if (nbx == 0) return;
+        std::vector<T> h(nbx);
+        for (int pt = 0; pt < nparts; ++pt) {
+            T *dv = which ? q.Pp[pt][comp] : q.P[pt][comp];
+            if (rd) {
+                HIPCHK(hipMemcpyAsync(h.data(), dv, nbx * sizeof(T), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+            for (int k = 0; k < q.n[2]; ++k)
+                for (int j = 0; j < q.n[1]; ++j)
+                    for (int i = 0; i < q.n[0]; ++i) {
+                        const size_t hc = (size_t)(q.lo[0] + i) + (size_t)r[0] * ((size_t)(q.lo[1] + j) + (size_t)r[1] * (q.lo[2] + k));
+                        const size_t bc = (size_t)i + (size_t)q.n[0] * ((size_t)j + (size_t)q.n[1] * k);
+                        const size_t hi = nparts == 2 ? 2 * hc + pt : hc;
+                        if (rd) hp[hi] = h[bc];
+                        else h[bc] = hp[hi];
+                    }
+            if (!rd) {
+                HIPCHK(hipMemcpyAsync(dv, h.data(), nbx * sizeof(T), hipMemcpyHostToDevice, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
         }
     }
     void read_pole(int id, int which, int comp, void *host, size_t bytes) override {
@@ -449,37 +482,47 @@ struct Ctx final : CtxBase {
         chi3_host.shrink_to_fit();
         const size_t nb = (size_t)mcells * sizeof(T);
         if (has_pole) {
-            for (int a = 0; a < 3; ++a) {
-                fpd[0][a] = (T *)dalloc(nb);
-                if (mode == KH_MODE_FAST) fpd[1][a] = (T *)dalloc(nb);
-            }
+            for (int pt = 0; pt < nparts; ++pt)
+                for (int a = 0; a < 3; ++a) {
+                    fpd[0][pt][a] = (T *)dalloc(nb);
+                    if (mode == KH_MODE_FAST) fpd[1][pt][a] = (T *)dalloc(nb);
+                }
             poles_dev = (kh::PoleDev<T> *)dalloc(poles.size() * sizeof(kh::PoleDev<T>));
             HIPCHK(hipMemcpyAsync(poles_dev, poles.data(), poles.size() * sizeof(kh::PoleDev<T>),
                                   hipMemcpyHostToDevice, st));
         }
         if (has_chi3 && mode == KH_MODE_FAST)
-            for (int a = 0; a < 3; ++a) delta[a] = (T *)dalloc(nb);
+            for (int pt = 0; pt < nparts; ++pt)
+                for (int a = 0; a < 3; ++a) delta[pt][a] = (T *)dalloc(nb);
         HIPCHK(hipStreamSynchronize(st));
     }
 
     void launch_material() {
         kh::MatArgs<T> q{};
         for (int a = 0; a < 3; ++a) {
-            q.E[a] = E[a];
-            q.W[a] = WD[a];
+            for (int pt = 0; pt < nparts; ++pt) {
+                q.E[pt][a] = fp[pt].E[a];
+                q.W[pt][a] = fp[pt].WD[a];
+                q.delta[pt][a] = delta[pt][a];
+                q.fcur[pt][a] = fpd[fcur][pt][a];
+                q.fnext[pt][a] = mode == KH_MODE_FAST ? fpd[1 - fcur][pt][a] : fpd[0][pt][a];
+            }
             q.minv[a] = epsi[a];
             q.sig[a] = sigD[a];
-            q.delta[a] = delta[a];
-            q.fcur[a] = fpd[fcur][a];
-            q.fnext[a] = mode == KH_MODE_FAST ? fpd[1 - fcur][a] : fpd[0][a];
             q.lo[a] = mlo[a];
             q.n[a] = mn[a];
+            q.N[a] = g.n[a];
+            q.per[a] = periodic[a] ? 1 : 0;
+            for (int k = 0; k < 2; ++k) {
+                q.fwd[a][k] = ph_fwd[a][k];
+                q.rev[a][k] = ph_rev[a][k];
+            }
         }
         q.minv_s = eps_s;
         q.chi3 = chi3_dev;
         q.sy = g.sy();
         q.sz = g.sz();
-        q.npole = fpd[0][0] ? (int)poles.size() : 0;
+        q.npole = fpd[0][0][0] ? (int)poles.size() : 0;
         q.poles = poles_dev;
         EvPair ev{};
         if (timing) {
@@ -487,8 +530,13 @@ struct Ctx final : CtxBase {
             HIPCHK(hipEventRecord(ev.a, st));
         }
         const dim3 grid((unsigned)((mcells + 255) / 256)), blk(256);
-        if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true>), grid, blk, 0, st, q);
-        else hipLaunchKernelGGL((kh::material_kernel<T, false>), grid, blk, 0, st, q);
+        if (nparts == 2) {
+            if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true, 2>), grid, blk, 0, st, q);
+            else hipLaunchKernelGGL((kh::material_kernel<T, false, 2>), grid, blk, 0, st, q);
+        } else {
+            if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true, 1>), grid, blk, 0, st, q);
+            else hipLaunchKernelGGL((kh::material_kernel<T, false, 1>), grid, blk, 0, st, q);
+        }
         HIPCHK(hipGetLastError());
         if (timing) {
             HIPCHK(hipEventRecord(ev.b, st));
@@ -602,8 +650,7 @@ struct Ctx final : CtxBase {
             if (P.inner) {
                 kh::InnerArgs<T> &ia = P.ia;
                 for (int a = 0; a < 3; ++a) {
-                    ia.A[a] = e ? E[a] : H[a];
-                    ia.F[a] = e ? H[a] : E[a];
+                    // field pointers are set per part at launch
                     ia.minv[a] = e ? epsi[a] : mui[a];
                     ia.inv[a] = T(1) / dx[a];
                     ia.lo[a] = P.lo[a];
@@ -632,15 +679,16 @@ struct Ctx final : CtxBase {
         planned = true;
     }
 
-    kh::CascadeArgs<T> cascade_args(int e) const {
+    kh::CascadeArgs<T> cascade_args(int e, int part) const {
         const Plan &P = plan[e];
+        const Part &f = fp[part];
         kh::CascadeArgs<T> p{};
         for (int a = 0; a < 3; ++a) {
-            p.A[a] = e ? E[a] : H[a];
-            p.F[a] = e ? H[a] : E[a];
-            p.Tq[a] = e ? D[a] : B[a];
-            p.U[a] = e ? UD[a] : UB[a];
-            p.W[a] = e ? WD[a] : WB[a];
+            p.A[a] = e ? f.E[a] : f.H[a];
+            p.F[a] = e ? f.H[a] : f.E[a];
+            p.Tq[a] = e ? f.D[a] : f.B[a];
+            p.U[a] = e ? f.UD[a] : f.UB[a];
+            p.W[a] = e ? f.WD[a] : f.WB[a];
             p.sig[a] = e ? sigD[a] : sigB[a];
             p.minv[a] = e ? epsi[a] : mui[a];
             p.inv[a] = T(1) / dx[a];
@@ -653,9 +701,9 @@ struct Ctx final : CtxBase {
         for (int s = 0; s < P.nslab; ++s) p.slab[s] = P.slab[s];
         p.nblock = P.cblocks;
         p.per_xcd = (P.cblocks + kh::NXCD - 1) / kh::NXCD;
-        if (e && fpd[0][0]) {
+        if (e && fpd[0][0][0]) {
             for (int a = 0; a < 3; ++a) {
-                p.fpd[a] = fpd[0][a];
+                p.fpd[a] = fpd[0][part][a];
                 p.ulo[a] = mlo[a];
                 p.un[a] = mn[a];
             }
@@ -673,8 +721,62 @@ struct Ctx final : CtxBase {
     }
 
     void launch_half(int e) {
+        for (int part = 0; part < nparts; ++part) launch_half_part(e, part);
+        // FAST mode sources: added after the update, for every source cell
+        // (real part only: S += Re(f A), src/Sources/Sources.jl:346-357)
+        const bool electric = e == 1;
+        if (mode == KH_MODE_FAST && (active || last_active)) {
+            const Part &f = fp[0];
+            for (auto &o : srcs) {
+                if ((o.comp < 3) != electric) continue;
+                T fr = 0, fi = 0, pr = 0, pi = 0;
+                if (active) wave_at(o, ts, fr, fi);
+                if (last_active) wave_at(o, ts - 1, pr, pi);
+                const int a = o.comp % 3, prv = (a + 2) % 3;
+                int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
+                unsigned nb = (unsigned)((n + 255) / 256);
+                T *A = electric ? f.E[a] : f.H[a];
+                T *W = electric ? f.WD[a] : f.WB[a];
+                const T *mi = electric ? epsi[a] : mui[a];
+                const T *so = electric ? sigD[a] : sigB[a];
+                const T *sp = electric ? sigD[prv] : sigB[prv];
+                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, W, mi,
+                                   electric ? eps_s : mu_s, so, sp, a, prv, (const T *)o.amp, o.s[0], o.s[1],
+                                   o.s[2], o.d[0], o.d[1], o.d[2], fr, fi, pr, pi, g.n[0], g.n[1], g.n[2],
+                                   g.sy(), g.sz());
+                HIPCHK(hipGetLastError());
+            }
+        }
+        launch_wrap(e);
+    }
+
+    // periodic / Bloch ghost fill of the group just updated, axis by axis
+    void launch_wrap(int e) {
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!periodic[ax]) continue;
+            kh::WrapArgs<T> w{};
+            for (int pt = 0; pt < nparts; ++pt)
+                for (int a = 0; a < 3; ++a) w.F[pt][a] = e ? fp[pt].E[a] : fp[pt].H[a];
+            w.nparts = nparts;
+            w.ax = ax;
+            for (int a = 0; a < 3; ++a) w.N[a] = g.n[a];
+            w.sy = g.sy();
+            w.sz = g.sz();
+            for (int k = 0; k < 2; ++k) {
+                w.fwd[k] = ph_fwd[ax][k];
+                w.rev[k] = ph_rev[ax][k];
+            }
+            w.unit = ph_fwd[ax][0] == 1.0 && ph_fwd[ax][1] == 0.0 && ph_rev[ax][0] == 1.0 && ph_rev[ax][1] == 0.0;
+            const int64_t n = (int64_t)g.n[(ax + 1) % 3] * g.n[(ax + 2) % 3] * 6;
+            hipLaunchKernelGGL((kh::wrap_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    void launch_half_part(int e, int part) {
         const Plan &P = plan[e];
         const bool electric = e == 1;
+        const Part &f = fp[part];
         dim3 blk(kh::TX, kh::TY, 1);
         EvPair ev{};
         if (P.inner) {
@@ -682,9 +784,14 @@ struct Ctx final : CtxBase {
                 ev = take_event(2 * e);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
-            const bool pervox = P.ia.minv[0] != nullptr;
+            kh::InnerArgs<T> ia = P.ia;
+            for (int a = 0; a < 3; ++a) {
+                ia.A[a] = electric ? f.E[a] : f.H[a];
+                ia.F[a] = electric ? f.H[a] : f.E[a];
+            }
+            const bool pervox = ia.minv[0] != nullptr;
             const dim3 grid(P.inner_blocks);
-#define KH_INNER(KER, PV, NTV) hipLaunchKernelGGL((kh::KER<T, PV, NTV>), grid, blk, 0, st, P.ia)
+#define KH_INNER(KER, PV, NTV) hipLaunchKernelGGL((kh::KER<T, PV, NTV>), grid, blk, 0, st, ia)
             if (electric) {
                 if (pervox) { if (nt) KH_INNER(inner_e_kernel, true, true); else KH_INNER(inner_e_kernel, true, false); }
                 else { if (nt) KH_INNER(inner_e_kernel, false, true); else KH_INNER(inner_e_kernel, false, false); }
@@ -704,9 +811,11 @@ struct Ctx final : CtxBase {
                 ev = take_event(2 * e + 1);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
-            kh::CascadeArgs<T> p = cascade_args(e);
+            kh::CascadeArgs<T> p = cascade_args(e, part);
             p.nsrc = 0;
+            // sources add Re(f A) to the real part only
             for (auto &o : srcs) {
+                if (part != 0) break;
                 if ((o.comp < 3) != electric) continue;
                 kh::SrcL<T> &l = p.src[p.nsrc++];
                 l.comp = o.comp % 3;
@@ -731,10 +840,10 @@ struct Ctx final : CtxBase {
             }
             kh::PmlArgs<T> q{};
             for (int a = 0; a < 3; ++a) {
-                q.A[a] = electric ? E[a] : H[a];
-                q.F[a] = electric ? H[a] : E[a];
-                q.U[a] = electric ? UD[a] : UB[a];
-                q.W[a] = electric ? WD[a] : WB[a];
+                q.A[a] = electric ? f.E[a] : f.H[a];
+                q.F[a] = electric ? f.H[a] : f.E[a];
+                q.U[a] = electric ? f.UD[a] : f.UB[a];
+                q.W[a] = electric ? f.WD[a] : f.WB[a];
                 q.sig[a] = electric ? sigD[a] : sigB[a];
                 q.minv[a] = electric ? epsi[a] : mui[a];
                 q.inv[a] = T(1) / dx[a];
@@ -762,28 +871,6 @@ struct Ctx final : CtxBase {
                 evs.push_back(ev);
             }
         }
-        // FAST mode sources: added after the update, for every source cell
-        if (mode == KH_MODE_FAST && (active || last_active)) {
-            for (auto &o : srcs) {
-                if ((o.comp < 3) != electric) continue;
-                T fr = 0, fi = 0, pr = 0, pi = 0;
-                if (active) wave_at(o, ts, fr, fi);
-                if (last_active) wave_at(o, ts - 1, pr, pi);
-                const int a = o.comp % 3, prv = (a + 2) % 3;
-                int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
-                unsigned nb = (unsigned)((n + 255) / 256);
-                T *A = electric ? E[a] : H[a];
-                T *W = electric ? WD[a] : WB[a];
-                const T *mi = electric ? epsi[a] : mui[a];
-                const T *so = electric ? sigD[a] : sigB[a];
-                const T *sp = electric ? sigD[prv] : sigB[prv];
-                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, W, mi,
-                                   electric ? eps_s : mu_s, so, sp, a, prv, (const T *)o.amp, o.s[0], o.s[1],
-                                   o.s[2], o.d[0], o.d[1], o.d[2], fr, fi, pr, pi, g.n[0], g.n[1], g.n[2],
-                                   g.sy(), g.sz());
-                HIPCHK(hipGetLastError());
-            }
-        }
     }
 
     void launch_dfts(bool electric, double t) {
@@ -792,7 +879,8 @@ struct Ctx final : CtxBase {
         for (auto &m : dfts) {
             if ((m.comp < 3) != electric) continue;
             if (m.dec > 1 && (ts % m.dec) != 0) continue;
-            const T *f = electric ? E[m.comp] : H[m.comp - 3];
+            const T *f = electric ? fp[0].E[m.comp] : fp[0].H[m.comp - 3];
+            const T *fi = nparts == 2 ? (electric ? fp[1].E[m.comp] : fp[1].H[m.comp - 3]) : nullptr;
             int threads = 256;
             int64_t blocks = (m.ncell + threads - 1) / threads;
             size_t shm = (size_t)m.nf * 2 * sizeof(double);
@@ -803,12 +891,12 @@ struct Ctx final : CtxBase {
             }
             if (acc64)
                 hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3((unsigned)blocks), dim3(threads), shm, st,
-                                   (double *)m.acc, f, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
-                                   g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
+                                   (double *)m.acc, f, fi, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
+                                   g.n[0], g.n[1], g.n[2], g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
             else
                 hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3((unsigned)blocks), dim3(threads), shm, st,
-                                   (T *)m.acc, f, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
-                                   g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
+                                   (T *)m.acc, f, fi, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
+                                   g.n[0], g.n[1], g.n[2], g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
             HIPCHK(hipGetLastError());
             if (timing) {
                 HIPCHK(hipEventRecord(ev.b, st));
@@ -839,26 +927,33 @@ struct Ctx final : CtxBase {
     int64_t timestep() const override { return ts; }
     bool sources_active() const override { return active; }
 
-    T *field_ptr(int comp) const { return comp < 3 ? E[comp] : H[comp - 3]; }
+    T *field_ptr(int comp, int part = 0) const { return comp < 3 ? fp[part].E[comp] : fp[part].H[comp - 3]; }
 
+    // Reference layout: (component voxel count + 2) per axis, column-major;
+    // grid indices 0..N+1 of every axis live on the device grid (an index past
+    // N+1 on a staggered axis reads as zero and is not written).  Complex
+    // fields interleave (re, im).
     void read_field(int comp, void *host, size_t bytes) override {
         require(comp >= 0 && comp < 6, "component out of range");
         int cnt[3];
         comp_count(g.n, comp, cnt);
         int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
-        size_t need = (size_t)r[0] * r[1] * r[2] * sizeof(T);
-        require(bytes == need, "field buffer size mismatch");
+        size_t nh = (size_t)r[0] * r[1] * r[2];
+        require(bytes == nh * nparts * sizeof(T), "field buffer size mismatch");
         std::vector<T> h((size_t)g.total - g.front);
-        HIPCHK(hipMemcpyAsync(h.data(), field_ptr(comp), h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
         T *o = (T *)host;
-        for (int k = 0; k < r[2]; ++k)
-            for (int j = 0; j < r[1]; ++j)
-                for (int i = 0; i < r[0]; ++i) {
-                    T v = T(0);
-                    if (i <= g.n[0] + 1 && j <= g.n[1] + 1 && k <= g.n[2] + 1) v = h[g.lin(i, j, k)];
-                    o[(size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k)] = v;
-                }
+        for (int part = 0; part < nparts; ++part) {
+            HIPCHK(hipMemcpyAsync(h.data(), field_ptr(comp, part), h.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < r[2]; ++k)
+                for (int j = 0; j < r[1]; ++j)
+                    for (int i = 0; i < r[0]; ++i) {
+                        T v = T(0);
+                        if (i <= g.n[0] + 1 && j <= g.n[1] + 1 && k <= g.n[2] + 1) v = h[g.lin(i, j, k)];
+                        const size_t hi = (size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k);
+                        o[nparts == 2 ? 2 * hi + part : hi] = v;
+                    }
+        }
     }
 
     void write_field(int comp, const void *host, size_t bytes) override {
@@ -866,19 +961,26 @@ struct Ctx final : CtxBase {
         int cnt[3];
         comp_count(g.n, comp, cnt);
         int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
-        size_t need = (size_t)r[0] * r[1] * r[2] * sizeof(T);
-        require(bytes == need, "field buffer size mismatch");
-        // Only the updated cells (grid 1..N) are taken; the boundary planes stay
-        // zero.  D/B and the PML auxiliaries are left untouched.
+        size_t nh = (size_t)r[0] * r[1] * r[2];
+        require(bytes == nh * nparts * sizeof(T), "field buffer size mismatch");
+        // D/B and the PML auxiliaries are left untouched
         std::vector<T> h((size_t)g.total - g.front, T(0));
         const T *in = (const T *)host;
-        for (int k = 1; k <= g.n[2]; ++k)
-            for (int j = 1; j <= g.n[1]; ++j)
-                for (int i = 1; i <= g.n[0]; ++i)
-                    h[g.lin(i, j, k)] = in[(size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k)];
-        HIPCHK(hipMemcpyAsync(field_ptr(comp), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
-        // a written E is taken as the linear field: no chi3 deficit carried over
-        if (comp < 3 && delta[comp]) HIPCHK(hipMemsetAsync(delta[comp], 0, (size_t)mcells * sizeof(T), st));
+        const int m[3] = {std::min(r[0] - 1, g.n[0] + 1), std::min(r[1] - 1, g.n[1] + 1),
+                          std::min(r[2] - 1, g.n[2] + 1)};
+        for (int part = 0; part < nparts; ++part) {
+            for (int k = 0; k <= m[2]; ++k)
+                for (int j = 0; j <= m[1]; ++j)
+                    for (int i = 0; i <= m[0]; ++i) {
+                        const size_t hi = (size_t)i + (size_t)r[0] * ((size_t)j + (size_t)r[1] * k);
+                        h[g.lin(i, j, k)] = in[nparts == 2 ? 2 * hi + part : hi];
+                    }
+            HIPCHK(hipMemcpyAsync(field_ptr(comp, part), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            // a written E is taken as the linear field: no chi3 deficit carried over
+            if (comp < 3 && delta[part][comp])
+                HIPCHK(hipMemsetAsync(delta[part][comp], 0, (size_t)mcells * sizeof(T), st));
+        }
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -952,20 +1054,23 @@ struct Ctx final : CtxBase {
     void reset() override {
         HIPCHK(hipStreamSynchronize(st));
         size_t fb = (size_t)g.total * sizeof(T);
-        for (int a = 0; a < 3; ++a) {
-            T *arrs[8] = {E[a], H[a], D[a], B[a], UD[a], UB[a], WD[a], WB[a]};
-            for (T *p : arrs)
-                if (p) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
-        }
+        for (int part = 0; part < nparts; ++part)
+            for (int a = 0; a < 3; ++a) {
+                const Part &f = fp[part];
+                T *arrs[8] = {f.E[a], f.H[a], f.D[a], f.B[a], f.UD[a], f.UB[a], f.WD[a], f.WB[a]};
+                for (T *p : arrs)
+                    if (p) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
+            }
         for (auto &m : dfts)
             HIPCHK(hipMemsetAsync(m.acc, 0, (size_t)m.ncell * m.nf * 2 * (acc64 ? sizeof(double) : sizeof(T)), st));
         for (auto &q : poles) {
             const size_t nb = (size_t)q.n[0] * q.n[1] * q.n[2] * sizeof(T);
-            for (int a = 0; a < 3; ++a)
-                if (q.P[a]) {
-                    HIPCHK(hipMemsetAsync(q.P[a], 0, nb, st));
-                    HIPCHK(hipMemsetAsync(q.Pp[a], 0, nb, st));
-                }
+            for (int part = 0; part < nparts; ++part)
+                for (int a = 0; a < 3; ++a)
+                    if (q.P[part][a]) {
+                        HIPCHK(hipMemsetAsync(q.P[part][a], 0, nb, st));
+                        HIPCHK(hipMemsetAsync(q.Pp[part][a], 0, nb, st));
+                    }
         }
         clear_material_state();
         HIPCHK(hipStreamSynchronize(st));
@@ -976,11 +1081,12 @@ struct Ctx final : CtxBase {
 
     void clear_material_state() {
         const size_t nb = (size_t)mcells * sizeof(T);
-        for (int a = 0; a < 3; ++a) {
-            for (int b = 0; b < 2; ++b)
-                if (fpd[b][a]) HIPCHK(hipMemsetAsync(fpd[b][a], 0, nb, st));
-            if (delta[a]) HIPCHK(hipMemsetAsync(delta[a], 0, nb, st));
-        }
+        for (int part = 0; part < 2; ++part)
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 2; ++b)
+                    if (fpd[b][part][a]) HIPCHK(hipMemsetAsync(fpd[b][part][a], 0, nb, st));
+                if (delta[part][a]) HIPCHK(hipMemsetAsync(delta[part][a], 0, nb, st));
+            }
         fcur = 0;
     }
 
@@ -1019,8 +1125,8 @@ size_t footprint(const kh_sim_desc &d) {
     Grid g;
     g.init(d.nx, d.ny, d.nz);
     size_t es = d.precision == KH_F32 ? 4 : 8;
-    // E, H, U and W per component; D and B only in EXACT mode
-    size_t arrays = d.mode == KH_MODE_EXACT ? 24 : 18;
+    // E, H, U and W per component (per part with complex fields); D and B only in EXACT mode
+    size_t arrays = (d.mode == KH_MODE_EXACT ? 24 : 18) * (d.complex_fields ? 2 : 1);
     for (int a = 0; a < 3; ++a) arrays += (d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr);
     size_t sig = 0;
     for (int a = 0; a < 3; ++a) sig += 2 * ((size_t)g.n[a] + 2) * es;

@@ -25,6 +25,8 @@ class KhSimDesc(C.Structure):
         ("eps_inv", C.c_void_p * 3), ("mu_inv", C.c_void_p * 3),
         ("eps_inv_scalar", C.c_double), ("mu_inv_scalar", C.c_double),
         ("source_cutoff_time", C.c_double),
+        ("complex_fields", C.c_int32), ("periodic", C.c_int32 * 3),
+        ("bloch_fwd", (C.c_double * 2) * 3), ("bloch_rev", (C.c_double * 2) * 3),
     ]
 
 

@@ -54,6 +54,15 @@ class HipEngine:
         d.eps_inv_scalar = spec.eps_inv_scalar
         d.mu_inv_scalar = spec.mu_inv_scalar
         d.source_cutoff_time = spec.cutoff
+        self.cplx = bool(getattr(spec, "complex_fields", False))
+        d.complex_fields = 1 if self.cplx else 0
+        per = tuple(getattr(spec, "periodic", (False, False, False)))
+        for a in range(3):
+            d.periodic[a] = 1 if per[a] else 0
+            if per[a]:
+                f, r = spec.bloch_fwd[a], spec.bloch_rev[a]
+                d.bloch_fwd[a][0], d.bloch_fwd[a][1] = f.real, f.imag
+                d.bloch_rev[a][0], d.bloch_rev[a][1] = r.real, r.imag
         self.desc = d
         ctx = C.c_void_p()
         _lib.check(self.lib.kh_create(C.byref(d), C.byref(ctx)))
@@ -158,14 +167,17 @@ class HipEngine:
         cnt = component_voxel_count(*self.spec.N, component_from_code(code))
         return tuple(c + 2 for c in cnt)
 
+    def _cdtype(self):
+        return np.complex64 if self.R is np.float32 else np.complex128
+
     def read_field(self, code: int) -> np.ndarray:
         shp = self.field_shape(code)
-        out = np.zeros(shp, dtype=self.R, order="F")
+        out = np.zeros(shp, dtype=self._cdtype() if self.cplx else self.R, order="F")
         _lib.check(self.lib.kh_read_field(self.ctx, code, _ptr(out), out.nbytes))
         return out
 
     def write_field(self, code: int, arr: np.ndarray):
-        a = np.asfortranarray(arr, dtype=self.R)
+        a = np.asfortranarray(arr, dtype=self._cdtype() if self.cplx else self.R)
         assert a.shape == self.field_shape(code)
         _lib.check(self.lib.kh_write_field(self.ctx, code, _ptr(a), a.nbytes))
 
@@ -174,12 +186,12 @@ class HipEngine:
 
     def read_pole(self, pole: int, which: int, a: int) -> np.ndarray:
         """P (which 0) or P_prev (which 1) of a pole, (Nx+2, Ny+2, Nz+2)."""
-        out = np.zeros(self.pole_shape(), dtype=self.R, order="F")
+        out = np.zeros(self.pole_shape(), dtype=self._cdtype() if self.cplx else self.R, order="F")
         _lib.check(self.lib.kh_read_pole(self.ctx, self.pole_ids[pole], which, a, _ptr(out), out.nbytes))
         return out
 
     def write_pole(self, pole: int, which: int, a: int, arr: np.ndarray):
-        h = np.asfortranarray(arr, dtype=self.R)
+        h = np.asfortranarray(arr, dtype=self._cdtype() if self.cplx else self.R)
         _lib.check(self.lib.kh_write_pole(self.ctx, self.pole_ids[pole], which, a, _ptr(h), h.nbytes))
 
     def read_dft(self, i: int) -> np.ndarray:

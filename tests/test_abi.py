@@ -30,7 +30,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_footprint_without_device():
     lib = _lib.load()
-    assert lib.kh_version() == 1
+    assert lib.kh_version() == 2
+    assert C.sizeof(_lib.KhSimDesc) == 288  # layout of kh_sim_desc in include/khronos_hip.h
     d = _lib.KhSimDesc()
     d.nx, d.ny, d.nz = 384, 192, 160
     d.precision = _lib.KH_F32
@@ -43,6 +44,9 @@ def test_version_and_footprint_without_device():
     d.mode = _lib.KH_MODE_EXACT
     assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == 0
     assert 24 * cells * 4 <= n.value <= 32 * cells * 4  # EXACT adds D and B
+    d.complex_fields = 1
+    assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == 0
+    assert 48 * cells * 4 <= n.value <= 64 * cells * 4  # complex: two parts
     d.nz = 0
     assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == -1
     assert b"extents" in lib.kh_last_error()
