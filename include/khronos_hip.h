@@ -171,6 +171,16 @@ int kh_write_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component
  * Asynchronous with respect to the host.  Replaces: step! (src/Kernels/Kernels.jl:20-88). */
 int kh_step(kh_ctx *ctx, int64_t n_steps);
 
+/* Steps per captured graph (0 = launch every kernel of every step).  kh_step
+ * captures one block of that many steps as a hipGraph and replays it while at
+ * least a block remains; the launches of a step carry no per-step arguments
+ * (the timestep, waveform values and DFT gates are read on the device), so
+ * sources and monitors stay inside the graph.  Graphs are not used while
+ * kernel timing is on.  Default 16, or the environment's KH_GRAPH_STEPS.
+ * Replaces: _try_capture_graphs! and the graph replay branch of step!
+ * (src/Kernels/Kernels.jl:40-61, :99-145; KHRONOS_CUDA_GRAPHS=0 there). */
+int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph);
+
 /* Wait for all queued work of the context. */
 int kh_sync(kh_ctx *ctx);
 

@@ -38,6 +38,7 @@ namespace kh {
 constexpr int TX = 64;       // lanes per inner-tile row
 constexpr int TY = 4;        // rows per inner tile
 constexpr int MAX_SRC = 8;
+constexpr int MAX_DFT = 16;   // DFT monitors per half step (one launch)
 constexpr int MAX_SLAB = 6;
 constexpr int NXCD = 8;
 constexpr int CBLK = 256;    // threads per cascade block
@@ -50,13 +51,40 @@ template <typename T> struct EV2;
 template <> struct EV2<float> { typedef float t __attribute__((ext_vector_type(2))); };
 template <> struct EV2<double> { typedef double t __attribute__((ext_vector_type(2))); };
 
+// The timestep lives in device memory and is advanced by tick_kernel at the
+// end of every step, so no launch of a step carries per-step arguments: the
+// time of a half step, the waveform values and the DFT gates are read from it
+// (a block of steps can then be captured once as a graph and replayed).
+template <typename T>
+__device__ __forceinline__ double step_time(int64_t ts, T dt) {
+    return (double)((T)ts * dt);  // src/Simulation.jl:22
+}
+
+// Sources are injected while t <= cutoff (src/Kernels/Kernels.jl:27-35); t
+// is non-decreasing in ts, so the host's sticky switch-off is this test.
+template <typename T>
+__device__ __forceinline__ bool sources_on(int64_t ts, T dt, double cutoff) {
+    return ts >= 0 && !(step_time(ts, dt) > cutoff);
+}
+
+// Waveform value of step n from the caller's table (zero past it).
+template <typename T>
+__device__ __forceinline__ void wave_value(const double *w, int64_t nw, int64_t n, T &fr, T &fi) {
+    fr = T(0);
+    fi = T(0);
+    if (n < 0 || n >= nw) return;
+    fr = (T)w[2 * n];
+    fi = (T)w[2 * n + 1];
+}
+
 template <typename T>
 struct SrcL {
-    int comp;       // 0..2 within the half step
-    int s[3];       // first grid index of the box
-    int d[3];       // box extent
-    const T *amp;   // complex interleaved, column-major over d
-    T fr, fi;       // waveform value for this half step
+    int comp;          // 0..2 within the half step
+    int s[3];          // first grid index of the box
+    int d[3];          // box extent
+    const T *amp;      // complex interleaved, column-major over d
+    const double *wave;  // (re, im) per step of this half step
+    int64_t nwave;
 };
 
 struct Slab {
@@ -84,6 +112,8 @@ struct CascadeArgs {
     int per_xcd;
     int nsrc;
     SrcL<T> src[MAX_SRC];
+    const int64_t *clk;   // device timestep
+    double cutoff;        // last source time
     // total polarization subtracted in the material stage (E, EXACT mode):
     // dense over the material box ulo..ulo+un-1, or null
     const T *fpd[3];
@@ -116,8 +146,11 @@ struct MatArgs {
     const T *sig[3];      // sampled D-side PML sigma per axis (N+2)
     const T *chi3;        // dense over the box, or null
     T *delta[2][3];       // FAST with chi3: linear minus stored E per cell, or null
-    T *fcur[2][3];        // total polarization the update used (null without poles)
-    T *fnext[2][3];       // FAST: previous total on entry; new total on exit.  EXACT: == fcur
+    // total polarization, two buffers (FAST: alternating by timestep parity,
+    // buffer ts % 2 holds the total the update used and buffer 1 - ts % 2 the
+    // previous one, replaced by the new total; EXACT: buffer 0 only)
+    T *fpd[2][2][3];      // [buffer][part][component]; null without poles
+    const int64_t *clk;
     int lo[3], n[3];
     int64_t sy, sz;
     int npole;
@@ -210,14 +243,18 @@ __device__ __forceinline__ T source_value(const CascadeArgs<T> &p, int comp, int
     // (src/Sources/Sources.jl:346-357, accumulated into a zeroed S array).
     T s = T(0);
     inside = false;
+    const int64_t ts = *p.clk;
+    const bool on = sources_on(ts, p.dt, p.cutoff);
     for (int q = 0; q < p.nsrc; ++q) {
         const SrcL<T> &o = p.src[q];
         if (o.comp != comp) continue;
         int ix = gx - o.s[0], iy = gy - o.s[1], iz = gz - o.s[2];
         if (ix < 0 || iy < 0 || iz < 0 || ix >= o.d[0] || iy >= o.d[1] || iz >= o.d[2]) continue;
+        T fr = T(0), fi = T(0);
+        if (on) wave_value(o.wave, o.nwave, ts, fr, fi);
         size_t ai = (size_t)ix + (size_t)o.d[0] * ((size_t)iy + (size_t)o.d[1] * (size_t)iz);
         T ar = o.amp[2 * ai], aim = o.amp[2 * ai + 1];
-        T v = (o.fr * ar) - (o.fi * aim);
+        T v = (fr * ar) - (fi * aim);
         s = s + v;
         inside = true;
     }
@@ -688,35 +725,75 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
 //   dW = m (S_n - (1 - s_p) / (1 + s_p) S_{n-1});
 //   s_o == 0: A += dW;   s_o != 0: W += dW, A += (1 + s_o) dW.
 // In the inner box (all sigma zero) this is A += m (S_n - S_{n-1}).
-// S = Re(f * amp); launched once per source, in order.
+// S = Re(f * amp).  All sources of a half step in one launch: consecutive
+// block ranges per source, in registration order (their boxes may overlap
+// only across components, which are different arrays).
 template <typename T>
-__global__ __launch_bounds__(256) void source_fix_kernel(T *A, T *W, const T *minv, T minv_s, const T *sig_own,
-                                                         const T *sig_prev, int own, int prev, const T *amp,
-                                                         int s0, int s1, int s2, int d0, int d1, int d2, T fr,
-                                                         T fi, T pr, T pi, int n0, int n1, int n2, int64_t sy,
-                                                         int64_t sz) {
-    const int64_t n = (int64_t)d0 * d1 * d2;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct FixSrc {
+    T *A;
+    T *W;
+    const T *minv;        // per-voxel or null (minv_s)
+    const T *sig_own, *sig_prev;
+    int own, prev;
+    const T *amp;
+    int s[3], d[3];
+    const double *wave;
+    int64_t nwave;
+    int first;            // first block
+};
+
+template <typename T>
+struct FixArgs {
+    int nsrc;
+    FixSrc<T> src[MAX_SRC];
+    T minv_s;
+    T dt;
+    double cutoff;
+    const int64_t *clk;
+    int n0, n1, n2;
+    int64_t sy, sz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
+    int q = 0;
+    const int b = blockIdx.x;
+    while (q + 1 < p.nsrc && b >= p.src[q + 1].first) ++q;
+    const FixSrc<T> &o = p.src[q];
+    const int64_t ts = *p.clk;
+    const bool now = sources_on(ts, p.dt, p.cutoff);
+    const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
+    if (!now && !before) return;
+    const int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
+    const int64_t c = (int64_t)(b - o.first) * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const int ix = (int)(c % d0);
-    const int64_t r = c / d0;
-    const int iy = (int)(r % d1), iz = (int)(r / d1);
-    const int g[3] = {s0 + ix, s1 + iy, s2 + iz};
+    const int ix = (int)(c % o.d[0]);
+    const int64_t r = c / o.d[0];
+    const int iy = (int)(r % o.d[1]), iz = (int)(r / o.d[1]);
+    const int g[3] = {o.s[0] + ix, o.s[1] + iy, o.s[2] + iz};
     // only updated cells consume sources (grid index 1..N)
-    if (g[0] < 1 || g[0] > n0 || g[1] < 1 || g[1] > n1 || g[2] < 1 || g[2] > n2) return;
-    const T ar = amp[2 * c], ai = amp[2 * c + 1];
+    if (g[0] < 1 || g[0] > p.n0 || g[1] < 1 || g[1] > p.n1 || g[2] < 1 || g[2] > p.n2) return;
+    T fr = 0, fi = 0, pr = 0, pi = 0;
+    if (now) wave_value(o.wave, o.nwave, ts, fr, fi);
+    if (before) wave_value(o.wave, o.nwave, ts - 1, pr, pi);
+    const T ar = o.amp[2 * c], ai = o.amp[2 * c + 1];
     const T vn = (fr * ar) - (fi * ai);
     const T vo = (pr * ar) - (pi * ai);
-    const size_t idx = (size_t)g[0] + (size_t)g[1] * sy + (size_t)g[2] * sz;
-    const T m = minv ? minv[idx] : minv_s;
-    const T so = sig_own[g[own]], sp = sig_prev[g[prev]];
+    const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+    const T m = o.minv ? o.minv[idx] : p.minv_s;
+    const T so = o.sig_own[g[o.own]], sp = o.sig_prev[g[o.prev]];
     const T dw = m * (vn - ((T(1) - sp) / (T(1) + sp)) * vo);
     if (so == T(0)) {
-        A[idx] = A[idx] + dw;
+        o.A[idx] = o.A[idx] + dw;
     } else {
-        W[idx] = W[idx] + dw;
-        A[idx] = A[idx] + (T(1) + so) * dw;
+        o.W[idx] = o.W[idx] + dw;
+        o.A[idx] = o.A[idx] + (T(1) + so) * dw;
     }
+}
+
+// End of a step: advance the device timestep (one lane writes).
+__global__ void tick_kernel(int64_t *clk) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = clk[0] + 1;
 }
 
 // ----------------------------------------------------------- materials
@@ -742,6 +819,8 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
     const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
     const int g[3] = {p.lo[0] + ux, p.lo[1] + uy, p.lo[2] + uz};
     const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+    const int bcur = FAST ? (int)(*p.clk & 1) : 0;
+    const int bnext = FAST ? 1 - bcur : 0;
     T e[NP][3], so[3];
     for (int q = 0; q < NP; ++q)
         for (int a = 0; a < 3; ++a) e[q][a] = p.E[q][a][idx];
@@ -756,7 +835,7 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
             for (int q = 0; q < NP; ++q) {
                 if (p.delta[0][0] && so[a] == T(0)) e[q][a] = e[q][a] + ratio * p.delta[q][a][c];
                 if (p.npole) {
-                    const T dw = m * (ratio * p.fnext[q][a][c] - p.fcur[q][a][c]);
+                    const T dw = m * (ratio * p.fpd[bnext][q][a][c] - p.fpd[bcur][q][a][c]);
                     if (so[a] == T(0)) {
                         e[q][a] = e[q][a] + dw;
                     } else {
@@ -845,7 +924,7 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
                 }
         }
         for (int q = 0; q < NP; ++q)
-            for (int a = 0; a < 3; ++a) p.fnext[q][a][c] = sum[q][a];
+            for (int a = 0; a < 3; ++a) p.fpd[bnext][q][a][c] = sum[q][a];
     }
 }
 
@@ -897,39 +976,68 @@ __global__ __launch_bounds__(256) void wrap_kernel(WrapArgs<T> w) {
 // acc[cell, k] += (dt * exp(i * f_k * theta)) * field[cell].  The phase
 // weights are formed once per block in float64 and kept in LDS; each
 // accumulator update is formed in float64 and stored in ACC.
+template <typename T>
+struct DftMon {
+    void *acc;              // complex (re, im) per cell and frequency, ACC
+    const T *f, *fim;       // field (imaginary part with complex fields, else null)
+    const double *freqs;
+    int s[3], d[3];
+    int nf, dec;
+    int first;              // first block
+};
+
+template <typename T>
+struct DftArgs {
+    int nmon;
+    DftMon<T> mon[MAX_DFT];
+    const int64_t *clk;
+    T dt;
+    int electric;           // E monitors sample at t + dt/2
+    int n0, n1, n2;
+    int64_t sy, sz;
+};
+
+// All DFT monitors of one half step in one launch; a monitor not due this
+// step (ts % decimation != 0) exits (src/Kernels/Kernels.jl:465-497).
 template <typename T, typename ACC>
-__global__ __launch_bounds__(256) void dft_kernel(ACC *__restrict__ acc, const T *__restrict__ field,
-                                                  const T *__restrict__ field_im,
-                                                  int d0, int d1, int d2, int s0, int s1, int s2,
-                                                  int n0, int n1, int n2, int64_t sy, int64_t sz, int nf,
-                                                  const double *__restrict__ freqs, double theta,
-                                                  double dts) {
+__global__ __launch_bounds__(256) void dft_kernel(DftArgs<T> p) {
     extern __shared__ double wsh[];
-    for (int k = threadIdx.x; k < nf; k += blockDim.x) {
-        double arg = freqs[k] * theta;
+    int q = 0;
+    const int b = blockIdx.x;
+    while (q + 1 < p.nmon && b >= p.mon[q + 1].first) ++q;
+    const DftMon<T> &m = p.mon[q];
+    const int64_t ts = *p.clk;
+    if (m.dec > 1 && (ts % m.dec) != 0) return;
+    // time_fac = complex_backend_number(im * 2 * pi * time) (src/Monitors/Monitors.jl:287)
+    double t = step_time(ts, p.dt);
+    if (p.electric) t = t + (double)(p.dt / T(2));
+    const double theta = (double)(T)(6.283185307179586 * t);
+    const double dts = (double)p.dt;
+    for (int k = threadIdx.x; k < m.nf; k += blockDim.x) {
         double sn, cs;
-        sincos(arg, &sn, &cs);
+        sincos(m.freqs[k] * theta, &sn, &cs);
         wsh[2 * k] = dts * cs;
         wsh[2 * k + 1] = dts * sn;
     }
     __syncthreads();
+    const int d0 = m.d[0], d1 = m.d[1], d2 = m.d[2];
     const int64_t ncell = (int64_t)d0 * d1 * d2;
-    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t cell = (int64_t)(b - m.first) * blockDim.x + threadIdx.x;
     if (cell >= ncell) return;
     int ix = (int)(cell % d0);
     int64_t r = cell / d0;
     int iy = (int)(r % d1);
     int iz = (int)(r / d1);
-    const int gx = s0 + ix, gy = s1 + iy, gz = s2 + iz;
+    const int gx = m.s[0] + ix, gy = m.s[1] + iy, gz = m.s[2] + iz;
     // past grid index N+1 the reference's array holds an always-zero ghost
-    const bool in = gx <= n0 + 1 && gy <= n1 + 1 && gz <= n2 + 1;
-    const size_t fi = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
-    const double v = in ? (double)field[fi] : 0.0;
-    ACC *a = acc + 2 * cell;
-    if (field_im) {
+    const bool in = gx <= p.n0 + 1 && gy <= p.n1 + 1 && gz <= p.n2 + 1;
+    const size_t fi = (size_t)gx + (size_t)gy * p.sy + (size_t)gz * p.sz;
+    const double v = in ? (double)m.f[fi] : 0.0;
+    ACC *a = (ACC *)m.acc + 2 * cell;
+    if (m.fim) {
         // complex field: (w_r + i w_i)(v + i u)
-        const double u = in ? (double)field_im[fi] : 0.0;
-        for (int k = 0; k < nf; ++k, a += 2 * ncell) {
+        const double u = in ? (double)m.fim[fi] : 0.0;
+        for (int k = 0; k < m.nf; ++k, a += 2 * ncell) {
             const double wr = wsh[2 * k], wi = wsh[2 * k + 1];
             double re = (double)a[0] + (wr * v - wi * u);
             double im = (double)a[1] + (wr * u + wi * v);
@@ -938,7 +1046,7 @@ __global__ __launch_bounds__(256) void dft_kernel(ACC *__restrict__ acc, const T
         }
         return;
     }
-    for (int k = 0; k < nf; ++k, a += 2 * ncell) {
+    for (int k = 0; k < m.nf; ++k, a += 2 * ncell) {
         double re = (double)a[0] + wsh[2 * k] * v;
         double im = (double)a[1] + wsh[2 * k + 1] * v;
         a[0] = (ACC)re;

@@ -111,6 +111,7 @@ struct CtxBase {
     virtual void read_pole(int id, int which, int comp, void *host, size_t bytes) = 0;
     virtual void write_pole(int id, int which, int comp, const void *host, size_t bytes) = 0;
     virtual void step(int64_t n) = 0;
+    virtual void set_graph(int steps) = 0;
     virtual void sync() = 0;
     virtual int64_t timestep() const = 0;
     virtual bool sources_active() const = 0;
@@ -156,7 +157,8 @@ struct Ctx final : CtxBase {
         int comp;
         int s[3], d[3];
         T *amp;
-        std::vector<double> wave;
+        double *wave = nullptr;  // device copy of the caller's table, (re, im) per step
+        int64_t nwave = 0, cap = 0;
     };
     std::vector<Source> srcs;
 
@@ -176,13 +178,13 @@ struct Ctx final : CtxBase {
     kh::PoleDev<T> *poles_dev = nullptr;
     T *chi3_dev = nullptr;                     // dense over the material box
     T *fpd[2][2][3] = {};                      // total polarization [buffer][part][comp] (FAST: two buffers, alternating)
-    int fcur = 0;
     T *delta[2][3] = {};                       // FAST with chi3, [part][comp]
     bool mat = false;
     int mlo[3] = {1, 1, 1}, mn[3] = {0, 0, 0};
     int64_t mcells = 0;
 
     double *scratch = nullptr;  // reduction partials
+    int64_t *clk = nullptr;     // device timestep
     static constexpr int NPART = 256;
 
     bool timing = false;
@@ -283,11 +285,14 @@ struct Ctx final : CtxBase {
             (c < 3 ? epsi : mui)[c % 3] = dv;
         }
         scratch = (double *)dalloc(NPART * 64 * sizeof(double));
+        clk = (int64_t *)dalloc(256);
+        graph_steps = env_int("KH_GRAPH_STEPS", 16);
         HIPCHK(hipStreamSynchronize(st));
     }
 
     ~Ctx() override {
         if (st) (void)hipStreamSynchronize(st);
+        drop_graph();
         for (auto &e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto &e : pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto &b : owned) (void)hipFree(b.p);
@@ -321,7 +326,21 @@ struct Ctx final : CtxBase {
     void set_waveform(int id, int64_t n, const double *v) override {
         require(id >= 0 && id < (int)srcs.size(), "bad source id");
         require(n >= 0 && (n == 0 || v), "bad waveform");
-        srcs[id].wave.assign(v, v + 2 * n);
+        Source &o = srcs[id];
+        if (n > o.cap) {
+            // a new table: captured step graphs hold the old pointer
+            int64_t cap = std::max<int64_t>(n, 2 * o.cap);
+            HIPCHK(hipStreamSynchronize(st));
+            o.wave = (double *)dalloc((size_t)(2 * cap) * sizeof(double));
+            o.cap = cap;
+            drop_graph();
+        }
+        if (n > 0) {
+            HIPCHK(hipMemcpyAsync(o.wave, v, (size_t)(2 * n) * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        o.nwave = n;
+        drop_graph();  // nwave is a launch argument
     }
 
     void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) override {
@@ -504,8 +523,7 @@ struct Ctx final : CtxBase {
                 q.E[pt][a] = fp[pt].E[a];
                 q.W[pt][a] = fp[pt].WD[a];
                 q.delta[pt][a] = delta[pt][a];
-                q.fcur[pt][a] = fpd[fcur][pt][a];
-                q.fnext[pt][a] = mode == KH_MODE_FAST ? fpd[1 - fcur][pt][a] : fpd[0][pt][a];
+                for (int b = 0; b < 2; ++b) q.fpd[b][pt][a] = fpd[b][pt][a];
             }
             q.minv[a] = epsi[a];
             q.sig[a] = sigD[a];
@@ -524,6 +542,7 @@ struct Ctx final : CtxBase {
         q.sz = g.sz();
         q.npole = fpd[0][0][0] ? (int)poles.size() : 0;
         q.poles = poles_dev;
+        q.clk = clk;
         EvPair ev{};
         if (timing) {
             ev = take_event(5);
@@ -542,7 +561,6 @@ struct Ctx final : CtxBase {
             HIPCHK(hipEventRecord(ev.b, st));
             evs.push_back(ev);
         }
-        if (mode == KH_MODE_FAST && q.npole) fcur = 1 - fcur;
     }
 
     double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22
@@ -711,39 +729,44 @@ struct Ctx final : CtxBase {
         return p;
     }
 
-    bool wave_at(const Source &o, int64_t n, T &fr, T &fi) const {
-        fr = T(0);
-        fi = T(0);
-        if (n < 0 || n >= (int64_t)(o.wave.size() / 2)) return false;
-        fr = (T)o.wave[2 * n];
-        fi = (T)o.wave[2 * n + 1];
-        return true;
-    }
-
     void launch_half(int e) {
         for (int part = 0; part < nparts; ++part) launch_half_part(e, part);
-        // FAST mode sources: added after the update, for every source cell
-        // (real part only: S += Re(f A), src/Sources/Sources.jl:346-357)
         const bool electric = e == 1;
+        // FAST mode sources: added after the update, for every source cell
+        // (real part only: S += Re(f A), src/Sources/Sources.jl:346-357).
+        // Launched while sources are or were on the previous step; the switch
+        // is permanent, and the kernel gates itself by the device timestep.
         if (mode == KH_MODE_FAST && (active || last_active)) {
+            kh::FixArgs<T> q{};
             const Part &f = fp[0];
+            int blocks = 0;
             for (auto &o : srcs) {
                 if ((o.comp < 3) != electric) continue;
-                T fr = 0, fi = 0, pr = 0, pi = 0;
-                if (active) wave_at(o, ts, fr, fi);
-                if (last_active) wave_at(o, ts - 1, pr, pi);
+                kh::FixSrc<T> &x = q.src[q.nsrc++];
                 const int a = o.comp % 3, prv = (a + 2) % 3;
-                int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
-                unsigned nb = (unsigned)((n + 255) / 256);
-                T *A = electric ? f.E[a] : f.H[a];
-                T *W = electric ? f.WD[a] : f.WB[a];
-                const T *mi = electric ? epsi[a] : mui[a];
-                const T *so = electric ? sigD[a] : sigB[a];
-                const T *sp = electric ? sigD[prv] : sigB[prv];
-                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(nb), dim3(256), 0, st, A, W, mi,
-                                   electric ? eps_s : mu_s, so, sp, a, prv, (const T *)o.amp, o.s[0], o.s[1],
-                                   o.s[2], o.d[0], o.d[1], o.d[2], fr, fi, pr, pi, g.n[0], g.n[1], g.n[2],
-                                   g.sy(), g.sz());
+                x.A = electric ? f.E[a] : f.H[a];
+                x.W = electric ? f.WD[a] : f.WB[a];
+                x.minv = electric ? epsi[a] : mui[a];
+                x.sig_own = electric ? sigD[a] : sigB[a];
+                x.sig_prev = electric ? sigD[prv] : sigB[prv];
+                x.own = a;
+                x.prev = prv;
+                x.amp = o.amp;
+                for (int k = 0; k < 3; ++k) { x.s[k] = o.s[k]; x.d[k] = o.d[k]; }
+                x.wave = o.wave;
+                x.nwave = o.nwave;
+                x.first = blocks;
+                blocks += (int)(((int64_t)o.d[0] * o.d[1] * o.d[2] + 255) / 256);
+            }
+            if (q.nsrc) {
+                q.minv_s = electric ? eps_s : mu_s;
+                q.dt = dt;
+                q.cutoff = cutoff;
+                q.clk = clk;
+                q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
+                q.sy = g.sy();
+                q.sz = g.sz();
+                hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(blocks), dim3(256), 0, st, q);
                 HIPCHK(hipGetLastError());
             }
         }
@@ -821,9 +844,11 @@ struct Ctx final : CtxBase {
                 l.comp = o.comp % 3;
                 for (int a = 0; a < 3; ++a) { l.s[a] = o.s[a]; l.d[a] = o.d[a]; }
                 l.amp = o.amp;
-                if (active) wave_at(o, ts, l.fr, l.fi);
-                else { l.fr = T(0); l.fi = T(0); }
+                l.wave = o.wave;
+                l.nwave = o.nwave;
             }
+            p.clk = clk;
+            p.cutoff = cutoff;
             const dim3 cgrid(p.per_xcd * kh::NXCD), cblk(kh::CBLK);
             if (electric) hipLaunchKernelGGL((kh::cascade_e_kernel<T>), cgrid, cblk, 0, st, p);
             else hipLaunchKernelGGL((kh::cascade_h_kernel<T>), cgrid, cblk, 0, st, p);
@@ -873,30 +898,43 @@ struct Ctx final : CtxBase {
         }
     }
 
-    void launch_dfts(bool electric, double t) {
-        // time_fac = complex_backend_number(im * 2 * pi * time)  (src/Monitors/Monitors.jl:287)
-        double theta = (double)(T)(6.283185307179586 * t);
-        for (auto &m : dfts) {
-            if ((m.comp < 3) != electric) continue;
-            if (m.dec > 1 && (ts % m.dec) != 0) continue;
-            const T *f = electric ? fp[0].E[m.comp] : fp[0].H[m.comp - 3];
-            const T *fi = nparts == 2 ? (electric ? fp[1].E[m.comp] : fp[1].H[m.comp - 3]) : nullptr;
-            int threads = 256;
-            int64_t blocks = (m.ncell + threads - 1) / threads;
-            size_t shm = (size_t)m.nf * 2 * sizeof(double);
+    // All DFT monitors of a half step, in launches of up to MAX_DFT monitors;
+    // each gates itself by its decimation against the device timestep.
+    void launch_dfts(bool electric) {
+        std::vector<const Dft *> list;
+        for (auto &m : dfts)
+            if ((m.comp < 3) == electric) list.push_back(&m);
+        for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
+            kh::DftArgs<T> q{};
+            int blocks = 0, nfmax = 1;
+            for (size_t i = i0; i < list.size() && q.nmon < kh::MAX_DFT; ++i) {
+                const Dft &m = *list[i];
+                kh::DftMon<T> &x = q.mon[q.nmon++];
+                x.acc = m.acc;
+                x.f = electric ? fp[0].E[m.comp] : fp[0].H[m.comp - 3];
+                x.fim = nparts == 2 ? (electric ? fp[1].E[m.comp] : fp[1].H[m.comp - 3]) : nullptr;
+                x.freqs = m.freqs;
+                for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
+                x.nf = m.nf;
+                x.dec = m.dec;
+                x.first = blocks;
+                blocks += (int)((m.ncell + 255) / 256);
+                nfmax = std::max(nfmax, m.nf);
+            }
+            q.clk = clk;
+            q.dt = dt;
+            q.electric = electric ? 1 : 0;
+            q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
+            q.sy = g.sy();
+            q.sz = g.sz();
+            const size_t shm = (size_t)nfmax * 2 * sizeof(double);
             EvPair ev{};
             if (timing) {
                 ev = take_event(4);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
-            if (acc64)
-                hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3((unsigned)blocks), dim3(threads), shm, st,
-                                   (double *)m.acc, f, fi, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
-                                   g.n[0], g.n[1], g.n[2], g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
-            else
-                hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3((unsigned)blocks), dim3(threads), shm, st,
-                                   (T *)m.acc, f, fi, m.d[0], m.d[1], m.d[2], m.s[0], m.s[1], m.s[2],
-                                   g.n[0], g.n[1], g.n[2], g.sy(), g.sz(), m.nf, m.freqs, theta, (double)dt);
+            if (acc64) hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3(blocks), dim3(256), shm, st, q);
+            else hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3(blocks), dim3(256), shm, st, q);
             HIPCHK(hipGetLastError());
             if (timing) {
                 HIPCHK(hipEventRecord(ev.b, st));
@@ -905,22 +943,107 @@ struct Ctx final : CtxBase {
         }
     }
 
+    // One time step's launches (src/Kernels/Kernels.jl:64-87 order); every
+    // time-dependent quantity comes from the device timestep.
+    void launch_step() {
+        launch_half(0);
+        launch_dfts(false);
+        launch_half(1);
+        if (mat) launch_material();
+        launch_dfts(true);
+        hipLaunchKernelGGL(kh::tick_kernel, dim3(1), dim3(64), 0, st, clk);
+        HIPCHK(hipGetLastError());
+    }
+
+    // host mirror of the source switch for the step about to run
+    void begin_step() {
+        if (active && round_time(ts) > cutoff) active = false;  // src/Kernels/Kernels.jl:27-35
+    }
+    void end_step() {
+        last_active = active;
+        ++ts;
+    }
+
+    // ---------------------------------------------------------------- graphs
+    // A block of graph_steps steps is captured once and replayed (the
+    // reference captures its H and E updates after the sources switch off,
+    // src/Kernels/Kernels.jl:99-145; here sources and monitors are inside the
+    // graph too, gated by the device timestep).  Re-captured when a launch
+    // argument changes (a new waveform table, the FAST source correction
+    // dropping out after the switch-off).
+    int graph_steps = 16;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_src = false;   // the captured steps include the FAST source correction
+    bool graph_failed = false;
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+
+    bool capture_graph() {
+        drop_graph();
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        bool ok = true;
+        try {
+            for (int i = 0; i < graph_steps; ++i) launch_step();
+        } catch (const KhError &) {
+            ok = false;
+        }
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (!ok || e != hipSuccess || !gr) {
+            if (gr) (void)hipGraphDestroy(gr);
+            (void)hipGetLastError();
+            return false;
+        }
+        graph = gr;
+        if (hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            drop_graph();
+            return false;
+        }
+        graph_src = active || last_active;
+        return true;
+    }
+
     void step(int64_t n) override {
         require(n >= 0, "n_steps must be >= 0");
         if (!planned) make_plans();
         stepped = true;
-        for (int64_t it = 0; it < n; ++it) {
-            double t = round_time(ts);
-            if (active && t > cutoff) active = false;  // src/Kernels/Kernels.jl:27-35
-            double th = t + (double)(dt / T(2));
-            launch_half(0);
-            launch_dfts(false, t);
-            launch_half(1);
-            if (mat) launch_material();
-            launch_dfts(true, th);
-            last_active = active;
-            ++ts;
+        int64_t done = 0;
+        while (done < n) {
+            begin_step();
+            const bool src_now = active || last_active;
+            if (graph_steps > 0 && !timing && !graph_failed && n - done >= graph_steps) {
+                if (!graph_exec || graph_src != src_now) {
+                    if (!capture_graph()) graph_failed = true;
+                }
+                if (graph_exec) {
+                    HIPCHK(hipGraphLaunch(graph_exec, st));
+                    end_step();
+                    for (int i = 1; i < graph_steps; ++i) {
+                        begin_step();
+                        end_step();
+                    }
+                    done += graph_steps;
+                    continue;
+                }
+            }
+            launch_step();
+            end_step();
+            ++done;
         }
+    }
+
+    void set_graph(int steps) override {
+        require(steps >= 0 && steps <= 4096, "graph steps must be 0..4096");
+        graph_steps = steps;
+        graph_failed = false;
+        drop_graph();
     }
 
     void sync() override { HIPCHK(hipStreamSynchronize(st)); }
@@ -1073,6 +1196,7 @@ struct Ctx final : CtxBase {
                     }
         }
         clear_material_state();
+        HIPCHK(hipMemsetAsync(clk, 0, 256, st));
         HIPCHK(hipStreamSynchronize(st));
         ts = 0;
         active = true;
@@ -1087,7 +1211,6 @@ struct Ctx final : CtxBase {
                     if (fpd[b][part][a]) HIPCHK(hipMemsetAsync(fpd[b][part][a], 0, nb, st));
                 if (delta[part][a]) HIPCHK(hipMemsetAsync(delta[part][a], 0, nb, st));
             }
-        fcur = 0;
     }
 
     void set_timing(int n) override {
@@ -1243,6 +1366,13 @@ int kh_step(kh_ctx *ctx, int64_t n_steps) {
     return guarded([&] {
         CTX_OK;
         ctx->impl->step(n_steps);
+    });
+}
+
+int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->set_graph(steps_per_graph);
     });
 }
 

@@ -149,6 +149,10 @@ class HipEngine:
         _lib.check(self.lib.kh_step(self.ctx, int(n)))
         self.ts += n
 
+    def set_graph(self, steps: int):
+        """Steps per captured hipGraph (0: no graphs)."""
+        _lib.check(self.lib.kh_set_graph(self.ctx, int(steps)))
+
     def sync(self):
         _lib.check(self.lib.kh_sync(self.ctx))
 
