@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "khronos.jl_amd"))
 
@@ -73,6 +75,47 @@ def measured_traffic(kernel_prefix: str):
     return None
 
 
+def serving_bench(args):
+    """Serving form: a trace of independent FP32 requests (grids drawn from
+    64^3, 96^3, 128^3, 192x192x128, 256^3; 10-cell PML; 1-3 random boxes or
+    spheres; Ez dipole; one flux plane at 20 frequencies; 1000-3000 steps),
+    served with continuous batching and, as the baseline, one at a time."""
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    from khronos_hip import serving
+    reqs = serving.serving_trace(args.requests, seed=0)
+    free, total = torch.cuda.mem_get_info()
+    budget = 0.85 * free
+    sys.stderr.write(f"serving {len(reqs)} requests ({sum(r.cells * r.steps for r in reqs) / 1e9:.1f} G cell-updates)\n")
+    served = serving.Server(budget_bytes=budget, max_active=args.max_active, chunk=args.chunk).run(reqs)
+    sys.stderr.write(f"served: makespan {served.makespan:.2f} s; one at a time next\n")
+    solo = serving.run_one_at_a_time(reqs)
+    sys.stderr.write(f"one at a time: makespan {solo.makespan:.2f} s\n")
+    same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
+    line = {
+        "metric": "serving throughput (Mcell-updates/s), continuous batching vs one at a time",
+        "value": round(served.throughput, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests, grids 64^3..256^3, 1000-3000 steps, all arrived at t=0",
+                   "max_active": args.max_active, "chunk_steps": args.chunk,
+                   "budget_GB": round(budget / 1e9, 1), "cell_updates": served.cell_updates},
+        "one_at_a_time": round(solo.throughput, 2),
+        "speedup": round(served.throughput / solo.throughput, 3),
+        "latency_s": {"served_p50": round(served.latency_pct(50), 3), "served_p99": round(served.latency_pct(99), 3),
+                      "solo_p50": round(solo.latency_pct(50), 3), "solo_p99": round(solo.latency_pct(99), 3)},
+        "makespan_s": {"served": round(served.makespan, 3), "solo": round(solo.makespan, 3)},
+        "fluxes_identical": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +126,13 @@ def main(argv=None):
     ap.add_argument("--graph-steps", type=int, default=16, help="steps per captured hipGraph (0: eager launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (counter calibration)")
+    ap.add_argument("--serving", action="store_true", help="serving form instead of the headline benchmark")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--max-active", type=int, default=6)
+    ap.add_argument("--chunk", type=int, default=256)
     args = ap.parse_args(argv)
+    if args.serving:
+        return serving_bench(args)
     if args.calibrate_copy:
         import torch
         src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")

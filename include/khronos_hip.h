@@ -181,6 +181,12 @@ int kh_step(kh_ctx *ctx, int64_t n_steps);
  * (src/Kernels/Kernels.jl:40-61, :99-145; KHRONOS_CUDA_GRAPHS=0 there). */
 int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph);
 
+/* Whether all queued work of the context has finished (*idle = 1), without
+ * waiting; a scheduler serving many contexts polls this to keep each one's
+ * stream fed.  Replaces: the task fetch of run_batch_concurrent
+ * (src/Batch.jl:183-263). */
+int kh_query(kh_ctx *ctx, int32_t *idle);
+
 /* Wait for all queued work of the context. */
 int kh_sync(kh_ctx *ctx);
 

@@ -113,6 +113,7 @@ struct CtxBase {
     virtual void step(int64_t n) = 0;
     virtual void set_graph(int steps) = 0;
     virtual void sync() = 0;
+    virtual bool idle() = 0;
     virtual int64_t timestep() const = 0;
     virtual bool sources_active() const = 0;
     virtual void read_field(int comp, void *host, size_t bytes) = 0;
@@ -1047,6 +1048,12 @@ struct Ctx final : CtxBase {
     }
 
     void sync() override { HIPCHK(hipStreamSynchronize(st)); }
+    bool idle() override {
+        hipError_t e = hipStreamQuery(st);
+        if (e == hipErrorNotReady) return false;
+        HIPCHK(e);
+        return true;
+    }
     int64_t timestep() const override { return ts; }
     bool sources_active() const override { return active; }
 
@@ -1373,6 +1380,14 @@ int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph) {
     return guarded([&] {
         CTX_OK;
         ctx->impl->set_graph(steps_per_graph);
+    });
+}
+
+int kh_query(kh_ctx *ctx, int32_t *idle) {
+    return guarded([&] {
+        CTX_OK;
+        require(idle != nullptr, "idle is null");
+        *idle = ctx->impl->idle() ? 1 : 0;
     });
 }
 

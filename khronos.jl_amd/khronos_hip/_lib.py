@@ -48,6 +48,7 @@ SIGNATURES = {
     "kh_write_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),
     "kh_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "kh_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "kh_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "kh_sync": (C.c_int, [C.c_void_p]),
     "kh_get_timestep": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "kh_sources_active": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),

@@ -130,7 +130,9 @@ class HipEngine:
             need = n_end if cap is None else min(n_end, cap)
             if have >= need:
                 continue
-            target = need if cap is not None else max(need, have + self.WAVE_BLOCK)
+            # with a cutoff the whole table goes up at once (one device table,
+            # one captured graph); without, in blocks
+            target = cap if cap is not None else max(need, have + self.WAVE_BLOCK)
             times = self._wave_times(sd.component.electric, have, target)
             new = np.empty(2 * len(times))
             for i, t in enumerate(times):
@@ -152,6 +154,18 @@ class HipEngine:
     def set_graph(self, steps: int):
         """Steps per captured hipGraph (0: no graphs)."""
         _lib.check(self.lib.kh_set_graph(self.ctx, int(steps)))
+
+    def idle(self) -> bool:
+        """All queued work finished (non-blocking)."""
+        v = C.c_int32()
+        _lib.check(self.lib.kh_query(self.ctx, C.byref(v)))
+        return bool(v.value)
+
+    def close(self):
+        """Free the device context now (instead of at garbage collection)."""
+        if getattr(self, "ctx", None):
+            _lib.check(self.lib.kh_destroy(self.ctx))
+            self.ctx = None
 
     def sync(self):
         _lib.check(self.lib.kh_sync(self.ctx))
