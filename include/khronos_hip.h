@@ -36,8 +36,12 @@ extern "C" {
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
-/* Field component codes (src/DataStructures.jl:24-35). */
-enum kh_component { KH_EX = 0, KH_EY = 1, KH_EZ = 2, KH_HX = 3, KH_HY = 4, KH_HZ = 5 };
+/* Field component codes (src/DataStructures.jl:24-35).  D and B
+ * (KH_DX..KH_BZ) are accepted by kh_add_dft only. */
+enum kh_component {
+    KH_EX = 0, KH_EY = 1, KH_EZ = 2, KH_HX = 3, KH_HY = 4, KH_HZ = 5,
+    KH_DX = 6, KH_DY = 7, KH_DZ = 8, KH_BX = 9, KH_BY = 10, KH_BZ = 11
+};
 
 /* KH_MODE_FAST: cells with no PML, source or polarisation term advance E and
  *   H directly (E += eps^-1 * dt * curl H); other cells run the full D/B
@@ -128,11 +132,24 @@ int kh_set_waveform(kh_ctx *ctx, int32_t source_id, int64_t n_steps, const doubl
 
 /* Register a DFT monitor on one component: every `decimation`-th step,
  * F[i,k] += dt * exp(i*2*pi*freqs[k]*t) * field[start + i]
- * (H at t, E at t + dt/2).  Replaces: init_monitors(DFTMonitor) and
- * update_dft_monitor! (src/Monitors/Monitors.jl:202-290, :361-379). */
+ * (H and B at t, E and D at t + dt/2).  D and B monitors record the D/B
+ * arrays in KH_MODE_EXACT; in KH_MODE_FAST, which keeps no D/B, the value is
+ * rebuilt from the carried stage, (W where the own-axis PML sigma is
+ * non-zero, else E or H) / eps^-1 (mu^-1) -- exact outside dispersive
+ * materials and source cells.  Replaces: init_monitors(DFTMonitor) and
+ * update_dft_monitor! (src/Monitors/Monitors.jl:202-290, :361-379); the
+ * design-region D monitors of install_design_region_monitors!
+ * (src/Adjoint/DesignRegion.jl:290-312). */
 int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3],
                const int32_t dims[3], int32_t n_freq, const double *freqs,
                int32_t decimation, int32_t *dft_id);
+
+/* Replace the per-voxel inverse permittivity of component 0..2 (reference
+ * geometry layout, as in kh_sim_desc.eps_inv); the context must have been
+ * created with per-voxel permittivity.  Takes effect from the next step;
+ * fields are untouched.  Replaces: update_design! / _write_design_to_sim!
+ * (src/Adjoint/DesignRegion.jl:211-288). */
+int kh_set_eps_inv(kh_ctx *ctx, int32_t component, const void *eps_inv);
 
 /* Kerr coefficient per cell, (nx, ny, nz) column-major, backend precision,
  * painted on the cell centres (src/Geometry.jl:610-636).  After every E update

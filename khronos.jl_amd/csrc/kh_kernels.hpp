@@ -984,6 +984,14 @@ struct DftMon {
     int s[3], d[3];
     int nf, dec;
     int first;              // first block
+    // FAST-mode D/B: value = (W where the own-axis sigma is non-zero, else
+    // the field) / inverse material; rebuild = 0 reads f as it is
+    int rebuild;
+    const T *w, *wim;
+    const T *minv;          // per-voxel (padded) or null: minv_s
+    T minv_s;
+    const T *sig_own;       // sampled sigma along the own axis (N+2)
+    int own;
 };
 
 template <typename T>
@@ -1032,11 +1040,21 @@ __global__ __launch_bounds__(256) void dft_kernel(DftArgs<T> p) {
     // past grid index N+1 the reference's array holds an always-zero ghost
     const bool in = gx <= p.n0 + 1 && gy <= p.n1 + 1 && gz <= p.n2 + 1;
     const size_t fi = (size_t)gx + (size_t)gy * p.sy + (size_t)gz * p.sz;
-    const double v = in ? (double)m.f[fi] : 0.0;
+    const T *src = m.f, *srci = m.fim;
+    T mi = T(1);
+    if (m.rebuild && in) {
+        const int go = m.own == 0 ? gx : (m.own == 1 ? gy : gz);
+        if (m.sig_own[go] != T(0)) {
+            src = m.w;
+            srci = m.wim;
+        }
+        mi = m.minv ? m.minv[fi] : m.minv_s;
+    }
+    const double v = in ? (double)(m.rebuild ? src[fi] / mi : src[fi]) : 0.0;
     ACC *a = (ACC *)m.acc + 2 * cell;
     if (m.fim) {
         // complex field: (w_r + i w_i)(v + i u)
-        const double u = in ? (double)m.fim[fi] : 0.0;
+        const double u = in ? (double)(m.rebuild ? srci[fi] / mi : srci[fi]) : 0.0;
         for (int k = 0; k < m.nf; ++k, a += 2 * ncell) {
             const double wr = wsh[2 * k], wi = wsh[2 * k + 1];
             double re = (double)a[0] + (wr * v - wi * u);

@@ -107,6 +107,7 @@ struct CtxBase {
     virtual void set_waveform(int id, int64_t n, const double *v) = 0;
     virtual void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) = 0;
     virtual void set_chi3(const void *chi3) = 0;
+    virtual void set_eps_inv(int comp, const void *a) = 0;
     virtual void add_pole(const double *c, int is_drude, const void *sigma, int32_t *id) = 0;
     virtual void read_pole(int id, int which, int comp, void *host, size_t bytes) = 0;
     virtual void write_pole(int id, int which, int comp, const void *host, size_t bytes) = 0;
@@ -272,23 +273,36 @@ struct Ctx final : CtxBase {
         for (int c = 0; c < 6; ++c) {
             const void *src = c < 3 ? d.eps_inv[c] : d.mu_inv[c - 3];
             if (!src) continue;
-            int cnt[3];
-            comp_count(g.n, c, cnt);
-            std::vector<T> h((size_t)g.total - g.front, T(1));
-            const T *ps = (const T *)src;
-            for (int k = 1; k <= g.n[2]; ++k)
-                for (int j = 1; j <= g.n[1]; ++j)
-                    for (int i = 1; i <= g.n[0]; ++i)
-                        h[g.lin(i, j, k)] = ps[(size_t)(i - 1) + (size_t)cnt[0] * ((size_t)(j - 1) + (size_t)cnt[1] * (k - 1))];
             T *dv = field_alloc();
-            HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));
+            upload_material(dv, c, (const T *)src);
             (c < 3 ? epsi : mui)[c % 3] = dv;
         }
         scratch = (double *)dalloc(NPART * 64 * sizeof(double));
         clk = (int64_t *)dalloc(256);
         graph_steps = env_int("KH_GRAPH_STEPS", 16);
         HIPCHK(hipStreamSynchronize(st));
+    }
+
+    // reference geometry layout (component voxel count, no ghosts) -> the
+    // padded grid, cells 1..N; 1 elsewhere
+    void upload_material(T *dv, int c, const T *ps) {
+        int cnt[3];
+        comp_count(g.n, c, cnt);
+        std::vector<T> h((size_t)g.total - g.front, T(1));
+        for (int k = 1; k <= g.n[2]; ++k)
+            for (int j = 1; j <= g.n[1]; ++j)
+                for (int i = 1; i <= g.n[0]; ++i)
+                    h[g.lin(i, j, k)] = ps[(size_t)(i - 1) + (size_t)cnt[0] * ((size_t)(j - 1) + (size_t)cnt[1] * (k - 1))];
+        HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void set_eps_inv(int comp, const void *a) override {
+        require(comp >= 0 && comp < 3, "component must be 0..2");
+        require(a != nullptr, "eps_inv array is null");
+        require(epsi[comp] != nullptr, "the context has no per-voxel permittivity (create it with eps_inv arrays)");
+        HIPCHK(hipStreamSynchronize(st));
+        upload_material(epsi[comp], comp, (const T *)a);
     }
 
     ~Ctx() override {
@@ -345,11 +359,11 @@ struct Ctx final : CtxBase {
     }
 
     void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) override {
-        require(comp >= 0 && comp < 6, "component out of range");
+        require(comp >= 0 && comp < 12, "component out of range");
         require(nf >= 1 && nf <= 4096, "n_freq must be in 1..4096");
         require(dec >= 1, "decimation must be >= 1");
         int cnt[3];
-        comp_count(g.n, comp, cnt);
+        comp_count(g.n, comp % 6, cnt);  // D, B sit where E, H do
         // the reference's array of the component spans indices 0..count+1; the
         // device grid holds 0..N+1, and an index past it (the ghost beyond a
         // staggered component's extra plane, never written) reads as zero
@@ -904,7 +918,7 @@ struct Ctx final : CtxBase {
     void launch_dfts(bool electric) {
         std::vector<const Dft *> list;
         for (auto &m : dfts)
-            if ((m.comp < 3) == electric) list.push_back(&m);
+            if ((m.comp % 6 < 3) == electric) list.push_back(&m);
         for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
             kh::DftArgs<T> q{};
             int blocks = 0, nfmax = 1;
@@ -912,8 +926,24 @@ struct Ctx final : CtxBase {
                 const Dft &m = *list[i];
                 kh::DftMon<T> &x = q.mon[q.nmon++];
                 x.acc = m.acc;
-                x.f = electric ? fp[0].E[m.comp] : fp[0].H[m.comp - 3];
-                x.fim = nparts == 2 ? (electric ? fp[1].E[m.comp] : fp[1].H[m.comp - 3]) : nullptr;
+                const int a = m.comp % 3;
+                const bool db = m.comp >= 6;  // D or B
+                if (db && mode == KH_MODE_EXACT) {
+                    x.f = electric ? fp[0].D[a] : fp[0].B[a];
+                    x.fim = nparts == 2 ? (electric ? fp[1].D[a] : fp[1].B[a]) : nullptr;
+                } else {
+                    x.f = electric ? fp[0].E[a] : fp[0].H[a];
+                    x.fim = nparts == 2 ? (electric ? fp[1].E[a] : fp[1].H[a]) : nullptr;
+                }
+                if (db && mode == KH_MODE_FAST) {
+                    x.rebuild = 1;
+                    x.w = electric ? fp[0].WD[a] : fp[0].WB[a];
+                    x.wim = nparts == 2 ? (electric ? fp[1].WD[a] : fp[1].WB[a]) : nullptr;
+                    x.minv = electric ? epsi[a] : mui[a];
+                    x.minv_s = electric ? eps_s : mu_s;
+                    x.sig_own = electric ? sigD[a] : sigB[a];
+                    x.own = a;
+                }
                 x.freqs = m.freqs;
                 for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
                 x.nf = m.nf;
@@ -1334,6 +1364,13 @@ int kh_add_dft(kh_ctx *ctx, int32_t component, const int32_t start_idx[3], const
         CTX_OK;
         require(start_idx && dims && freqs && dft_id, "null argument");
         ctx->impl->add_dft(component, start_idx, dims, n_freq, freqs, decimation, dft_id);
+    });
+}
+
+int kh_set_eps_inv(kh_ctx *ctx, int32_t component, const void *eps_inv) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->set_eps_inv(component, eps_inv);
     });
 }
 

@@ -199,6 +199,12 @@ class HipEngine:
         assert a.shape == self.field_shape(code)
         _lib.check(self.lib.kh_write_field(self.ctx, code, _ptr(a), a.nbytes))
 
+    def set_eps_inv(self, a: int, arr: np.ndarray):
+        """Replace the per-voxel inverse permittivity of component a (reference
+        geometry layout); between runs in FAST mode."""
+        h = np.asfortranarray(arr, dtype=self.R)
+        _lib.check(self.lib.kh_set_eps_inv(self.ctx, int(a), _ptr(h)))
+
     def pole_shape(self):
         return tuple(n + 2 for n in self.spec.N)
 

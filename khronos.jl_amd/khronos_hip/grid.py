@@ -52,6 +52,14 @@ Ez = _mk("Ez", 2, True)
 Hx = _mk("Hx", 0, False)
 Hy = _mk("Hy", 1, False)
 Hz = _mk("Hz", 2, False)
+# displacement / induction components: the Yee positions of E / H
+# (src/DataStructures.jl:12-49); DFT monitors may record them
+Dx = _mk("Dx", 0, True)
+Dy = _mk("Dy", 1, True)
+Dz = _mk("Dz", 2, True)
+Bx = _mk("Bx", 0, False)
+By = _mk("By", 1, False)
+Bz = _mk("Bz", 2, False)
 
 
 class Center(Field):
@@ -63,8 +71,9 @@ class Center(Field):
 E_COMPONENTS = (Ex, Ey, Ez)
 H_COMPONENTS = (Hx, Hy, Hz)
 
-# C-ABI component codes (include/khronos_hip.h: KH_EX..KH_HZ)
-COMPONENT_CODE = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+# C-ABI component codes (include/khronos_hip.h: KH_EX..KH_BZ)
+COMPONENT_CODE = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5,
+                  "Dx": 6, "Dy": 7, "Dz": 8, "Bx": 9, "By": 10, "Bz": 11}
 
 
 def component_code(c: Field) -> int:
@@ -72,7 +81,7 @@ def component_code(c: Field) -> int:
 
 
 def component_from_code(code: int) -> Field:
-    return (Ex, Ey, Ez, Hx, Hy, Hz)[code]()
+    return (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)[code]()
 
 
 def is_electric(c: Field) -> bool:

@@ -281,6 +281,14 @@ int orc_set_periodic(orc_sim *s, int cplx, const int *periodic, const double *fw
     return 0;
 }
 
+/* Replace a per-voxel inverse material array (geometry layout), as the
+ * design update writes eps_inv (src/Adjoint/DesignRegion.jl:248-288). */
+int orc_set_minv(orc_sim *s, int comp, const real *m) {
+    if (!s->minv[comp]) return 1;
+    memcpy(s->minv[comp], m, vol(s->gdim[comp]) * sizeof(real));
+    return 0;
+}
+
 /* chi3 array (nx, ny, nz) painted on the cell centres (src/Geometry.jl:610-636) */
 int orc_set_chi3(orc_sim *s, const real *chi3) {
     size_t n = (size_t)s->nx * s->ny * (s->nz < 1 ? 1 : s->nz);
@@ -641,13 +649,16 @@ static void update_dfts(orc_sim *s, int magnetic, double t) {
     /* src/Kernels/Kernels.jl:465-497 gate, src/Monitors/Monitors.jl:361-379 */
     for (int q = 0; q < s->ndft; q++) {
         orc_dft *d = &s->dft[q];
-        if ((d->comp >= 3) != magnetic) continue;
+        /* comp 6..11: D / B (the T arrays) at the positions of E / H */
+        int base = d->comp % 6;
+        if ((base >= 3) != magnetic) continue;
         if (d->decim > 1 && (s->timestep % d->decim) != 0) continue;
         /* time_fac = complex_backend_number(im * 2 * pi * time) */
         double theta = (double)(real)(6.283185307179586 * t);
         double dts = (double)s->dt;
-        const real *F = s->F[d->comp];
-        const int *fd = s->dim[d->comp];
+        const real *F = d->comp >= 6 ? s->T[base] : s->F[base];
+        const real *Fim = s->cplx ? (d->comp >= 6 ? s->Ti[base] : s->Fi[base]) : NULL;
+        const int *fd = s->dim[base];
         size_t ncell = vol(d->dims);
         for (int k = 0; k < d->nf; k++) {
             double arg = d->freqs[k] * theta;
@@ -663,7 +674,7 @@ static void update_dfts(orc_sim *s, int magnetic, double t) {
                         /* (w_r + i w_i)(v + i u): complex fields add the u terms */
                         double re = wr * v, im = wi * v;
                         if (s->cplx) {
-                            double u = (double)s->Fi[d->comp][fi];
+                            double u = (double)Fim[fi];
                             re = wr * v - wi * u;
                             im = wr * u + wi * v;
                         }

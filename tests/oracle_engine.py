@@ -55,6 +55,8 @@ def load_path(real, path):
     lib.orc_set_periodic.restype = C.c_int
     lib.orc_set_periodic.argtypes = [P, C.c_int, P, P, P]
     lib.orc_set_chi3.argtypes = [P, P]
+    lib.orc_set_minv.restype = C.c_int
+    lib.orc_set_minv.argtypes = [P, C.c_int, P]
     lib.orc_add_pole.restype = C.c_int
     lib.orc_add_pole.argtypes = [P, P, C.c_int, P]
     lib.orc_read_pole.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
@@ -213,6 +215,18 @@ class OracleEngine:
         if self.cplx:
             return self.read(which, code) + 1j * self.read(which + 6, code)
         return self.read(which, code)
+
+    def set_eps_inv(self, a, arr):
+        h = np.asfortranarray(arr, dtype=self.R)
+        assert self.lib.orc_set_minv(self.sim, a, _p(h)) == 0
+
+    def close(self):
+        if getattr(self, "sim", None):
+            self.lib.orc_destroy(self.sim)
+            self.sim = None
+
+    def set_graph(self, steps):
+        pass
 
     def pole_shape(self):
         return tuple(n + 2 for n in self.spec.N)
