@@ -9,22 +9,27 @@
 //
 // A half step (H or E) is split by cell class:
 //
-//   inner_*_kernel  -- the box of cells whose PML sigma is zero on all three
-//     axes (FAST mode).  E/H advance directly (E += eps^-1 * dt * curl H, the
+//   inner_*_kernel  -- FAST mode, the box of cells whose PML sigma is zero on
+//     all three axes.  E/H advance directly (E += eps^-1 * dt * curl H, the
 //     D/B-eliminated form of src/Kernels/CUDAKernels.jl:46-74).  Each lane owns
 //     two adjacent x cells (one 8/16-byte access per array) of ZB planes; with
 //     ZB > 1 the loads of plane z+1 are issued before the stores of plane z.
 //     Streams touched once (updated field, material) use nontemporal access.
 //
-//   cascade_*_kernel -- every other cell (the six PML slabs around the inner
-//     box, or the whole grid in EXACT mode), one cell per thread: the
+//   pml_*_kernel -- FAST mode, the six PML slabs around the inner box: the
+//     sigma cascade with D/B eliminated (carried on E/H or W), x pairs per lane.
+//
+//   cascade_*_kernel -- EXACT mode, the whole grid, one cell per thread: the
 //     reference's sigma cascade keeping D/B, U and W (src/Kernels/Helpers.jl:
 //     113-139, :323-340) in the CPU path's operation order, with current
-//     sources added to the material stage.
+//     sources and the total polarization in the material stage.
 //
-//   source_fix_kernel -- FAST mode, source cells inside the inner box: the
-//     reference computes E = eps^-1 (D + S_n); with D eliminated this is
-//     E += eps^-1 (S_n - S_{n-1}).
+// After the E update: source_fix_kernel (FAST: the terms the eliminated D/B
+// would have carried), wrap_kernel (periodic / Bloch ghosts), material_kernel
+// (chi3 and dispersive poles); then dft_kernel for the due monitors of each
+// half step and tick_kernel, which advances the device timestep every
+// per-step quantity is computed from.  Complex fields run the real kernels on
+// the real and imaginary parts in turn.
 //
 // Blocks are assigned so that consecutive logical blocks (neighbouring rows
 // and planes) run on the same XCD and share its L2.

@@ -660,7 +660,9 @@ struct Ctx final : CtxBase {
                 zero_run(h, g.n[a], P.lo[a], P.hi[a]);
                 any_empty |= P.lo[a] > P.hi[a];
             }
-            P.inner = (mode == KH_MODE_FAST) && !any_empty;
+            // KH_INNER=0: no separate inner-box launch; the PML kernel, whose
+            // sigma = 0 branch is the inner update, covers the whole grid
+            P.inner = (mode == KH_MODE_FAST) && !any_empty && env_int("KH_INNER", 1) != 0;
             if (!P.inner)
                 for (int a = 0; a < 3; ++a) { P.lo[a] = 1; P.hi[a] = 0; }
             // cascade boxes: the six disjoint slabs around the inner box
