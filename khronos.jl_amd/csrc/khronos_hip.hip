@@ -660,9 +660,15 @@ struct Ctx final : CtxBase {
                 zero_run(h, g.n[a], P.lo[a], P.hi[a]);
                 any_empty |= P.lo[a] > P.hi[a];
             }
-            // KH_INNER=0: no separate inner-box launch; the PML kernel, whose
-            // sigma = 0 branch is the inner update, covers the whole grid
-            P.inner = (mode == KH_MODE_FAST) && !any_empty && env_int("KH_INNER", 1) != 0;
+            // FAST mode runs the PML kernel over the whole grid (its sigma = 0
+            // branch is the inner update): one launch, full 128-cell rows for
+            // the x slabs, measured 1.3x faster per step than an inner-box
+            // launch plus six slab launches.  The inner kernel serves grids
+            // without PML (inner box = whole grid), or KH_INNER=1 forces the
+            // split.
+            bool whole = !any_empty;
+            for (int a = 0; a < 3; ++a) whole = whole && P.lo[a] == 1 && P.hi[a] == g.n[a];
+            P.inner = (mode == KH_MODE_FAST) && !any_empty && (whole || env_int("KH_INNER", 0) != 0);
             if (!P.inner)
                 for (int a = 0; a < 3; ++a) { P.lo[a] = 1; P.hi[a] = 0; }
             // cascade boxes: the six disjoint slabs around the inner box

@@ -183,13 +183,15 @@ def wirtinger_derivative(func: Callable, args: List[np.ndarray], idx: int) -> np
     (src/Adjoint/OptimizationProblem.jl:336-374), through torch autograd."""
     import torch
     x = np.asarray(args[idx])
-    xr = torch.tensor(x.real, dtype=torch.float64, requires_grad=True)
-    xi = torch.tensor(x.imag, dtype=torch.float64, requires_grad=True)
-    targs = [torch.tensor(np.asarray(a)) for a in args]
-    targs[idx] = torch.complex(xr, xi)
-    out = func(*targs)
-    out = out if torch.is_tensor(out) else torch.tensor(out)
-    gr, gi = torch.autograd.grad(out.real.sum(), [xr, xi], allow_unused=True)
+    # may run inside an autograd backward, where grad mode is off
+    with torch.enable_grad():
+        xr = torch.tensor(x.real, dtype=torch.float64, requires_grad=True)
+        xi = torch.tensor(x.imag, dtype=torch.float64, requires_grad=True)
+        targs = [torch.tensor(np.asarray(a)) for a in args]
+        targs[idx] = torch.complex(xr, xi)
+        out = func(*targs)
+        out = out if torch.is_tensor(out) else torch.tensor(out)
+        gr, gi = torch.autograd.grad(out.real.sum(), [xr, xi], allow_unused=True)
     gr = np.zeros(x.shape) if gr is None else gr.numpy()
     gi = np.zeros(x.shape) if gi is None else gi.numpy()
     return (gr + 1j * gi).ravel(order="C") / 2

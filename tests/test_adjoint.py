@@ -101,3 +101,12 @@ def test_adjoint_gradient_matches_central_differences():
     errs, pairs, g = directional_errors(opt, rho)
     assert np.median(errs) < 0.03 and max(errs) < 0.05, (errs, pairs)
     assert np.any(g != 0)
+
+
+def test_torch_reverse_rule_on_oracle():
+    """simulate_torch: backward runs the adjoint and contracts with the cotangent."""
+    opt = slab_problem(freqs=(0.45, 0.55))
+    rho = torch.tensor(0.3 + 0.4 * np.random.default_rng(1).random(24), requires_grad=True)
+    f = A.simulate_torch(opt, rho)
+    (2.0 * f.sum()).backward()
+    assert np.allclose(rho.grad.numpy(), 2.0 * opt.gradient.sum(axis=1), rtol=1e-12, atol=0)
