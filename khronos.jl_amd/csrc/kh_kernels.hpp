@@ -92,6 +92,21 @@ struct SrcL {
     int64_t nwave;
 };
 
+// A current source as the FAST correction applies it (see source_fix_kernel).
+template <typename T>
+struct FixSrc {
+    T *A;
+    T *W;
+    const T *minv;        // per-voxel or null (minv_s)
+    const T *sig_own, *sig_prev;
+    int own, prev;
+    const T *amp;
+    int s[3], d[3];
+    const double *wave;
+    int64_t nwave;
+    int first;            // first block
+};
+
 struct Slab {
     int lo[3];      // first grid index
     int n[3];       // extent
@@ -117,7 +132,8 @@ struct CascadeArgs {
     int per_xcd;
     int nsrc;
     SrcL<T> src[MAX_SRC];
-    const int64_t *clk;   // device timestep
+    const int64_t *clk;   // device timestep of the first step of the launch block
+    int koff;             // this step's offset in the block (graph position)
     double cutoff;        // last source time
     // total polarization subtracted in the material stage (E, EXACT mode):
     // dense over the material box ulo..ulo+un-1, or null
@@ -156,6 +172,7 @@ struct MatArgs {
     // previous one, replaced by the new total; EXACT: buffer 0 only)
     T *fpd[2][2][3];      // [buffer][part][component]; null without poles
     const int64_t *clk;
+    int koff;
     int lo[3], n[3];
     int64_t sy, sz;
     int npole;
@@ -248,7 +265,7 @@ __device__ __forceinline__ T source_value(const CascadeArgs<T> &p, int comp, int
     // (src/Sources/Sources.jl:346-357, accumulated into a zeroed S array).
     T s = T(0);
     inside = false;
-    const int64_t ts = *p.clk;
+    const int64_t ts = *p.clk + p.koff;
     const bool on = sources_on(ts, p.dt, p.cutoff);
     for (int q = 0; q < p.nsrc; ++q) {
         const SrcL<T> &o = p.src[q];
@@ -730,22 +747,10 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
 //   dW = m (S_n - (1 - s_p) / (1 + s_p) S_{n-1});
 //   s_o == 0: A += dW;   s_o != 0: W += dW, A += (1 + s_o) dW.
 // In the inner box (all sigma zero) this is A += m (S_n - S_{n-1}).
-// S = Re(f * amp).  All sources of a half step in one launch: consecutive
-// block ranges per source, in registration order (their boxes may overlap
-// only across components, which are different arrays).
-template <typename T>
-struct FixSrc {
-    T *A;
-    T *W;
-    const T *minv;        // per-voxel or null (minv_s)
-    const T *sig_own, *sig_prev;
-    int own, prev;
-    const T *amp;
-    int s[3], d[3];
-    const double *wave;
-    int64_t nwave;
-    int first;            // first block
-};
+// S = Re(f * amp).  Sources go in one launch as consecutive block ranges,
+// in registration order, as long as no two of them in the launch write the
+// same cell of one component; the host splits sources that do into
+// successive launches, so every overlapped cell takes them in turn.
 
 template <typename T>
 struct FixArgs {
@@ -755,6 +760,7 @@ struct FixArgs {
     T dt;
     double cutoff;
     const int64_t *clk;
+    int koff;
     int n0, n1, n2;
     int64_t sy, sz;
 };
@@ -765,7 +771,7 @@ __global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
     const int b = blockIdx.x;
     while (q + 1 < p.nsrc && b >= p.src[q + 1].first) ++q;
     const FixSrc<T> &o = p.src[q];
-    const int64_t ts = *p.clk;
+    const int64_t ts = *p.clk + p.koff;
     const bool now = sources_on(ts, p.dt, p.cutoff);
     const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
     if (!now && !before) return;
@@ -796,9 +802,9 @@ __global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
     }
 }
 
-// End of a step: advance the device timestep (one lane writes).
-__global__ void tick_kernel(int64_t *clk) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = clk[0] + 1;
+// End of a block of steps: advance the device timestep (one lane writes).
+__global__ void tick_kernel(int64_t *clk, int64_t n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = clk[0] + n;
 }
 
 // ----------------------------------------------------------- materials
@@ -824,7 +830,7 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
     const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
     const int g[3] = {p.lo[0] + ux, p.lo[1] + uy, p.lo[2] + uz};
     const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
-    const int bcur = FAST ? (int)(*p.clk & 1) : 0;
+    const int bcur = FAST ? (int)((*p.clk + p.koff) & 1) : 0;
     const int bnext = FAST ? 1 - bcur : 0;
     T e[NP][3], so[3];
     for (int q = 0; q < NP; ++q)
@@ -1004,6 +1010,7 @@ struct DftArgs {
     int nmon;
     DftMon<T> mon[MAX_DFT];
     const int64_t *clk;
+    int koff;
     T dt;
     int electric;           // E monitors sample at t + dt/2
     int n0, n1, n2;
@@ -1019,7 +1026,7 @@ __global__ __launch_bounds__(256) void dft_kernel(DftArgs<T> p) {
     const int b = blockIdx.x;
     while (q + 1 < p.nmon && b >= p.mon[q + 1].first) ++q;
     const DftMon<T> &m = p.mon[q];
-    const int64_t ts = *p.clk;
+    const int64_t ts = *p.clk + p.koff;
     if (m.dec > 1 && (ts % m.dec) != 0) return;
     // time_fac = complex_backend_number(im * 2 * pi * time) (src/Monitors/Monitors.jl:287)
     double t = step_time(ts, p.dt);

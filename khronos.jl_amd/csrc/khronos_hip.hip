@@ -558,6 +558,7 @@ struct Ctx final : CtxBase {
         q.npole = fpd[0][0][0] ? (int)poles.size() : 0;
         q.poles = poles_dev;
         q.clk = clk;
+        q.koff = koff;
         EvPair ev{};
         if (timing) {
             ev = take_event(5);
@@ -755,45 +756,75 @@ struct Ctx final : CtxBase {
     void launch_half(int e) {
         for (int part = 0; part < nparts; ++part) launch_half_part(e, part);
         const bool electric = e == 1;
-        // FAST mode sources: added after the update, for every source cell
-        // (real part only: S += Re(f A), src/Sources/Sources.jl:346-357).
-        // Launched while sources are or were on the previous step; the switch
-        // is permanent, and the kernel gates itself by the device timestep.
+        // FAST mode sources (real part only: S += Re(f A),
+        // src/Sources/Sources.jl:346-357) while sources are or were on the
+        // previous step; the switch is permanent and the kernel gates itself
+        // by the device timestep.  Sources share a launch unless two of them
+        // write the same cell of one component: then they go to successive
+        // launches, in registration order.
         if (mode == KH_MODE_FAST && (active || last_active)) {
-            kh::FixArgs<T> q{};
-            const Part &f = fp[0];
-            int blocks = 0;
-            for (auto &o : srcs) {
-                if ((o.comp < 3) != electric) continue;
-                kh::FixSrc<T> &x = q.src[q.nsrc++];
-                const int a = o.comp % 3, prv = (a + 2) % 3;
-                x.A = electric ? f.E[a] : f.H[a];
-                x.W = electric ? f.WD[a] : f.WB[a];
-                x.minv = electric ? epsi[a] : mui[a];
-                x.sig_own = electric ? sigD[a] : sigB[a];
-                x.sig_prev = electric ? sigD[prv] : sigB[prv];
-                x.own = a;
-                x.prev = prv;
-                x.amp = o.amp;
-                for (int k = 0; k < 3; ++k) { x.s[k] = o.s[k]; x.d[k] = o.d[k]; }
-                x.wave = o.wave;
-                x.nwave = o.nwave;
-                x.first = blocks;
-                blocks += (int)(((int64_t)o.d[0] * o.d[1] * o.d[2] + 255) / 256);
-            }
-            if (q.nsrc) {
+            std::vector<kh::FixSrc<T>> list = fix_sources(electric);
+            size_t q0 = 0;
+            while (q0 < list.size()) {
+                kh::FixArgs<T> q{};
+                int blocks = 0;
+                size_t q1 = q0;
+                for (; q1 < list.size() && q1 - q0 < (size_t)kh::MAX_SRC; ++q1) {
+                    bool clash = false;
+                    for (size_t r = q0; r < q1; ++r) clash = clash || boxes_meet(list[r], list[q1]);
+                    if (clash) break;
+                    q.src[q.nsrc] = list[q1];
+                    q.src[q.nsrc].first = blocks;
+                    const auto &x = list[q1];
+                    blocks += (int)(((int64_t)x.d[0] * x.d[1] * x.d[2] + 255) / 256);
+                    ++q.nsrc;
+                }
+                q0 = q1;
                 q.minv_s = electric ? eps_s : mu_s;
                 q.dt = dt;
                 q.cutoff = cutoff;
                 q.clk = clk;
+                q.koff = koff;
                 q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
                 q.sy = g.sy();
                 q.sz = g.sz();
+                if (blocks == 0) continue;
                 hipLaunchKernelGGL((kh::source_fix_kernel<T>), dim3(blocks), dim3(256), 0, st, q);
                 HIPCHK(hipGetLastError());
             }
         }
         launch_wrap(e);
+    }
+
+    std::vector<kh::FixSrc<T>> fix_sources(bool electric) const {
+        std::vector<kh::FixSrc<T>> out;
+        const Part &f = fp[0];
+        for (auto &o : srcs) {
+            if ((o.comp < 3) != electric) continue;
+            kh::FixSrc<T> x{};
+            const int a = o.comp % 3, prv = (a + 2) % 3;
+            x.A = electric ? f.E[a] : f.H[a];
+            x.W = electric ? f.WD[a] : f.WB[a];
+            x.minv = electric ? epsi[a] : mui[a];
+            x.sig_own = electric ? sigD[a] : sigB[a];
+            x.sig_prev = electric ? sigD[prv] : sigB[prv];
+            x.own = a;
+            x.prev = prv;
+            x.amp = o.amp;
+            for (int k = 0; k < 3; ++k) { x.s[k] = o.s[k]; x.d[k] = o.d[k]; }
+            x.wave = o.wave;
+            x.nwave = o.nwave;
+            out.push_back(x);
+        }
+        return out;
+    }
+
+    // two sources that write a common cell of one component
+    static bool boxes_meet(const kh::FixSrc<T> &a, const kh::FixSrc<T> &b) {
+        if (a.A != b.A) return false;
+        for (int k = 0; k < 3; ++k)
+            if (a.s[k] + a.d[k] <= b.s[k] || b.s[k] + b.d[k] <= a.s[k]) return false;
+        return true;
     }
 
     // periodic / Bloch ghost fill of the group just updated, axis by axis
@@ -871,6 +902,7 @@ struct Ctx final : CtxBase {
                 l.nwave = o.nwave;
             }
             p.clk = clk;
+            p.koff = koff;
             p.cutoff = cutoff;
             const dim3 cgrid(p.per_xcd * kh::NXCD), cblk(kh::CBLK);
             if (electric) hipLaunchKernelGGL((kh::cascade_e_kernel<T>), cgrid, cblk, 0, st, p);
@@ -924,9 +956,14 @@ struct Ctx final : CtxBase {
     // All DFT monitors of a half step, in launches of up to MAX_DFT monitors;
     // each gates itself by its decimation against the device timestep.
     void launch_dfts(bool electric) {
+        // the host knows the timestep of every launched or captured step
+        // (graphs start on a multiple of every decimation): only due
+        // monitors are launched; the kernel gates itself as well
+        const int64_t step_ts = ts + koff;
         std::vector<const Dft *> list;
         for (auto &m : dfts)
-            if ((m.comp % 6 < 3) == electric) list.push_back(&m);
+            if ((m.comp % 6 < 3) == electric && (dft_gate_device || m.dec <= 1 || step_ts % m.dec == 0))
+                list.push_back(&m);
         for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
             kh::DftArgs<T> q{};
             int blocks = 0, nfmax = 1;
@@ -961,6 +998,7 @@ struct Ctx final : CtxBase {
                 nfmax = std::max(nfmax, m.nf);
             }
             q.clk = clk;
+            q.koff = koff;
             q.dt = dt;
             q.electric = electric ? 1 : 0;
             q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
@@ -984,13 +1022,22 @@ struct Ctx final : CtxBase {
 
     // One time step's launches (src/Kernels/Kernels.jl:64-87 order); every
     // time-dependent quantity comes from the device timestep.
-    void launch_step() {
+    int koff = 0;                 // offset of the step being launched in its block
+    bool dft_gate_device = false;  // decimations too long for a graph period: launch every step
+    int64_t dft_period = 1;        // lcm of the DFT decimations
+
+    void launch_step(int k) {
+        koff = k;
         launch_half(0);
         launch_dfts(false);
         launch_half(1);
         if (mat) launch_material();
         launch_dfts(true);
-        hipLaunchKernelGGL(kh::tick_kernel, dim3(1), dim3(64), 0, st, clk);
+        koff = 0;
+    }
+
+    void launch_tick(int64_t n) {
+        hipLaunchKernelGGL(kh::tick_kernel, dim3(1), dim3(64), 0, st, clk, n);
         HIPCHK(hipGetLastError());
     }
 
@@ -1028,7 +1075,8 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         bool ok = true;
         try {
-            for (int i = 0; i < graph_steps; ++i) launch_step();
+            for (int i = 0; i < graph_len; ++i) launch_step(i);
+            launch_tick(graph_len);
         } catch (const KhError &) {
             ok = false;
         }
@@ -1049,30 +1097,52 @@ struct Ctx final : CtxBase {
         return true;
     }
 
+    int graph_len = 16;  // steps per captured graph: graph_steps rounded up to the DFT period
+
+    void plan_graph_len() {
+        dft_period = 1;
+        for (auto &m : dfts) {
+            const int64_t d = std::max(1, m.dec);
+            int64_t a = dft_period, b = d;
+            while (b) { const int64_t t = a % b; a = b; b = t; }
+            dft_period = dft_period / a * d;
+            if (dft_period > 256) break;
+        }
+        dft_gate_device = dft_period > 256;
+        if (dft_gate_device) dft_period = 1;
+        graph_len = graph_steps <= 0 ? 0 : (int)(((graph_steps + dft_period - 1) / dft_period) * dft_period);
+    }
+
     void step(int64_t n) override {
         require(n >= 0, "n_steps must be >= 0");
-        if (!planned) make_plans();
+        if (!planned) {
+            make_plans();
+            plan_graph_len();
+        }
         stepped = true;
         int64_t done = 0;
         while (done < n) {
             begin_step();
             const bool src_now = active || last_active;
-            if (graph_steps > 0 && !timing && !graph_failed && n - done >= graph_steps) {
+            // a graph starts on a multiple of the DFT period, so the due
+            // monitors of each captured position are the same on every replay
+            if (graph_len > 0 && !timing && !graph_failed && n - done >= graph_len && ts % dft_period == 0) {
                 if (!graph_exec || graph_src != src_now) {
                     if (!capture_graph()) graph_failed = true;
                 }
                 if (graph_exec) {
                     HIPCHK(hipGraphLaunch(graph_exec, st));
                     end_step();
-                    for (int i = 1; i < graph_steps; ++i) {
+                    for (int i = 1; i < graph_len; ++i) {
                         begin_step();
                         end_step();
                     }
-                    done += graph_steps;
+                    done += graph_len;
                     continue;
                 }
             }
-            launch_step();
+            launch_step(0);
+            launch_tick(1);
             end_step();
             ++done;
         }
@@ -1083,6 +1153,7 @@ struct Ctx final : CtxBase {
         graph_steps = steps;
         graph_failed = false;
         drop_graph();
+        if (planned) plan_graph_len();
     }
 
     void sync() override { HIPCHK(hipStreamSynchronize(st)); }
