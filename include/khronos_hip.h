@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 2
+#define KH_ABI_VERSION 3
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -94,6 +94,13 @@ typedef struct kh_sim_desc {
     int32_t periodic[3];
     double bloch_fwd[3][2];
     double bloch_rev[3][2];
+    /* Material conductivity sigma_D of Ex,Ey,Ez (0..2) and sigma_B of
+     * Hx,Hy,Hz (3..5) per voxel, reference geometry layout, absorber profiles
+     * already added (src/Geometry.jl:467-476, :117-118, :708-789).  A component
+     * with one runs the C stage of the cascade in every cell, sigma~ =
+     * 0.5 * (dt * sigma) (src/Kernels/Helpers.jl:39-70, :278-279).  Needs the
+     * six PML profiles above (zeros where there is no PML).  NULL = none. */
+    const void *sigma_mat[6];
 } kh_sim_desc;
 
 /* Message describing the last failure on this thread ("" if none). */

@@ -132,6 +132,10 @@ struct CascadeArgs {
     int per_xcd;
     int nsrc;
     SrcL<T> src[MAX_SRC];
+    // material conductivity (per voxel, padded grid) and the C stage
+    // arrays; null for a component without conductivity
+    const T *msig[3];
+    T *C[3];
     const int64_t *clk;   // device timestep of the first step of the launch block
     int koff;             // this step's offset in the block (graph position)
     double cutoff;        // last source time
@@ -289,10 +293,27 @@ template <typename T>
 __device__ __forceinline__ void cascade_component(const CascadeArgs<T> &p, int a, size_t idx, T K, T so,
                                                   T sn, T sp, int gx, int gy, int gz) {
     const T m = p.minv[a] ? ld1(p.minv[a], idx) : p.minv_s;
-    // curl stage: C absent, U present (src/Kernels/Helpers.jl:113-139)
     const T tq = ld1((const T *)p.Tq[a], idx);
     T tn;
-    if (sn == T(0)) {
+    if (p.C[a]) {
+        // curl stage with C and U allocated and a conductivity array
+        // (src/Kernels/Helpers.jl:39-70): sigma~ = 0.5 (dt sigma_D)
+        // (:278-279); sigma~ = 0 skips C, and U is never bypassed here
+        const T sd = T(0.5) * (p.dt * ld1(p.msig[a], idx));
+        const T uo = ld1((const T *)p.U[a], idx);
+        T un;
+        if (sd == T(0)) {
+            un = ((T(1) - sn) * uo + K) / (T(1) + sn);
+        } else {
+            const T co = ld1((const T *)p.C[a], idx);
+            const T cn = ((T(1) - sd) * co + K) / (T(1) + sd);
+            st1(p.C[a], idx, cn);
+            un = (((T(1) - sn) * uo + cn) - co) / (T(1) + sn);
+        }
+        st1(p.U[a], idx, un);
+        tn = (((T(1) - sp) * tq + un) - uo) / (T(1) + sp);
+    } else if (sn == T(0)) {
+        // curl stage: C absent, U present (src/Kernels/Helpers.jl:113-139)
         tn = ((T(1) - sp) * tq + K) / (T(1) + sp);
     } else {
         const T uo = ld1((const T *)p.U[a], idx);
@@ -580,6 +601,12 @@ struct PmlArgs {
     PSlab slab[MAX_SLAB];
     int nblock;
     int per_xcd;
+    // conductivity (COND kernels): per-voxel sigma on the padded grid and the
+    // C stage arrays, touched only inside the box clo..chi where some sigma
+    // of the group is nonzero
+    const T *msig[3];
+    T *C[3];
+    int clo[3], chi[3];
 };
 
 template <typename T>
@@ -606,7 +633,9 @@ __device__ __forceinline__ bool pml_lane(const PmlArgs<T> &p, int &x, int &y, in
 template <typename T>
 struct PmlComp {
     typename V2<T>::t m, ao, uo, wo;
+    typename V2<T>::t co, sd;  // C stage: old C, sigma~ = 0.5 (dt sigma)
     bool nu, nw;  // U stage (next-axis sigma) / W stage (own-axis sigma) present
+    bool nc;      // C stage (lane inside the conductivity box)
 };
 
 template <typename T, bool PERVOX>
@@ -635,14 +664,27 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t id
                                            T K1, T so0, T so1, T sn0, T sn1, T sp0, T sp1, bool v0, bool v1) {
     using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
+    if (c.nc) {
+        // C stage (src/Kernels/Helpers.jl:58-63): dC/dt + sigma_D C = K
+        V cn = c.co;
+        if (c.sd.x != T(0) && v0) {
+            cn.x = ((T(1) - c.sd.x) * c.co.x + K0) / (T(1) + c.sd.x);
+            in0 = cn.x - c.co.x;
+        }
+        if (c.sd.y != T(0) && v1) {
+            cn.y = ((T(1) - c.sd.y) * c.co.y + K1) / (T(1) + c.sd.y);
+            in1 = cn.y - c.co.y;
+        }
+        sto<T, true>(p.C[a], idx, cn);
+    }
     if (c.nu) {
         V un = c.uo;
         if (sn0 != T(0) && v0) {
-            un.x = ((T(1) - sn0) * c.uo.x + K0) / (T(1) + sn0);
+            un.x = ((T(1) - sn0) * c.uo.x + in0) / (T(1) + sn0);
             in0 = un.x - c.uo.x;
         }
         if (sn1 != T(0) && v1) {
-            un.y = ((T(1) - sn1) * c.uo.y + K1) / (T(1) + sn1);
+            un.y = ((T(1) - sn1) * c.uo.y + in1) / (T(1) + sn1);
             in1 = un.y - c.uo.y;
         }
         sto<T, true>(p.U[a], idx, un);
@@ -676,9 +718,9 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t id
 
 // Loads of the updated field's state for the three components:
 // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y).
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool COND>
 __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T sx0, T sx1, T sgy, T sgz,
-                                             PmlComp<T> c[3]) {
+                                             int x, int y, int z, PmlComp<T> c[3]) {
     pml_load_a<T, PERVOX>(p, 0, idx, c[0]);
     pml_load_a<T, PERVOX>(p, 1, idx, c[1]);
     pml_load_a<T, PERVOX>(p, 2, idx, c[2]);
@@ -686,9 +728,20 @@ __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T 
     pml_load_uw(p, 0, idx, py, px, c[0]);
     pml_load_uw(p, 1, idx, pz, py, c[1]);
     pml_load_uw(p, 2, idx, px, pz, c[2]);
+    const bool inbox = COND && x + 1 >= p.clo[0] && x <= p.chi[0] && y >= p.clo[1] && y <= p.chi[1] &&
+                       z >= p.clo[2] && z <= p.chi[2];
+    for (int a = 0; a < 3; ++a) {
+        c[a].nc = inbox && p.C[a] != nullptr;
+        if (c[a].nc) {
+            c[a].co = ldo<T, true>((const T *)p.C[a], idx);
+            const auto sg = ldo<T, true>(p.msig[a], idx);
+            c[a].sd.x = T(0.5) * (p.dt * sg.x);
+            c[a].sd.y = T(0.5) * (p.dt * sg.y);
+        }
+    }
 }
 
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool COND>
 __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     int x, y, z;
     bool v0, v1;
@@ -702,7 +755,7 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const auto ez_y1 = ld2(Ez, idx + sy), ex_y1 = ld2(Ex, idx + sy);
     const T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
     PmlComp<T> c[3];
-    pml_load_all<T, PERVOX>(p, idx, sx0, sx1, sgy, sgz, c);
+    pml_load_all<T, PERVOX, COND>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (ey1.x - ey.x) - iy * (ez_y1.x - ez.x));
     const T Kx1 = dt * (iz * (ey1.y - ey.y) - iy * (ez_y1.y - ez.y));
@@ -715,7 +768,7 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
 }
 
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool COND>
 __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     int x, y, z;
     bool v0, v1;
@@ -729,7 +782,7 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
     const T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
     PmlComp<T> c[3];
-    pml_load_all<T, PERVOX>(p, idx, sx0, sx1, sgy, sgz, c);
+    pml_load_all<T, PERVOX, COND>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (hym.x - hy.x) - iy * (hz_ym.x - hz.x));
     const T Kx1 = dt * (iz * (hym.y - hy.y) - iy * (hz_ym.y - hz.y));

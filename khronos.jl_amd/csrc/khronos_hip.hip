@@ -143,6 +143,7 @@ struct Ctx final : CtxBase {
     // readout combine them.
     struct Part {
         T *E[3]{}, *H[3]{}, *D[3]{}, *B[3]{}, *UD[3]{}, *UB[3]{}, *WD[3]{}, *WB[3]{};
+        T *CD[3]{}, *CB[3]{};  // C stage, components with a conductivity array
     };
     Part fp[2];
     int nparts = 1;
@@ -150,6 +151,11 @@ struct Ctx final : CtxBase {
     double ph_fwd[3][2] = {}, ph_rev[3][2] = {};
     T *epsi[3]{}, *mui[3]{};
     T *sigB[3]{}, *sigD[3]{};
+    // material conductivity sigma_D (Ex,Ey,Ez) / sigma_B (Hx,Hy,Hz) on the
+    // padded grid, 0 outside 1..N; cbox[e] = cells where some sigma of group e
+    // (0: H, 1: E) is nonzero (lo > hi: none)
+    T *msig[6]{};
+    int cbox[2][6] = {{1, 1, 1, 0, 0, 0}, {1, 1, 1, 0, 0, 0}};
     T eps_s, mu_s;
     int64_t ts = 0;
     bool active = true;
@@ -277,6 +283,15 @@ struct Ctx final : CtxBase {
             upload_material(dv, c, (const T *)src);
             (c < 3 ? epsi : mui)[c % 3] = dv;
         }
+        // conductivity: per-voxel sigma and the C stage arrays
+        // (src/Fields.jl needs_C: conductivity and PML arrays present)
+        for (int c = 0; c < 6; ++c) {
+            if (!d.sigma_mat[c]) continue;
+            msig[c] = field_alloc();
+            upload_material(msig[c], c, (const T *)d.sigma_mat[c], T(0));
+            grow_box(cbox[c < 3 ? 1 : 0], c, (const T *)d.sigma_mat[c]);
+            for (int p = 0; p < nparts; ++p) (c < 3 ? fp[p].CD : fp[p].CB)[c % 3] = field_alloc();
+        }
         scratch = (double *)dalloc(NPART * 64 * sizeof(double));
         clk = (int64_t *)dalloc(256);
         graph_steps = env_int("KH_GRAPH_STEPS", 16);
@@ -284,17 +299,36 @@ struct Ctx final : CtxBase {
     }
 
     // reference geometry layout (component voxel count, no ghosts) -> the
-    // padded grid, cells 1..N; 1 elsewhere
-    void upload_material(T *dv, int c, const T *ps) {
+    // padded grid, cells 1..N; `fill` elsewhere
+    void upload_material(T *dv, int c, const T *ps, T fill = T(1)) {
         int cnt[3];
         comp_count(g.n, c, cnt);
-        std::vector<T> h((size_t)g.total - g.front, T(1));
+        std::vector<T> h((size_t)g.total - g.front, fill);
         for (int k = 1; k <= g.n[2]; ++k)
             for (int j = 1; j <= g.n[1]; ++j)
                 for (int i = 1; i <= g.n[0]; ++i)
                     h[g.lin(i, j, k)] = ps[(size_t)(i - 1) + (size_t)cnt[0] * ((size_t)(j - 1) + (size_t)cnt[1] * (k - 1))];
         HIPCHK(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
+    }
+
+    // widen box b (lo[3], hi[3]) to the cells 1..N where ps (geometry layout)
+    // is nonzero
+    void grow_box(int *b, int c, const T *ps) const {
+        int cnt[3];
+        comp_count(g.n, c, cnt);
+        for (int k = 1; k <= g.n[2]; ++k)
+            for (int j = 1; j <= g.n[1]; ++j)
+                for (int i = 1; i <= g.n[0]; ++i) {
+                    if (ps[(size_t)(i - 1) + (size_t)cnt[0] * ((size_t)(j - 1) + (size_t)cnt[1] * (k - 1))] == T(0))
+                        continue;
+                    const int q[3] = {i, j, k};
+                    for (int a = 0; a < 3; ++a) {
+                        if (b[3 + a] < b[a]) { b[a] = q[a]; b[3 + a] = q[a]; }
+                        b[a] = std::min(b[a], q[a]);
+                        b[3 + a] = std::max(b[3 + a], q[a]);
+                    }
+                }
     }
 
     void set_eps_inv(int comp, const void *a) override {
@@ -669,7 +703,9 @@ struct Ctx final : CtxBase {
             // split.
             bool whole = !any_empty;
             for (int a = 0; a < 3; ++a) whole = whole && P.lo[a] == 1 && P.hi[a] == g.n[a];
-            P.inner = (mode == KH_MODE_FAST) && !any_empty && (whole || env_int("KH_INNER", 0) != 0);
+            // a group with conductivity needs the C stage of the PML kernel
+            const bool cond = (e ? msig[0] || msig[1] || msig[2] : msig[3] || msig[4] || msig[5]);
+            P.inner = (mode == KH_MODE_FAST) && !any_empty && !cond && (whole || env_int("KH_INNER", 0) != 0);
             if (!P.inner)
                 for (int a = 0; a < 3; ++a) { P.lo[a] = 1; P.hi[a] = 0; }
             // cascade boxes: the six disjoint slabs around the inner box
@@ -734,6 +770,8 @@ struct Ctx final : CtxBase {
             p.sig[a] = e ? sigD[a] : sigB[a];
             p.minv[a] = e ? epsi[a] : mui[a];
             p.inv[a] = T(1) / dx[a];
+            p.msig[a] = msig[e ? a : 3 + a];
+            p.C[a] = e ? f.CD[a] : f.CB[a];
         }
         p.minv_s = e ? eps_s : mu_s;
         p.dt = dt;
@@ -927,7 +965,12 @@ struct Ctx final : CtxBase {
                 q.sig[a] = electric ? sigD[a] : sigB[a];
                 q.minv[a] = electric ? epsi[a] : mui[a];
                 q.inv[a] = T(1) / dx[a];
+                q.msig[a] = msig[electric ? a : 3 + a];
+                q.C[a] = electric ? f.CD[a] : f.CB[a];
+                q.clo[a] = cbox[e][a];
+                q.chi[a] = cbox[e][3 + a];
             }
+            const bool cond = q.C[0] || q.C[1] || q.C[2];
             q.minv_s = electric ? eps_s : mu_s;
             q.dt = dt;
             q.sy = g.sy();
@@ -939,11 +982,21 @@ struct Ctx final : CtxBase {
             const bool pervox = q.minv[0] != nullptr;
             const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
             if (electric) {
-                if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true>), pgrid, pblk, 0, st, q);
-                else hipLaunchKernelGGL((kh::pml_e_kernel<T, false>), pgrid, pblk, 0, st, q);
+                if (cond) {
+                    if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, true>), pgrid, pblk, 0, st, q);
+                } else {
+                    if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, false>), pgrid, pblk, 0, st, q);
+                }
             } else {
-                if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true>), pgrid, pblk, 0, st, q);
-                else hipLaunchKernelGGL((kh::pml_h_kernel<T, false>), pgrid, pblk, 0, st, q);
+                if (cond) {
+                    if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, true>), pgrid, pblk, 0, st, q);
+                } else {
+                    if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, false>), pgrid, pblk, 0, st, q);
+                }
             }
             HIPCHK(hipGetLastError());
             if (timing) {
@@ -1367,6 +1420,8 @@ size_t footprint(const kh_sim_desc &d) {
     // E, H, U and W per component (per part with complex fields); D and B only in EXACT mode
     size_t arrays = (d.mode == KH_MODE_EXACT ? 24 : 18) * (d.complex_fields ? 2 : 1);
     for (int a = 0; a < 3; ++a) arrays += (d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr);
+    // conductivity: sigma plus C per part
+    for (int c = 0; c < 6; ++c) arrays += (d.sigma_mat[c] != nullptr) * (1 + (d.complex_fields ? 2 : 1));
     size_t sig = 0;
     for (int a = 0; a < 3; ++a) sig += 2 * ((size_t)g.n[a] + 2) * es;
     return arrays * (size_t)g.total * es + sig + 256 * 64 * sizeof(double);
@@ -1381,6 +1436,12 @@ void validate(const kh_sim_desc *d) {
     int nb = 0;
     for (int a = 0; a < 3; ++a) nb += (d->sigma_b[a] != nullptr) + (d->sigma_d[a] != nullptr);
     require(nb == 0 || nb == 6, "sigma_b/sigma_d: give all six profiles or none");
+    int nc = 0;
+    for (int c = 0; c < 6; ++c) nc += d->sigma_mat[c] != nullptr;
+    // the reference has no curl update for a conductive component without PML
+    // arrays (src/Kernels/Helpers.jl:141-151 and :225-236 are ambiguous there)
+    require(nc == 0 || nb == 6,
+            "material conductivity needs the six PML profiles (zeros where there is no PML)");
 }
 
 }  // namespace

@@ -9,7 +9,7 @@ from .grid import (Ex, Ey, Ez, Hx, Hy, Hz, Center, Field, Volume, GridVolume, gr
 from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, UniformSource,
                       ArraySource, GaussianProfileSource)
 from .monitors import DFTMonitor, FluxMonitor
-from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary
+from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
 from .geometry import Material, Object, Box, Sphere, Cylinder
 from .susceptibility import (LorentzianSusceptibility, DrudeSusceptibility, compute_ade_coefficients,
                              eval_susceptibility)
@@ -21,7 +21,7 @@ __all__ = [
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
     "GaussianProfileSource", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
-    "PMCBoundary", "Material", "Object", "Box", "Sphere",
+    "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere",
     "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",
     "eval_susceptibility", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
     "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",

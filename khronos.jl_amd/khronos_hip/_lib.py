@@ -27,6 +27,7 @@ class KhSimDesc(C.Structure):
         ("source_cutoff_time", C.c_double),
         ("complex_fields", C.c_int32), ("periodic", C.c_int32 * 3),
         ("bloch_fwd", (C.c_double * 2) * 3), ("bloch_rev", (C.c_double * 2) * 3),
+        ("sigma_mat", C.c_void_p * 6),
     ]
 
 

@@ -59,6 +59,67 @@ class PMCBoundary(BoundaryCondition):
     zero_pml = True
 
 
+class Absorber:
+    """Adiabatic absorber on one side of one axis (src/Boundaries.jl:10-21):
+    conductivity sigma_max * (d / L)^sigma_order added to every sigma_D and
+    sigma_B array over the outermost ``num_layers`` cells; ``sigma_max = 0``
+    picks the value for a reflection of about 1e-6."""
+
+    def __init__(self, num_layers: int = 40, sigma_order: int = 3, sigma_max: float = 0.0):
+        self.num_layers = int(num_layers)
+        self.sigma_order = int(sigma_order)
+        self.sigma_max = float(sigma_max)
+
+    def __repr__(self):
+        return f"Absorber(num_layers={self.num_layers}, sigma_order={self.sigma_order}, sigma_max={self.sigma_max})"
+
+
+def has_absorbers(absorbers) -> bool:
+    return absorbers is not None and any(
+        side is not None for ax in absorbers if ax is not None for side in ax)
+
+
+def apply_absorbers(sim, absorbers, arrays) -> None:
+    """Add the absorber profiles to the conductivity arrays in place
+    (src/Geometry.jl:708-789).  ``absorbers`` is per axis: None or
+    ``[left, right]`` with None or an :class:`Absorber` per side; ``arrays``
+    are the six sigma_D / sigma_B arrays (None entries skipped).
+
+    Arithmetic as the reference: L = num_layers * delta in the backend type;
+    the automatic sigma_max = -(p + 1) ln(1e-6) / (2 L) in float64, rounded to
+    the backend type; per layer (1 = the outermost cell)
+    sigma = R(sigma_max * ((num_layers - layer + 1) / num_layers)^p), added to
+    the whole plane of cells at that index of the array's own extent.
+    """
+    R = sim.real
+    if absorbers is None:
+        return
+    for axis, ax in enumerate(absorbers):
+        if ax is None:
+            continue
+        for side, spec in enumerate(ax):
+            if spec is None:
+                continue
+            n, p = spec.num_layers, spec.sigma_order
+            delta = sim.delta_r[axis]
+            L = R(R(n) * delta)
+            if spec.sigma_max > 0:
+                smax = float(spec.sigma_max)
+            else:
+                smax = float(R(-(p + 1) * math.log(1e-6) / (2.0 * float(L))))
+            for arr in arrays:
+                if arr is None:
+                    continue
+                N = arr.shape[axis]
+                for layer in range(1, min(n, N) + 1):
+                    d = (n - layer + 1) / n
+                    val = R(smax * d ** p)
+                    i = (N - layer) if side == 1 else (layer - 1)
+                    sl = [slice(None)] * 3
+                    sl[axis] = i
+                    arr[tuple(sl)] += val
+
+
 def needs_complex_fields(boundary_conditions) -> bool:
     """Any Bloch side (src/Fields.jl:149-159)."""
     if boundary_conditions is None:

@@ -51,6 +51,11 @@ class HipEngine:
                     h = np.asfortranarray(arrs[a], dtype=R)
                     self._keep.append(h)
                     slot[a] = _ptr(h)
+        for c, arr in enumerate(getattr(spec, "cond", None) or [None] * 6):
+            if arr is not None:
+                h = np.asfortranarray(arr, dtype=R)
+                self._keep.append(h)
+                d.sigma_mat[c] = _ptr(h)
         d.eps_inv_scalar = spec.eps_inv_scalar
         d.mu_inv_scalar = spec.mu_inv_scalar
         d.source_cutoff_time = spec.cutoff

@@ -10,7 +10,9 @@ The reference tests membership with the GeometryPrimitives package (not
 vendored); the box / sphere / cylinder membership tests here are the closed
 forms of those shapes.  Subpixel smoothing is not applied here.
 
-Also here: the chi3 array (src/Geometry.jl:610-636) and the dispersive-pole
+Also here: the per-voxel conductivity arrays sigma_D / sigma_B
+(src/Geometry.jl:467-476, painted as :117-118 with every object, zero where an
+object sets none), the chi3 array (src/Geometry.jl:610-636) and the dispersive-pole
 setup of ``init_polarization!`` (src/Geometry.jl:1049-1355): unique poles,
 per-pole oscillator-strength arrays, their zeroing inside the PML, and the
 chi1 correction of the inverse permittivity.
@@ -36,15 +38,31 @@ class Material:
     εx: Optional[float] = None
     εy: Optional[float] = None
     εz: Optional[float] = None
+    σD: Optional[float] = None
+    σDx: Optional[float] = None
+    σDy: Optional[float] = None
+    σDz: Optional[float] = None
     μ: Optional[float] = None
+    μx: Optional[float] = None
+    μy: Optional[float] = None
+    μz: Optional[float] = None
+    σB: Optional[float] = None
+    σBx: Optional[float] = None
+    σBy: Optional[float] = None
+    σBz: Optional[float] = None
     chi3: Optional[float] = None
     susceptibilities: List[LorentzianSusceptibility] = field(default_factory=list)
 
     def perm(self, electric: bool, axis: int) -> Optional[float]:
-        if electric:
-            comp = (self.εx, self.εy, self.εz)[axis]
-            return comp if comp is not None else self.ε
-        return self.μ
+        """get_material_perm_comp, else get_material_perm (src/Geometry.jl:305-312, :64-70)."""
+        comp = (self.εx, self.εy, self.εz)[axis] if electric else (self.μx, self.μy, self.μz)[axis]
+        return comp if comp is not None else (self.ε if electric else self.μ)
+
+    def conductivity(self, electric: bool, axis: int) -> Optional[float]:
+        """get_material_conductivity_comp, else get_material_conductivity
+        (src/Geometry.jl:327-334, :72-80)."""
+        comp = (self.σDx, self.σDy, self.σDz)[axis] if electric else (self.σBx, self.σBy, self.σBz)[axis]
+        return comp if comp is not None else (self.σD if electric else self.σB)
 
 
 class Shape:
@@ -152,6 +170,35 @@ def rasterize(sim, geometry: List[Object]):
                 arr[np.ix_(ix, iy, iz)] = sub
             out[a] = arr
     return out_eps, out_mu
+
+
+def rasterize_conductivity(sim, geometry: List[Object], force: bool = False):
+    """Per-component conductivity arrays ``(sigma_D[3], sigma_B[3])``
+    (src/Geometry.jl:467-476, :233-249, :117-118).
+
+    A component's array exists when some object sets a conductivity for it
+    (``needs_conductivities``, :336-347), or for all six when ``force`` (the
+    reference allocates every array, zero-filled, when absorbers are given,
+    :479-511).  Objects are painted in reverse order with ``get_sigma`` (0 for
+    an object that sets none), so the first object wins where shapes overlap.
+    """
+    R = sim.real
+    out_d, out_b = [None] * 3, [None] * 3
+    for electric, comps, out in ((True, (Ex, Ey, Ez), out_d), (False, (Hx, Hy, Hz), out_b)):
+        for a, C in enumerate(comps):
+            if not (force or any(o.material.conductivity(electric, a) is not None for o in geometry)):
+                continue
+            c = C()
+            cnt = component_voxel_count(sim.Nx, sim.Ny, sim.Nz, c)
+            arr = np.zeros(cnt, dtype=R)
+            gv = full_grid_volume(sim, c)
+            xs, ys, zs = gv_coords(sim, gv)
+            xs, ys, zs = xs[:cnt[0]], ys[:cnt[1]], zs[:cnt[2]]
+            for obj in reversed(geometry):
+                v = obj.material.conductivity(electric, a)
+                _paint(arr, xs, ys, zs, obj.shape, R(0) if v is None else R(v))
+            out[a] = arr
+    return out_d, out_b
 
 
 def _index_range(coords, lo, hi):

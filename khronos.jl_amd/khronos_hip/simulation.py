@@ -23,8 +23,8 @@ from typing import Callable, List, Optional, Sequence
 import numpy as np
 
 from . import grid as G
-from .boundaries import init_boundaries, needs_complex_fields, periodic_wrap
-from .geometry import init_polarization, rasterize, rasterize_chi3
+from .boundaries import apply_absorbers, has_absorbers, init_boundaries, needs_complex_fields, periodic_wrap
+from .geometry import init_polarization, rasterize, rasterize_chi3, rasterize_conductivity
 from .monitors import (DFTMonitor, FluxMonitor, FluxMonitorData, auto_decimate, expand_monitors,
                        init_dft_monitor, flux_from_dft)
 from .sources import assemble_source
@@ -54,6 +54,8 @@ class SimSpec:
     periodic: tuple = (False, False, False)
     bloch_fwd: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(+i k L): lower interior -> upper ghost
     bloch_rev: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(-i k L): upper interior -> lower ghost
+    # conductivity sigma_D of Ex,Ey,Ez then sigma_B of Hx,Hy,Hz (geometry layout, absorbers added)
+    cond: list = field(default_factory=lambda: [None] * 6)
 
 
 class Simulation:
@@ -61,7 +63,7 @@ class Simulation:
 
     def __init__(self, *, cell_size, resolution, sources, cell_center=(0.0, 0.0, 0.0),
                  boundaries=None, boundary_conditions=None, geometry=None, monitors=None,
-                 Courant=0.5, precision="f64", mode="fast", dft_accum="f64", num_chunks=1,
+                 absorbers=None, Courant=0.5, precision="f64", mode="fast", dft_accum="f64", num_chunks=1,
                  engine=None):
         if num_chunks not in (1, None):
             raise NotImplementedError("only the single-domain layout (num_chunks=1) is provided")
@@ -78,6 +80,7 @@ class Simulation:
         self.sources = list(sources)
         self.boundaries = boundaries
         self.boundary_conditions = boundary_conditions
+        self.absorbers = absorbers
         self.geometry = list(geometry) if geometry else []
         self.monitors = list(monitors) if monitors else []
         self.Courant = Courant
@@ -122,6 +125,17 @@ class Simulation:
         eps_inv, mu_inv = rasterize(self, self.geometry)
         chi3 = rasterize_chi3(self, self.geometry)
         poles, eps_inv = init_polarization(self, self.geometry, eps_inv)
+        # conductivity (src/Geometry.jl:467-511, :708-789)
+        absorb = has_absorbers(self.absorbers)
+        sd, sb = rasterize_conductivity(self, self.geometry, force=absorb)
+        cond = list(sd) + list(sb)
+        if absorb:
+            apply_absorbers(self, self.absorbers, cond)
+        if any(c is not None for c in cond) and sig is None:
+            # without PML arrays the reference has no curl update for a
+            # conductive component (src/Kernels/Helpers.jl:141-151 vs :225-236)
+            raise ValueError("conductivity (material sigma or absorbers) needs `boundaries` "
+                             "(PML thicknesses, zeros allowed)")
         wrap = periodic_wrap(self)
         srcs = []
         for s in self.sources:
@@ -136,7 +150,7 @@ class Simulation:
             dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles,
             complex_fields=needs_complex_fields(self.boundary_conditions),
             periodic=tuple(w[0] for w in wrap), bloch_fwd=tuple(w[1] for w in wrap),
-            bloch_rev=tuple(w[2] for w in wrap))
+            bloch_rev=tuple(w[2] for w in wrap), cond=cond)
 
     def prepare(self):
         if self.is_prepared:

@@ -478,6 +478,10 @@ static void half_step(orc_sim *s, int electric) {
                     int hsD = s->msig[c] != NULL;
                     real sD = 0;
                     if (hsD) sD = (real)0.5 * (s->dt * s->msig[c][rix(s->gdim[c], ix, iy, iz)]);
+#ifdef ORC_PLANT_COND_SCALE
+                    /* planted mistake for the test suite: sigma~ without the 1/2 */
+                    if (hsD) sD = s->dt * s->msig[c][rix(s->gdim[c], ix, iy, iz)];
+#endif
                     int hn = sg[n1] != NULL, hp = sg[n2] != NULL;
                     real sn = hn ? sig_at(sg[n1], gi[n1]) : 0;
                     real sp = hp ? sig_at(sg[n2], gi[n2]) : 0;

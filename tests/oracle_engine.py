@@ -96,6 +96,8 @@ class OracleEngine:
             sig_d = [arr(spec.sigma["D"][a]) for a in range(3)]
         minv = [arr(spec.eps_inv[a], "F") for a in range(3)] + [arr(spec.mu_inv[a], "F") for a in range(3)]
         msig = [None] * 6
+        if material_sigma is None and any(c is not None for c in (getattr(spec, "cond", None) or [])):
+            material_sigma = spec.cond
         if material_sigma is not None:
             msig = [arr(m, "F") for m in material_sigma]
         self.sim = self.lib.orc_create(*spec.N, *spec.delta, spec.dt, *sig_b, *sig_d, *minv, *msig,
