@@ -70,7 +70,9 @@ typedef struct kh_sim_desc {
                                   0: in the backend precision, as the reference */
     double dx, dy, dz, dt;     /* backend-precision values (exact in double) */
     /* PML profiles, length 2*n+1 each, backend precision
-     * (src/Boundaries.jl:64-72, :112-163).  NULL = no PML arrays at all. */
+     * (src/Boundaries.jl:64-72, :112-163).  NULL = no PML arrays at all.
+     * 2D simulations (the reference's Nz = 0) pass nz = 1, dz = +inf and NULL
+     * for the two z profiles, which the reference does not create (:130-162). */
     const void *sigma_b[3];
     const void *sigma_d[3];
     /* Per-voxel inverse permittivity / permeability of Ex,Ey,Ez / Hx,Hy,Hz in

@@ -136,6 +136,7 @@ struct CascadeArgs {
     // arrays; null for a component without conductivity
     const T *msig[3];
     T *C[3];
+    int noU[3];           // component has no U stage (its next-axis profile is absent)
     const int64_t *clk;   // device timestep of the first step of the launch block
     int koff;             // this step's offset in the block (graph position)
     double cutoff;        // last source time
@@ -295,7 +296,18 @@ __device__ __forceinline__ void cascade_component(const CascadeArgs<T> &p, int a
     const T m = p.minv[a] ? ld1(p.minv[a], idx) : p.minv_s;
     const T tq = ld1((const T *)p.Tq[a], idx);
     T tn;
-    if (p.C[a]) {
+    if (p.C[a] && p.noU[a]) {
+        // C allocated, no U (2D, next axis z; src/Kernels/Helpers.jl:80-93)
+        const T sd = T(0.5) * (p.dt * ld1(p.msig[a], idx));
+        if (sd == T(0)) {
+            tn = ((T(1) - sp) * tq + K) / (T(1) + sp);
+        } else {
+            const T co = ld1((const T *)p.C[a], idx);
+            const T cn = ((T(1) - sd) * co + K) / (T(1) + sd);
+            st1(p.C[a], idx, cn);
+            tn = (((T(1) - sp) * tq + cn) - co) / (T(1) + sp);
+        }
+    } else if (p.C[a]) {
         // curl stage with C and U allocated and a conductivity array
         // (src/Kernels/Helpers.jl:39-70): sigma~ = 0.5 (dt sigma_D)
         // (:278-279); sigma~ = 0 skips C, and U is never bypassed here

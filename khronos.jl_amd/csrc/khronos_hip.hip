@@ -151,6 +151,7 @@ struct Ctx final : CtxBase {
     double ph_fwd[3][2] = {}, ph_rev[3][2] = {};
     T *epsi[3]{}, *mui[3]{};
     T *sigB[3]{}, *sigD[3]{};
+    bool hassig[2][3] = {};  // [group (0: H, 1: E)][axis]: profile given (the reference's array exists)
     // material conductivity sigma_D (Ex,Ey,Ez) / sigma_B (Hx,Hy,Hz) on the
     // padded grid, 0 outside 1..N; cbox[e] = cells where some sigma of group e
     // (0: H, 1: E) is nonzero (lo > hi: none)
@@ -264,6 +265,7 @@ struct Ctx final : CtxBase {
             int N = g.n[a];
             for (int which = 0; which < 2; ++which) {
                 const void *src = which == 0 ? d.sigma_b[a] : d.sigma_d[a];
+                hassig[which][a] = src != nullptr;
                 std::vector<T> h((size_t)N + 2, T(0));
                 if (src) {
                     const T *ps = (const T *)src;
@@ -772,6 +774,9 @@ struct Ctx final : CtxBase {
             p.inv[a] = T(1) / dx[a];
             p.msig[a] = msig[e ? a : 3 + a];
             p.C[a] = e ? f.CD[a] : f.CB[a];
+            // no next-axis profile: the reference allocates no U for the
+            // component (src/Fields.jl needs_U)
+            p.noU[a] = !hassig[e][(a + 1) % 3];
         }
         p.minv_s = e ? eps_s : mu_s;
         p.dt = dt;
@@ -1435,7 +1440,11 @@ void validate(const kh_sim_desc *d) {
     require(d->dt > 0 && d->dx > 0 && d->dy > 0 && d->dz > 0, "dt and grid spacings must be > 0");
     int nb = 0;
     for (int a = 0; a < 3; ++a) nb += (d->sigma_b[a] != nullptr) + (d->sigma_d[a] != nullptr);
-    require(nb == 0 || nb == 6, "sigma_b/sigma_d: give all six profiles or none");
+    // 2D (nz = 1, dz = inf) has no z profiles (src/Boundaries.jl:130-162)
+    const bool twod = d->nz == 1 && std::isinf(d->dz) && d->sigma_b[2] == nullptr && d->sigma_d[2] == nullptr;
+    require(nb == 0 || nb == 6 || (twod && nb == 4),
+            "sigma_b/sigma_d: give all six profiles, none, or (2D: nz = 1, dz = inf) the four x/y ones");
+    if (nb == 4) nb = 6;
     int nc = 0;
     for (int c = 0; c < 6; ++c) nc += d->sigma_mat[c] != nullptr;
     // the reference has no curl update for a conductive component without PML

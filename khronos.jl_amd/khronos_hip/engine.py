@@ -40,6 +40,8 @@ class HipEngine:
         d.dt = spec.dt
         if spec.sigma is not None:
             for a in range(3):
+                if spec.sigma["B"][a] is None:   # 2D: no z profiles
+                    continue
                 sb = np.ascontiguousarray(spec.sigma["B"][a], dtype=R)
                 sd = np.ascontiguousarray(spec.sigma["D"][a], dtype=R)
                 self._keep += [sb, sd]
@@ -270,7 +272,7 @@ class HipEngine:
             lohi = []
             for a in range(3):
                 n = self.spec.N[a]
-                if self.spec.sigma is None:
+                if self.spec.sigma is None or self.spec.sigma[key][a] is None:
                     lohi.append((1, n))
                     continue
                 z = np.asarray(self.spec.sigma[key][a])[0:2 * n:2] == 0

@@ -227,9 +227,10 @@ def rasterize_chi3(sim, geometry: List[Object]) -> Optional[np.ndarray]:
     R = sim.real
     if not any(o.material.chi3 is not None for o in geometry):
         return None
-    arr = np.zeros((sim.Nx, sim.Ny, sim.Nz), dtype=R)
+    nz = max(1, sim.Nz)
+    arr = np.zeros((sim.Nx, sim.Ny, nz), dtype=R)
     xs, ys, zs = gv_coords(sim, full_grid_volume(sim, Center()))
-    xs, ys, zs = xs[:sim.Nx], ys[:sim.Ny], zs[:sim.Nz]
+    xs, ys, zs = xs[:sim.Nx], ys[:sim.Ny], zs[:nz]
     for obj in reversed(geometry):
         c3 = R(0) if obj.material.chi3 is None else R(obj.material.chi3)
         if c3 != 0:
@@ -283,7 +284,10 @@ def init_polarization(sim, geometry: List[Object], eps_inv: list):
     cnt = component_voxel_count(sim.Nx, sim.Ny, sim.Nz, Ex())
     xs = build_coords(sim.delta_n[0], org[0], gv.start_idx[0], gv.Nx)[:cnt[0]]
     ys = build_coords(sim.delta_n[1], org[1], gv.start_idx[1], gv.Ny)[:cnt[1]]
-    zs = build_coords(sim.delta_n[2], org[2], gv.start_idx[2], gv.Nz)[:cnt[2]]
+    if sim.ndims == 3:
+        zs = build_coords(sim.delta_n[2], org[2], gv.start_idx[2], gv.Nz)[:cnt[2]]
+    else:
+        zs = np.array([0.0])   # 2D: one z sample at 0 (src/Geometry.jl:1153-1162)
     sigmas = []
     for pole in poles:
         arr = np.zeros((len(xs), len(ys), len(zs)), dtype=R)
@@ -328,7 +332,7 @@ def init_polarization(sim, geometry: List[Object], eps_inv: list):
         c = compute_ade_coefficients(pole, sim.dt_n)
         coeffs.append(c)
         chi1 = chi1 + arr * R(chi1_coefficient(c))
-    N = (sim.Nx, sim.Ny, sim.Nz)
+    N = (sim.Nx, sim.Ny, max(1, sim.Nz))
     sl = (slice(0, N[0]), slice(0, N[1]), slice(0, N[2]))
     out = [PoleData(p, c, np.ascontiguousarray(arr[sl])) for p, c, arr in zip(poles, coeffs, sigmas)]
     chunk_chi1 = chi1[sl]

@@ -87,12 +87,15 @@ class Simulation:
         # src/DataStructures.jl:730-741 (grid counts from the user's float64 values)
         self.Nx = int(math.floor(self.cell_size[0] * resolution))
         self.Ny = int(math.floor(self.cell_size[1] * resolution))
-        self.Nz = 0 if (len(self.cell_size) < 3 or self.cell_size[2] == 0.0) else int(math.floor(self.cell_size[2] * resolution))
-        if self.Nz == 0:
-            raise NotImplementedError("2D simulations are not provided by this time step")
-        self.ndims = 3
+        # 2D (Nz = 0): dz = Inf, so dt/dz = 0, and every array keeps one z
+        # cell (src/DataStructures.jl:689-712, src/utils.jl:20-37)
+        if len(self.cell_size) < 3:
+            self.cell_size = self.cell_size + [0.0] * (3 - len(self.cell_size))
+        self.Nz = 0 if self.cell_size[2] == 0.0 else int(math.floor(self.cell_size[2] * resolution))
+        self.ndims = 2 if self.Nz == 0 else 3
         self.cell_size_n = [float(R(v)) for v in self.cell_size]
-        self.delta_r = [R(self.cell_size[0] / self.Nx), R(self.cell_size[1] / self.Ny), R(self.cell_size[2] / self.Nz)]
+        dz = R(math.inf) if self.Nz == 0 else R(self.cell_size[2] / self.Nz)
+        self.delta_r = [R(self.cell_size[0] / self.Nx), R(self.cell_size[1] / self.Ny), dz]
         self.delta_n = [float(v) for v in self.delta_r]
         self.dt_r = R(min(self.delta_n) * Courant)
         self.dt_n = float(self.dt_r)
@@ -144,7 +147,7 @@ class Simulation:
         dft_mons = expand_monitors(self, self.monitors)
         dfts = [init_dft_monitor(self, m) for m in dft_mons]
         return SimSpec(
-            N=(self.Nx, self.Ny, self.Nz), real=self.real, delta=tuple(self.delta_n), dt=self.dt_n,
+            N=(self.Nx, self.Ny, max(1, self.Nz)), real=self.real, delta=tuple(self.delta_n), dt=self.dt_n,
             sigma=sig, eps_inv=eps_inv, mu_inv=mu_inv, eps_inv_scalar=1.0, mu_inv_scalar=1.0,
             sources=srcs, dfts=dfts, cutoff=self.last_source_time(), mode=self.mode,
             dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles,

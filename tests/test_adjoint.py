@@ -42,6 +42,23 @@ def slab_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.0,
     return A.OptimizationProblem(sim, [f], [obj], [dr], list(freqs), until=until)
 
 
+def slab2d_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.0, nd=(6, 4), freqs=(0.5,)):
+    """The slab problem as a true 2D simulation (cell_size z = 0, dz = Inf),
+    the setting of the reference's 2D adjoint tests (test/test_adjoint_gradient.jl:79-84)."""
+    pml, pad, lx, cy = 1.0, 0.5, 1.0, 2.6
+    cx = lx + 2 * pad + 2 * pml
+    geo = [kh.Object(kh.Box([0, 0, 0], [cx + 1, cy + 1, 10.0]), kh.Material(ε=1.0))]
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ez(),
+                            center=[-(lx / 2 + pad), 0.0, 0.0], size=[0.0, cy - 2 * pml, 0.0])]
+    sim = kh.Simulation(cell_size=[cx, cy, 0.0], resolution=10, sources=src, geometry=geo,
+                        boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode, engine=engine)
+    dr = A.DesignRegion(volume=Volume([0, 0, 0], [lx, cy - 2 * pml, 0.0]),
+                        design_parameters=np.full(nd[0] * nd[1], 0.5), grid_size=nd, ε_min=1.0, ε_max=4.0)
+    obj = A.FourierFieldsObjective(volume=Volume([lx / 2 + pad / 2, 0, 0], [0, cy - 2 * pml, 0.0]), component=kh.Ez())
+    f = lambda E: torch.sum(torch.abs(E) ** 2)
+    return A.OptimizationProblem(sim, [f], [obj], [dr], list(freqs), until=until)
+
+
 def directional_errors(opt, rho, n_dir=4, h=1e-2, seed=7):
     f0, g = opt(rho)
     g = g.sum(axis=1)
@@ -110,3 +127,11 @@ def test_torch_reverse_rule_on_oracle():
     f = A.simulate_torch(opt, rho)
     (2.0 * f.sum()).backward()
     assert np.allclose(rho.grad.numpy(), 2.0 * opt.gradient.sum(axis=1), rtol=1e-12, atol=0)
+
+
+def test_2d_gradient_matches_finite_differences():
+    opt = slab2d_problem()
+    assert opt.sim.ndims == 2
+    rho = 0.3 + 0.4 * np.random.default_rng(42).random(24)
+    errs, pairs, _ = directional_errors(opt, rho)
+    assert np.median(errs) < 0.03 and max(errs) < 0.05, (errs, pairs)
