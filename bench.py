@@ -216,8 +216,9 @@ def main(argv=None):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "strip waveguide 384x192x160, 12-cell CPML, per-voxel eps, Ez Gaussian-pulse "
-                        "planar source, 2 flux planes x 50 freqs (f64 DFT accumulators), FP32 fields",
+            "workload": "strip waveguide 384x192x160, 12-cell CPML, anisotropically smoothed diagonal "
+                        "per-voxel eps, Ez Gaussian-pulse planar source with Gaussian transverse profile, "
+                        "2 flux planes x 50 freqs (f64 DFT accumulators), FP32 fields",
             "cells": cells,
             "mode": args.mode,
             "graph_steps": args.graph_steps,

@@ -11,6 +11,7 @@ from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, U
 from .monitors import DFTMonitor, FluxMonitor
 from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
 from .geometry import Material, Object, Box, Sphere, Cylinder
+from .smoothing import SubpixelSmoothing, NoSmoothing, VolumeAveraging, AnisotropicSmoothing
 from .susceptibility import (LorentzianSusceptibility, DrudeSusceptibility, compute_ade_coefficients,
                              eval_susceptibility)
 from .simulation import (Simulation, SimSpec, run, run_until, stop_when_dft_decayed, run_benchmark,
@@ -21,7 +22,8 @@ __all__ = [
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
     "GaussianProfileSource", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
-    "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere",
+    "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere", "SubpixelSmoothing", "NoSmoothing",
+    "VolumeAveraging", "AnisotropicSmoothing",
     "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",
     "eval_susceptibility", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
     "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",

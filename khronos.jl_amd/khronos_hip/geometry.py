@@ -8,7 +8,8 @@ sample point lies inside it (boundaries inclusive).
 
 The reference tests membership with the GeometryPrimitives package (not
 vendored); the box / sphere / cylinder membership tests here are the closed
-forms of those shapes.  Subpixel smoothing is not applied here.
+forms of those shapes.  Subpixel smoothing is applied afterwards
+(``smoothing.apply_subpixel_smoothing``).
 
 Also here: the per-voxel conductivity arrays sigma_D / sigma_B
 (src/Geometry.jl:467-476, painted as :117-118 with every object, zero where an

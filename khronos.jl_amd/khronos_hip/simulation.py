@@ -25,6 +25,7 @@ import numpy as np
 from . import grid as G
 from .boundaries import apply_absorbers, has_absorbers, init_boundaries, needs_complex_fields, periodic_wrap
 from .geometry import init_polarization, rasterize, rasterize_chi3, rasterize_conductivity
+from .smoothing import NoSmoothing, apply_subpixel_smoothing
 from .monitors import (DFTMonitor, FluxMonitor, FluxMonitorData, auto_decimate, expand_monitors,
                        init_dft_monitor, flux_from_dft)
 from .sources import assemble_source
@@ -63,7 +64,7 @@ class Simulation:
 
     def __init__(self, *, cell_size, resolution, sources, cell_center=(0.0, 0.0, 0.0),
                  boundaries=None, boundary_conditions=None, geometry=None, monitors=None,
-                 absorbers=None, Courant=0.5, precision="f64", mode="fast", dft_accum="f64", num_chunks=1,
+                 absorbers=None, subpixel_smoothing=None, Courant=0.5, precision="f64", mode="fast", dft_accum="f64", num_chunks=1,
                  engine=None):
         if num_chunks not in (1, None):
             raise NotImplementedError("only the single-domain layout (num_chunks=1) is provided")
@@ -81,6 +82,7 @@ class Simulation:
         self.boundaries = boundaries
         self.boundary_conditions = boundary_conditions
         self.absorbers = absorbers
+        self.subpixel_smoothing = subpixel_smoothing if subpixel_smoothing is not None else NoSmoothing()
         self.geometry = list(geometry) if geometry else []
         self.monitors = list(monitors) if monitors else []
         self.Courant = Courant
@@ -126,6 +128,8 @@ class Simulation:
         """Host setup of ``prepare_simulation!`` (src/Simulation.jl:92-283)."""
         sig = init_boundaries(self)
         eps_inv, mu_inv = rasterize(self, self.geometry)
+        # subpixel smoothing of eps^-1 before anything reads it (src/Geometry.jl:602-603)
+        eps_inv = apply_subpixel_smoothing(self, self.geometry, eps_inv)
         chi3 = rasterize_chi3(self, self.geometry)
         poles, eps_inv = init_polarization(self, self.geometry, eps_inv)
         # conductivity (src/Geometry.jl:467-511, :708-789)

@@ -5,7 +5,8 @@
   0.5, one z-normal DFT plane at 5 frequencies (the reference's dipole
   benchmark geometry, benchmark/dipole.jl:33-52, at size 4 / resolution 20).
 * ``strip_waveguide`` -- 19.2 x 9.6 x 8.0 um at 20 cells/um (384 x 192 x 160),
-  12-cell PML, eps 12.1 strip of 0.5 x 0.22 um along x in eps 2.085 cladding,
+  12-cell PML, eps 12.1 strip of 0.5 x 0.22 um along x in eps 2.085 cladding
+  with anisotropic subpixel smoothing (diagonal per-component eps^-1),
   Ez planar Gaussian-pulse source (fcen 1/1.55, fwidth 0.2 fcen) with a
   Gaussian transverse envelope at x = -7 um, two x-normal flux planes at 50
   frequencies.
@@ -18,6 +19,7 @@ from . import grid as G
 from .geometry import Box, Material, Object
 from .monitors import DFTMonitor, FluxMonitor
 from .simulation import Simulation
+from .smoothing import AnisotropicSmoothing, NoSmoothing
 from .sources import GaussianProfileSource, GaussianPulseSource, UniformSource
 
 
@@ -32,7 +34,7 @@ def dipole_80(precision="f64", mode="fast", engine=None, dft_accum="f64"):
 
 
 def strip_waveguide(precision="f32", mode="fast", engine=None, amplitude=1.0, n_freq=50,
-                    flux=True, waist=1.0, dft_accum="f64"):
+                    flux=True, waist=1.0, dft_accum="f64", smoothing=True):
     fcen = 1 / 1.55
     src = [GaussianProfileSource(time_profile=GaussianPulseSource(fcen=fcen, fwidth=0.2 * fcen),
                                  component=G.Ez(), center=[-7.0, 0.0, 0.0], size=[0.0, 8.4, 6.8],
@@ -46,7 +48,8 @@ def strip_waveguide(precision="f32", mode="fast", engine=None, amplitude=1.0, n_
                 FluxMonitor(center=[6.0, 0.0, 0.0], size=[0.0, 8.4, 6.8], frequencies=freqs)]
     return Simulation(cell_size=[19.2, 9.6, 8.0], resolution=20, sources=src, geometry=geo,
                       monitors=mons, boundaries=[[0.6, 0.6]] * 3, precision=precision, mode=mode,
-                      engine=engine, dft_accum=dft_accum)
+                      engine=engine, dft_accum=dft_accum,
+                      subpixel_smoothing=AnisotropicSmoothing() if smoothing else NoSmoothing())
 
 
 def algorithmic_bytes(spec) -> dict:

@@ -197,7 +197,8 @@ def test_abi_errors_are_reported_not_fatal():
 
 
 def _strip_sim(precision, mode, amp=1.0, steps_flux=True):
-    """Survey config 2 geometry at full size (384 x 192 x 160, 12-cell PML)."""
+    """Config 2 geometry at full size (384 x 192 x 160, 12-cell PML,
+    anisotropically smoothed strip)."""
     res = 20
     fcen = 1 / 1.55
     src = [kh.GaussianProfileSource(time_profile=kh.GaussianPulseSource(fcen=fcen, fwidth=0.2 * fcen),
@@ -211,7 +212,8 @@ def _strip_sim(precision, mode, amp=1.0, steps_flux=True):
         mons = [kh.FluxMonitor(center=[-6.0, 0.0, 0.0], size=[0.0, 8.4, 6.8], frequencies=freqs),
                 kh.FluxMonitor(center=[6.0, 0.0, 0.0], size=[0.0, 8.4, 6.8], frequencies=freqs)]
     return kh.Simulation(cell_size=[19.2, 9.6, 8.0], resolution=res, sources=src, geometry=geo,
-                         monitors=mons, boundaries=[[0.6, 0.6]] * 3, precision=precision, mode=mode)
+                         monitors=mons, boundaries=[[0.6, 0.6]] * 3, precision=precision, mode=mode,
+                         subpixel_smoothing=kh.AnisotropicSmoothing())
 
 
 def test_config2_full_size_fast_vs_exact_and_linearity():
