@@ -193,8 +193,13 @@ def main(argv=None):
         dom, dom_bytes, dom_name = avg["e_inner"], ab["E_inner"], "inner_e_kernel<float,true,true>"
         trace_name = "kh::inner_e_kernel<float, true, true>"
     elif args.mode == "fast":
-        dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "pml_e_kernel<float,true>"
-        trace_name = "kh::pml_e_kernel<float, true>"
+        # the library reads a byte per cell into a value table when the
+        # per-voxel eps^-1 (with the ghost value 1) holds at most 256 values
+        vals = np.unique(np.concatenate([np.ravel(a) for a in sim.spec.eps_inv if a is not None] + [np.ones(1)]))
+        lut = os.environ.get("KH_LUT", "1") != "0" and len(vals) <= 256
+        tmpl = "float, true, false, " + ("true" if lut else "false")
+        dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "pml_e_kernel<" + tmpl.replace(" ", "") + ">"
+        trace_name = "kh::pml_e_kernel<" + tmpl + ">"
     else:
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "cascade_e_kernel<float>"
         trace_name = "kh::cascade_e_kernel<float>"

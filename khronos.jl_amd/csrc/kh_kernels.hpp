@@ -619,6 +619,10 @@ struct PmlArgs {
     const T *msig[3];
     T *C[3];
     int clo[3], chi[3];
+    // per-voxel material as one byte per cell into a table of at most 256
+    // values (LUT kernels): the same numbers as minv, a quarter of the bytes
+    const uint8_t *mid[3];
+    const T *lut;
 };
 
 template <typename T>
@@ -650,9 +654,13 @@ struct PmlComp {
     bool nc;      // C stage (lane inside the conductivity box)
 };
 
-template <typename T, bool PERVOX>
+template <typename T, bool PERVOX, bool LUT>
 __device__ __forceinline__ void pml_load_a(const PmlArgs<T> &p, int a, size_t idx, PmlComp<T> &c) {
-    if (PERVOX) {
+    if (LUT) {
+        const unsigned short q = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p.mid[a] + idx));
+        c.m.x = p.lut[q & 0xff];
+        c.m.y = p.lut[q >> 8];
+    } else if (PERVOX) {
         c.m = ldo<T, true>(p.minv[a], idx);
     } else {
         c.m.x = p.minv_s;
@@ -730,12 +738,12 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t id
 
 // Loads of the updated field's state for the three components:
 // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y).
-template <typename T, bool PERVOX, bool COND>
+template <typename T, bool PERVOX, bool COND, bool LUT>
 __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T sx0, T sx1, T sgy, T sgz,
                                              int x, int y, int z, PmlComp<T> c[3]) {
-    pml_load_a<T, PERVOX>(p, 0, idx, c[0]);
-    pml_load_a<T, PERVOX>(p, 1, idx, c[1]);
-    pml_load_a<T, PERVOX>(p, 2, idx, c[2]);
+    pml_load_a<T, PERVOX, LUT>(p, 0, idx, c[0]);
+    pml_load_a<T, PERVOX, LUT>(p, 1, idx, c[1]);
+    pml_load_a<T, PERVOX, LUT>(p, 2, idx, c[2]);
     const bool px = sx0 != T(0) || sx1 != T(0), py = sgy != T(0), pz = sgz != T(0);
     pml_load_uw(p, 0, idx, py, px, c[0]);
     pml_load_uw(p, 1, idx, pz, py, c[1]);
@@ -753,7 +761,7 @@ __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T 
     }
 }
 
-template <typename T, bool PERVOX, bool COND>
+template <typename T, bool PERVOX, bool COND, bool LUT = false>
 __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     int x, y, z;
     bool v0, v1;
@@ -767,7 +775,7 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const auto ez_y1 = ld2(Ez, idx + sy), ex_y1 = ld2(Ex, idx + sy);
     const T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
     PmlComp<T> c[3];
-    pml_load_all<T, PERVOX, COND>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
+    pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (ey1.x - ey.x) - iy * (ez_y1.x - ez.x));
     const T Kx1 = dt * (iz * (ey1.y - ey.y) - iy * (ez_y1.y - ez.y));
@@ -780,7 +788,7 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
 }
 
-template <typename T, bool PERVOX, bool COND>
+template <typename T, bool PERVOX, bool COND, bool LUT = false>
 __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     int x, y, z;
     bool v0, v1;
@@ -794,7 +802,7 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
     const T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
     PmlComp<T> c[3];
-    pml_load_all<T, PERVOX, COND>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
+    pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (hym.x - hy.x) - iy * (hz_ym.x - hz.x));
     const T Kx1 = dt * (iz * (hym.y - hy.y) - iy * (hz_ym.y - hz.y));

@@ -157,6 +157,11 @@ struct Ctx final : CtxBase {
     // (0: H, 1: E) is nonzero (lo > hi: none)
     T *msig[6]{};
     int cbox[2][6] = {{1, 1, 1, 0, 0, 0}, {1, 1, 1, 0, 0, 0}};
+    // per-voxel eps^-1 / mu^-1 as a byte per cell into a table of <= 256
+    // values (group 0: mu, 1: eps), used by the PML kernels when it fits
+    uint8_t *mid[6]{};
+    T *lut[2]{};
+    bool use_lut[2] = {false, false};
     T eps_s, mu_s;
     int64_t ts = 0;
     bool active = true;
@@ -294,6 +299,8 @@ struct Ctx final : CtxBase {
             grow_box(cbox[c < 3 ? 1 : 0], c, (const T *)d.sigma_mat[c]);
             for (int p = 0; p < nparts; ++p) (c < 3 ? fp[p].CD : fp[p].CB)[c % 3] = field_alloc();
         }
+        if (epsi[0]) rebuild_lut(1);
+        if (mui[0]) rebuild_lut(0);
         scratch = (double *)dalloc(NPART * 64 * sizeof(double));
         clk = (int64_t *)dalloc(256);
         graph_steps = env_int("KH_GRAPH_STEPS", 16);
@@ -339,6 +346,52 @@ struct Ctx final : CtxBase {
         require(epsi[comp] != nullptr, "the context has no per-voxel permittivity (create it with eps_inv arrays)");
         HIPCHK(hipStreamSynchronize(st));
         upload_material(epsi[comp], comp, (const T *)a);
+        const bool was = use_lut[1];
+        rebuild_lut(1);
+        if (was != use_lut[1]) drop_graph();  // captured launches name the kernel variant
+    }
+
+    // Table form of group e's per-voxel material (all three components):
+    // when the padded arrays hold at most 256 distinct values, a byte per
+    // cell indexes a table of them.  KH_LUT=0 keeps the float arrays.
+    void rebuild_lut(int e) {
+        T *const *src = e ? epsi : mui;
+        use_lut[e] = false;
+        if (env_int("KH_LUT", 1) == 0) return;
+        const size_t n = (size_t)(g.total - g.front);
+        std::vector<T> vals;
+        std::vector<std::vector<uint8_t>> idx(3, std::vector<uint8_t>(n));
+        std::vector<T> h(n);
+        for (int a = 0; a < 3; ++a) {
+            HIPCHK(hipMemcpyAsync(h.data(), src[a], n * sizeof(T), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            T last = T(0);
+            int li = -1;
+            for (size_t i = 0; i < n; ++i) {
+                const T v = h[i];
+                if (li < 0 || !(v == last) || std::signbit(v) != std::signbit(last)) {
+                    li = -1;
+                    for (size_t k = 0; k < vals.size(); ++k)
+                        if (vals[k] == v && std::signbit(vals[k]) == std::signbit(v)) { li = (int)k; break; }
+                    if (li < 0) {
+                        if (vals.size() == 256) return;  // too many values: float arrays
+                        vals.push_back(v);
+                        li = (int)vals.size() - 1;
+                    }
+                    last = v;
+                }
+                idx[a][i] = (uint8_t)li;
+            }
+        }
+        if (!lut[e]) lut[e] = (T *)dalloc(256 * sizeof(T));
+        HIPCHK(hipMemcpyAsync(lut[e], vals.data(), vals.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        for (int a = 0; a < 3; ++a) {
+            uint8_t *&m = mid[3 * (1 - e) + a];  // mid[0..2] eps (E), mid[3..5] mu (H)
+            if (!m) m = (uint8_t *)dalloc((size_t)g.total) + g.front;
+            HIPCHK(hipMemcpyAsync(m, idx[a].data(), n, hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        use_lut[e] = true;
     }
 
     ~Ctx() override {
@@ -976,6 +1029,8 @@ struct Ctx final : CtxBase {
                 q.chi[a] = cbox[e][3 + a];
             }
             const bool cond = q.C[0] || q.C[1] || q.C[2];
+            for (int a = 0; a < 3; ++a) q.mid[a] = mid[(electric ? 0 : 3) + a];
+            q.lut = lut[e];
             q.minv_s = electric ? eps_s : mu_s;
             q.dt = dt;
             q.sy = g.sy();
@@ -986,8 +1041,12 @@ struct Ctx final : CtxBase {
             q.per_xcd = (P.pblocks + kh::NXCD - 1) / kh::NXCD;
             const bool pervox = q.minv[0] != nullptr;
             const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
+            const bool lut_on = pervox && use_lut[e];
             if (electric) {
-                if (cond) {
+                if (lut_on) {
+                    if (cond) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, true>), pgrid, pblk, 0, st, q);
+                } else if (cond) {
                     if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true>), pgrid, pblk, 0, st, q);
                     else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, true>), pgrid, pblk, 0, st, q);
                 } else {
@@ -995,7 +1054,10 @@ struct Ctx final : CtxBase {
                     else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, false>), pgrid, pblk, 0, st, q);
                 }
             } else {
-                if (cond) {
+                if (lut_on) {
+                    if (cond) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, true, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, true>), pgrid, pblk, 0, st, q);
+                } else if (cond) {
                     if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, true>), pgrid, pblk, 0, st, q);
                     else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, true>), pgrid, pblk, 0, st, q);
                 } else {
@@ -1425,11 +1487,14 @@ size_t footprint(const kh_sim_desc &d) {
     // E, H, U and W per component (per part with complex fields); D and B only in EXACT mode
     size_t arrays = (d.mode == KH_MODE_EXACT ? 24 : 18) * (d.complex_fields ? 2 : 1);
     for (int a = 0; a < 3; ++a) arrays += (d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr);
+    // byte-per-cell material indices (when the values fit a table)
+    size_t idx_bytes = 0;
+    for (int a = 0; a < 3; ++a) idx_bytes += ((d.eps_inv[a] != nullptr) + (d.mu_inv[a] != nullptr)) * (size_t)g.total;
     // conductivity: sigma plus C per part
     for (int c = 0; c < 6; ++c) arrays += (d.sigma_mat[c] != nullptr) * (1 + (d.complex_fields ? 2 : 1));
     size_t sig = 0;
     for (int a = 0; a < 3; ++a) sig += 2 * ((size_t)g.n[a] + 2) * es;
-    return arrays * (size_t)g.total * es + sig + 256 * 64 * sizeof(double);
+    return arrays * (size_t)g.total * es + idx_bytes + sig + 256 * 64 * sizeof(double);
 }
 
 void validate(const kh_sim_desc *d) {

@@ -226,10 +226,15 @@ def test_config2_full_size_fast_vs_exact_and_linearity():
         s.step(steps)
     tot, per = field_errors(a, b)
     assert tot < 1e-5 and max(per.values()) < 1e-5, (tot, per)
-    for C in COMPS:
-        fa, fc = a.read_field(C()), c.read_field(C())
+    fas = [a.read_field(C()) for C in COMPS]
+    fcs = [c.read_field(C()) for C in COMPS]
+    norm = np.sqrt(sum(float(np.sum(2.0 * f.astype(np.float64) ** 2 * 2.0)) for f in fas))
+    for C, fa, fc in zip(COMPS, fas, fcs):
         assert np.all(np.isfinite(fa))
         # scaling every input by 2 scales every rounding by 2, except where
-        # values fall into the subnormal range (wavefront edge, deep PML)
-        assert rel_l2(fc, 2 * fa) < 1e-6, C.__name__
+        # values fall into the subnormal range (wavefront edge, deep PML); a
+        # numerically zero cross-polarized component (|Hz| ~ 1e-19 here) is
+        # measured against the whole field
+        d = np.linalg.norm(fc.astype(np.float64) - 2.0 * fa.astype(np.float64))
+        assert d / norm < 1e-6, C.__name__
     del a, b, c
