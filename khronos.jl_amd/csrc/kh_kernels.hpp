@@ -844,10 +844,6 @@ __global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
     const int b = blockIdx.x;
     while (q + 1 < p.nsrc && b >= p.src[q + 1].first) ++q;
     const FixSrc<T> &o = p.src[q];
-    const int64_t ts = *p.clk + p.koff;
-    const bool now = sources_on(ts, p.dt, p.cutoff);
-    const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
-    if (!now && !before) return;
     const int64_t n = (int64_t)o.d[0] * o.d[1] * o.d[2];
     const int64_t c = (int64_t)(b - o.first) * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -857,21 +853,30 @@ __global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
     const int g[3] = {o.s[0] + ix, o.s[1] + iy, o.s[2] + iz};
     // only updated cells consume sources (grid index 1..N)
     if (g[0] < 1 || g[0] > p.n0 || g[1] < 1 || g[1] > p.n1 || g[2] < 1 || g[2] > p.n2) return;
+    // everything that does not depend on the timestep is loaded together
+    // with the timestep, so the lane waits for memory twice (timestep, then
+    // the waveform tables) rather than once per dependent load
+    const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+    const T ar = o.amp[2 * c], ai = o.amp[2 * c + 1];
+    const T m = o.minv ? o.minv[idx] : p.minv_s;
+    const T so = o.sig_own[g[o.own]], sp = o.sig_prev[g[o.prev]];
+    const T ao = o.A[idx];
+    const T wo = so == T(0) ? T(0) : o.W[idx];
+    const int64_t ts = *p.clk + p.koff;
+    const bool now = sources_on(ts, p.dt, p.cutoff);
+    const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
+    if (!now && !before) return;
     T fr = 0, fi = 0, pr = 0, pi = 0;
     if (now) wave_value(o.wave, o.nwave, ts, fr, fi);
     if (before) wave_value(o.wave, o.nwave, ts - 1, pr, pi);
-    const T ar = o.amp[2 * c], ai = o.amp[2 * c + 1];
     const T vn = (fr * ar) - (fi * ai);
     const T vo = (pr * ar) - (pi * ai);
-    const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
-    const T m = o.minv ? o.minv[idx] : p.minv_s;
-    const T so = o.sig_own[g[o.own]], sp = o.sig_prev[g[o.prev]];
     const T dw = m * (vn - ((T(1) - sp) / (T(1) + sp)) * vo);
     if (so == T(0)) {
-        o.A[idx] = o.A[idx] + dw;
+        o.A[idx] = ao + dw;
     } else {
-        o.W[idx] = o.W[idx] + dw;
-        o.A[idx] = o.A[idx] + (T(1) + so) * dw;
+        o.W[idx] = wo + dw;
+        o.A[idx] = ao + (T(1) + so) * dw;
     }
 }
 
