@@ -7,7 +7,7 @@ hand-written gfx950 HIP library behind include/khronos_hip.h.
 from .grid import (Ex, Ey, Ez, Hx, Hy, Hz, Center, Field, Volume, GridVolume, grid_volume,
                    component_voxel_count, interpolation_weight)
 from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, UniformSource,
-                      ArraySource, GaussianProfileSource)
+                      ArraySource, GaussianProfileSource, PlaneWaveSource)
 from .monitors import DFTMonitor, FluxMonitor
 from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
 from .geometry import Material, Object, Box, Sphere, Cylinder
@@ -21,7 +21,7 @@ __all__ = [
     "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Center", "Field", "Volume", "GridVolume", "grid_volume",
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
-    "GaussianProfileSource", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
+    "GaussianProfileSource", "PlaneWaveSource", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
     "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere", "SubpixelSmoothing", "NoSmoothing",
     "VolumeAveraging", "AnisotropicSmoothing",
     "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",

@@ -28,7 +28,9 @@ class HipEngine:
     def __init__(self, spec):
         self.lib = _lib.load()
         self.spec = spec
-        R = spec.real
+        # field precision; the host setup (and the waveform values) may be
+        # in a wider one (the precision form: FP64 setup, FP32 fields)
+        R = getattr(spec, "field_real", None) or spec.real
         self.R = R
         self._keep = []  # host arrays referenced during kh_create
         d = _lib.KhSimDesc()
@@ -120,7 +122,7 @@ class HipEngine:
 
     # -------------------------------------------------------------- waveforms
     def _wave_times(self, electric: bool, n0: int, n1: int):
-        R = self.R
+        R = self.spec.real
         dt = R(self.spec.dt)
         half = float(dt / R(2))
         ts = [float(R(n) * dt) for n in range(n0, n1)]
@@ -143,7 +145,7 @@ class HipEngine:
             times = self._wave_times(sd.component.electric, have, target)
             new = np.empty(2 * len(times))
             for i, t in enumerate(times):
-                v = sd.time_profile.eval(t, self.R)
+                v = sd.time_profile.eval(t, self.spec.real)
                 new[2 * i] = v.real
                 new[2 * i + 1] = v.imag
             self._wave[q] = np.concatenate([self._wave[q], new])

@@ -55,6 +55,7 @@ class SimSpec:
     periodic: tuple = (False, False, False)
     bloch_fwd: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(+i k L): lower interior -> upper ghost
     bloch_rev: tuple = (1 + 0j, 1 + 0j, 1 + 0j)   # exp(-i k L): upper interior -> lower ghost
+    field_real: Optional[type] = None   # field precision when it differs from the setup's
     # conductivity sigma_D of Ex,Ey,Ez then sigma_B of Hx,Hy,Hz (geometry layout, absorbers added)
     cond: list = field(default_factory=lambda: [None] * 6)
 
@@ -64,13 +65,20 @@ class Simulation:
 
     def __init__(self, *, cell_size, resolution, sources, cell_center=(0.0, 0.0, 0.0),
                  boundaries=None, boundary_conditions=None, geometry=None, monitors=None,
-                 absorbers=None, subpixel_smoothing=None, Courant=0.5, precision="f64", mode="fast", dft_accum="f64", num_chunks=1,
-                 engine=None):
+                 absorbers=None, subpixel_smoothing=None, Courant=0.5, precision="f64", mode="fast",
+                 dft_accum="f64", num_chunks=1, engine=None, setup_precision=None):
         if num_chunks not in (1, None):
             raise NotImplementedError("only the single-domain layout (num_chunks=1) is provided")
         if precision not in ("f32", "f64"):
             raise ValueError("precision must be 'f32' or 'f64'")
-        self.real = np.float32 if precision == "f32" else np.float64
+        if setup_precision not in (None, "f32", "f64"):
+            raise ValueError("setup_precision must be 'f32' or 'f64'")
+        # the precision form: setup_precision="f64" with precision="f32" keeps
+        # the grid snapping, weights, profiles and waveforms of the FP64 run
+        # and runs the time step in FP32
+        self.field_real = np.float32 if precision == "f32" else np.float64
+        self.real = self.field_real if setup_precision is None else (
+            np.float32 if setup_precision == "f32" else np.float64)
         R = self.real
         self.precision = precision
         self.mode = mode
@@ -151,7 +159,7 @@ class Simulation:
         dft_mons = expand_monitors(self, self.monitors)
         dfts = [init_dft_monitor(self, m) for m in dft_mons]
         return SimSpec(
-            N=(self.Nx, self.Ny, max(1, self.Nz)), real=self.real, delta=tuple(self.delta_n), dt=self.dt_n,
+            N=(self.Nx, self.Ny, max(1, self.Nz)), real=self.real, field_real=self.field_real, delta=tuple(self.delta_n), dt=self.dt_n,
             sigma=sig, eps_inv=eps_inv, mu_inv=mu_inv, eps_inv_scalar=1.0, mu_inv_scalar=1.0,
             sources=srcs, dfts=dfts, cutoff=self.last_source_time(), mode=self.mode,
             dft_accum_f64=self.dft_accum_f64, chi3=chi3, poles=poles,

@@ -208,6 +208,88 @@ def GaussianProfileSource(*, time_profile, component, center, size, waist, ampli
                        amplitude=amplitude, center=center, size=size)
 
 
+class PlaneWaveSource(Source):
+    """Plane-wave current sheet (src/Sources/SpatialSources.jl:412-586).
+
+    ``k_vector`` or the angles (theta azimuth, phi polar) give the direction;
+    k = 2 pi fcen sqrt(eps mu).  The polarization is the normal of the plane
+    spanned by the sheet normal and k (fixed axes when they coincide) rotated
+    by ``polarization_angle`` about k.  Electric currents carry
+    n x (k x E / Z0), magnetic ones -n x E, on the sheet's transverse
+    components whose factor is nonzero."""
+
+    def __init__(self, *, time_profile, center=(0.0, 0.0, 0.0), size=(0.0, 0.0, 0.0), amplitude=1.0,
+                 ε=1.0, μ=1.0, polarization_angle=0.0, angle_theta=None, angle_phi=None, k_vector=None):
+        if k_vector is None and angle_theta is None and angle_phi is None:
+            raise ValueError("Must either specify a `k_vector` or `angle_phi` and `angle_theta`.")
+        if k_vector is None:
+            k_vector = [math.sin(angle_phi) * math.cos(angle_theta), math.sin(angle_phi) * math.sin(angle_theta),
+                        math.cos(angle_phi)]
+        self.time_profile = time_profile
+        self.center = [float(v) for v in center]
+        self.size = [float(v) for v in size]
+        self.amplitude = amplitude
+        self.ε, self.μ = ε, μ
+        self.k = 2 * math.pi * float(time_profile.fcen) * math.sqrt(ε * μ)
+        kv = np.asarray(k_vector, dtype=float)
+        self.k_vector = kv / np.linalg.norm(kv)
+        zeros = [a for a in range(3) if self.size[a] == 0.0]
+        if len(zeros) != 1:
+            raise ValueError(f"The specified volume is not a plane (size={self.size})")
+        self.normal_axis = int(np.argmin(self.size))
+        n = np.zeros(3)
+        n[self.normal_axis] = 1.0
+        self.normal_direction = n
+        if float(np.dot(n, self.k_vector)) == 1.0:
+            first = int(np.nonzero(self.k_vector != 0.0)[0][0])
+            ppn = {0: [0.0, 1.0, 0.0], 1: [0.0, 0.0, 1.0], 2: [1.0, 0.0, 0.0]}[first]
+            ppn = np.asarray(ppn)
+        else:
+            ppn = np.cross(n, self.k_vector)
+        # Rodrigues' rotation about k (the k (k . p) term vanishes: p is normal to k)
+        self.polarization = ppn * math.cos(polarization_angle) + np.cross(self.k_vector, ppn) * math.sin(
+            polarization_angle)
+
+    def pol_factor(self, component) -> float:
+        a = component.axis
+        if component.electric:
+            z0 = math.sqrt(self.μ / self.ε)
+            return float(np.cross(self.normal_direction, np.cross(self.k_vector, (1 / z0) * self.polarization))[a])
+        return float(-np.cross(self.normal_direction, self.polarization)[a])
+
+    def components(self):
+        from .grid import Ex, Ey, Ez, Hx, Hy, Hz
+        trans = {0: [Ey(), Ez(), Hy(), Hz()], 1: [Ex(), Ez(), Hx(), Hz()], 2: [Ex(), Ey(), Hx(), Hy()]}
+        return [c for c in trans[self.normal_axis] if self.pol_factor(c) != 0.0]
+
+
+def _planewave_amplitude(sim, src: PlaneWaveSource, comp, gv, org, dims, mn, mx, d):
+    """The reference's separable assembly (src/Sources/Sources.jl:142-233):
+    per axis weight(p) exp(-i k k_a p), weight 1 along an infinite extent;
+    the z factor is dropped when the sheet is one cell thick in z; times
+    (pol_factor * amplitude)."""
+    n1 = dims[0]
+    n2 = dims[1] if sim.ndims >= 2 else 1
+    n3 = dims[2] if sim.ndims >= 3 else 1
+    k = float(src.k)
+    fac = []
+    for a, n in enumerate((n1, n2, n3)):
+        out = np.empty(n, dtype=np.complex128)
+        for i in range(1, n + 1):
+            p = org[a] + (i + gv.start_idx[a] - 2) * d[a]
+            if a == 2 and (math.isinf(p) or math.isnan(p)):
+                p = float(sim.cell_center[2])
+            w = 1.0 if math.isinf(float(src.size[a])) else weight_1d(p, mn[a], mx[a], float(src.size[a]), d[a])
+            out[i - 1] = w * np.exp(-1j * k * float(src.k_vector[a]) * p)
+        fac.append(out)
+    scale = complex(src.pol_factor(comp) * src.amplitude)
+    amp = np.zeros(dims, dtype=np.complex128)
+    cxy = fac[0][:, None] * fac[1][None, :]
+    cz = fac[2] if n3 > 1 else np.ones(1, dtype=np.complex128)
+    amp[:n1, :n2, :n3] = (cxy[:, :, None] * cz[None, None, :]) * scale
+    return amp
+
+
 @dataclass
 class SourceData:
     """What the time step needs for one (source, component) pair."""
@@ -230,6 +312,10 @@ def assemble_source(sim, src: Source):
         gv = grid_volume(sim, vol, comp)
         org = component_origin(sim.cell_center, sim.cell_size_n, sim.delta_n, comp)
         dims = (gv.Nx, gv.Ny, max(1, gv.Nz))
+        if isinstance(src, PlaneWaveSource):
+            amp = _planewave_amplitude(sim, src, comp, gv, org, dims, mn, mx, d)
+            out.append(SourceData(comp, gv, np.ascontiguousarray(amp), src.time_profile))
+            continue
         coords = []
         weights = []
         for a in range(3):

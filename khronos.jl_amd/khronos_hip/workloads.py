@@ -10,17 +10,26 @@
   Ez planar Gaussian-pulse source (fcen 1/1.55, fwidth 0.2 fcen) with a
   Gaussian transverse envelope at x = -7 um, two x-normal flux planes at 50
   frequencies.
+* ``metasurface`` -- 8 x 8 lattice (a = 1 um) at 32 cells/um, 256 x 256 x 384
+  cells, Bloch-periodic x/y with (kx, ky) ~ U(-pi/a, pi/a), 16-cell PML in z;
+  cylinders of radius U[0.2, 0.4] a, height 0.5 um, eps_inf 2 with two
+  Lorentz poles (omega_0 ~ U[0.8, 1.5], gamma 0.05, per-cylinder sigma ~
+  U[0.5, 2]) and chi3 = 1e-3; a plane-wave Gaussian pulse whose transverse k
+  matches the Bloch phase; one z-normal flux plane at 100 frequencies;
+  complex FP32 fields, FP64 DFT accumulators (all draws from ``seed``).
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import grid as G
-from .geometry import Box, Material, Object
+from .boundaries import PML, Bloch
+from .geometry import Box, Cylinder, Material, Object
 from .monitors import DFTMonitor, FluxMonitor
 from .simulation import Simulation
 from .smoothing import AnisotropicSmoothing, NoSmoothing
-from .sources import GaussianProfileSource, GaussianPulseSource, UniformSource
+from .sources import GaussianProfileSource, GaussianPulseSource, PlaneWaveSource, UniformSource
+from .susceptibility import LorentzianSusceptibility
 
 
 def dipole_80(precision="f64", mode="fast", engine=None, dft_accum="f64"):
@@ -34,7 +43,7 @@ def dipole_80(precision="f64", mode="fast", engine=None, dft_accum="f64"):
 
 
 def strip_waveguide(precision="f32", mode="fast", engine=None, amplitude=1.0, n_freq=50,
-                    flux=True, waist=1.0, dft_accum="f64", smoothing=True):
+                    flux=True, waist=1.0, dft_accum="f64", smoothing=True, setup_precision=None):
     fcen = 1 / 1.55
     src = [GaussianProfileSource(time_profile=GaussianPulseSource(fcen=fcen, fwidth=0.2 * fcen),
                                  component=G.Ez(), center=[-7.0, 0.0, 0.0], size=[0.0, 8.4, 6.8],
@@ -48,8 +57,40 @@ def strip_waveguide(precision="f32", mode="fast", engine=None, amplitude=1.0, n_
                 FluxMonitor(center=[6.0, 0.0, 0.0], size=[0.0, 8.4, 6.8], frequencies=freqs)]
     return Simulation(cell_size=[19.2, 9.6, 8.0], resolution=20, sources=src, geometry=geo,
                       monitors=mons, boundaries=[[0.6, 0.6]] * 3, precision=precision, mode=mode,
-                      engine=engine, dft_accum=dft_accum,
+                      engine=engine, dft_accum=dft_accum, setup_precision=setup_precision,
                       subpixel_smoothing=AnisotropicSmoothing() if smoothing else NoSmoothing())
+
+
+def metasurface(precision="f32", mode="fast", engine=None, seed=0, n_freq=100, dft_accum="f64", nz=384,
+                lattice=8, res=32):
+    import math
+    rng = np.random.default_rng(seed)
+    a = 1.0
+    kx, ky = (float(v) for v in rng.uniform(-math.pi / a, math.pi / a, 2))
+    w0 = [float(v) for v in rng.uniform(0.8, 1.5, 2)]
+    L = lattice * a
+    geo = []
+    for i in range(lattice):
+        for j in range(lattice):
+            r = float(rng.uniform(0.2, 0.4)) * a
+            s1, s2 = (float(v) for v in rng.uniform(0.5, 2.0, 2))
+            poles = [LorentzianSusceptibility(w0[0], 0.05, s1), LorentzianSusceptibility(w0[1], 0.05, s2)]
+            c = [-L / 2 + (i + 0.5) * a, -L / 2 + (j + 0.5) * a, 0.0]
+            geo.append(Object(Cylinder(c, r, 0.5, axis=2), Material(ε=2.0, chi3=1e-3, susceptibilities=poles)))
+    fcen, fwidth = 1.0, 0.5
+    k = 2 * math.pi * fcen
+    # the sheet's phase exp(-i k.r) repeats with exp(-i k_t L): transverse k = -Bloch k
+    kv = [-kx / k, -ky / k, math.sqrt(max(0.0, 1 - (kx * kx + ky * ky) / (k * k)))]
+    cz = nz / res
+    src = [PlaneWaveSource(time_profile=GaussianPulseSource(fcen=fcen, fwidth=fwidth), center=[0.0, 0.0, -cz / 4],
+                           size=[math.inf, math.inf, 0.0], k_vector=kv)]
+    freqs = list(np.linspace(fcen - fwidth / 2, fcen + fwidth / 2, n_freq))
+    mons = [FluxMonitor(center=[0.0, 0.0, cz / 4], size=[L, L, 0.0], frequencies=freqs)]
+    pml = 16 / res
+    bcs = [[Bloch(k=kx), Bloch(k=kx)], [Bloch(k=ky), Bloch(k=ky)], [PML(), PML()]]
+    return Simulation(cell_size=[L, L, cz], resolution=res, sources=src, geometry=geo, monitors=mons,
+                      boundaries=[[0.0, 0.0], [0.0, 0.0], [pml, pml]], boundary_conditions=bcs,
+                      precision=precision, mode=mode, engine=engine, dft_accum=dft_accum)
 
 
 def algorithmic_bytes(spec) -> dict:
