@@ -239,6 +239,19 @@ int kh_read_dft(kh_ctx *ctx, int32_t dft_id, double *host, size_t bytes);
 int kh_flux(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2,
             int32_t normal_axis, double dA, double *out);
 
+/* Mode overlap integrals of a planar mode monitor from its four tangential
+ * DFTs (e1, e2, h1, h2 as for kh_flux): `mode` holds the mode's tangential
+ * E (m1, m2) and H (n1, n2) sampled where the normal-averaged DFT samples
+ * lie, complex (re, im) float64, index ((q * n_freq + k) * n2 + i2) * n1 + i1
+ * for q = m1, m2, n1, n2 and i1 < n1, i2 < n2 along the two tangential axes
+ * in increasing axis order.  out[4 * k ..] = sum (s + t) dA and sum (s - t) dA
+ * as (re, im), with s = (E_sim x H_mode*) . n and t = (E_mode* x H_sim) . n
+ * in kh_flux's component order; a+- = these / (4 P_mode).  Reduction on the
+ * device.  Replaces: compute_mode_amplitudes
+ * (src/Monitors/ModeMonitor.jl:345-507). */
+int kh_mode_overlap(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                    int32_t n1, int32_t n2, const double *mode, double dA, double *out);
+
 /* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
  * dft_id = -1 gives the sum of that over all monitors.
  * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */

@@ -1228,6 +1228,69 @@ __global__ __launch_bounds__(256) void flux_kernel(FluxMon e1, FluxMon e2, FluxM
     if (threadIdx.x == 0) out[k] = s;
 }
 
+// Mode overlaps of a planar monitor (src/Monitors/ModeMonitor.jl:345-507):
+// one block per frequency; after the normal-axis average of each DFT (as in
+// flux_kernel), with (e1, e2, h1, h2) the simulation's and (m1, m2, n1, n2)
+// the mode's tangential E and H at the same samples,
+//   s = e1 conj(n2) - e2 conj(n1)      (E_sim x H_mode*) . n
+//   t = conj(m1) h2 - conj(m2) h1      (E_mode* x H_sim) . n
+// out[4k..4k+3] = sum (s + t) dA, sum (s - t) dA as (re, im) pairs.  The
+// mode is mode[((q * nf + k) * n2 + i2) * n1 + i1] (re, im), q = m1, m2, n1, n2.
+template <typename ACC>
+__global__ __launch_bounds__(256) void mode_overlap_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2,
+                                                           int ax, int t0, int t1, int n1, int n2, int nf,
+                                                           const double *mode, double dA, double *out) {
+    const int k = blockIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
+    const int64_t plane = (int64_t)n1 * n2;
+    for (int c = threadIdx.x; c < n1 * n2; c += blockDim.x) {
+        const int i1 = c % n1, i2 = c / n1;
+        double re[4], im[4];
+        for (int q = 0; q < 4; ++q) {
+            const FluxMon &M = *m[q];
+            const ACC *A = (const ACC *)M.acc;
+            int ii[3];
+            ii[ax] = 0;
+            ii[t0] = i1;
+            ii[t1] = i2;
+            const int64_t nc = (int64_t)M.d[0] * M.d[1] * M.d[2];
+            const int64_t lin = ii[0] + (int64_t)M.d[0] * (ii[1] + (int64_t)M.d[1] * ii[2]);
+            const ACC *p0 = A + 2 * (lin + nc * k);
+            double r = (double)p0[0], i = (double)p0[1];
+            if (M.d[ax] >= 2) {
+                const int64_t step = ax == 0 ? 1 : (ax == 1 ? (int64_t)M.d[0] : (int64_t)M.d[0] * M.d[1]);
+                const ACC *p1 = p0 + 2 * step;
+                r = (r + (double)p1[0]) / 2;
+                i = (i + (double)p1[1]) / 2;
+            }
+            re[q] = r;
+            im[q] = i;
+        }
+        double mr[4], mi[4];
+        for (int q = 0; q < 4; ++q) {
+            const double *pm = mode + 2 * ((int64_t)(q * nf + k) * plane + c);
+            mr[q] = pm[0];
+            mi[q] = pm[1];
+        }
+        // s = e1 conj(n2) - e2 conj(n1)
+        const double sr = (re[0] * mr[3] + im[0] * mi[3]) - (re[1] * mr[2] + im[1] * mi[2]);
+        const double si = (im[0] * mr[3] - re[0] * mi[3]) - (im[1] * mr[2] - re[1] * mi[2]);
+        // t = conj(m1) h2 - conj(m2) h1
+        const double tr = (mr[0] * re[3] + mi[0] * im[3]) - (mr[1] * re[2] + mi[1] * im[2]);
+        const double ti = (mr[0] * im[3] - mi[0] * re[3]) - (mr[1] * im[2] - mi[1] * re[2]);
+        acc[0] += (sr + tr) * dA;
+        acc[1] += (si + ti) * dA;
+        acc[2] += (sr - tr) * dA;
+        acc[3] += (si - ti) * dA;
+    }
+    for (int q = 0; q < 4; ++q) {
+        const double v = block_sum(acc[q]);
+        if (threadIdx.x == 0) out[4 * k + q] = v;
+        __syncthreads();   // block_sum's LDS partials are reused by the next sum
+    }
+}
+
 // Partial sums of |F|^2 over a complex array of n elements.
 template <typename ACC>
 __global__ __launch_bounds__(256) void sumsq_kernel(const ACC *a, int64_t n, double *partial) {

@@ -122,6 +122,8 @@ struct CtxBase {
     virtual void read_dft(int id, double *host, size_t bytes) = 0;
     virtual void flux(int e1, int e2, int h1, int h2, int ax, double dA, double *out) = 0;
     virtual void dft_norm(int id, double *out) = 0;
+    virtual void mode_overlap(int e1, int e2, int h1, int h2, int ax, int n1, int n2, const double *mode,
+                              double dA, double *out) = 0;
     virtual void reset() = 0;
     virtual void set_timing(int n) = 0;
     virtual void get_timing(double *out) = 0;
@@ -1388,6 +1390,43 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    double *mode_buf = nullptr;   // mode profiles of the last overlap (grown on demand)
+    size_t mode_cap = 0;
+
+    void mode_overlap(int e1, int e2, int h1, int h2, int ax1, int n1, int n2, const double *mode, double dA,
+                      double *out) override {
+        int ids[4] = {e1, e2, h1, h2};
+        for (int q = 0; q < 4; ++q) require(ids[q] >= 0 && ids[q] < (int)dfts.size(), "bad dft id");
+        require(ax1 >= 1 && ax1 <= 3, "normal axis must be 1..3");
+        const int nf = dfts[e1].nf;
+        for (int q = 0; q < 4; ++q) require(dfts[ids[q]].nf == nf, "mode monitor DFTs differ in n_freq");
+        int ax = ax1 - 1, t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+        if (t0 > t1) std::swap(t0, t1);
+        kh::FluxMon fm[4];
+        for (int q = 0; q < 4; ++q) {
+            Dft &m = dfts[ids[q]];
+            fm[q].acc = m.acc;
+            for (int a = 0; a < 3; ++a) fm[q].d[a] = m.d[a];
+            require(n1 >= 1 && n2 >= 1 && n1 <= m.d[t0] && n2 <= m.d[t1], "mode plane larger than a DFT plane");
+        }
+        require(nf <= NPART * 16, "too many frequencies");
+        const size_t bytes = (size_t)4 * nf * n1 * n2 * 2 * sizeof(double);
+        if (bytes > mode_cap) {
+            mode_buf = (double *)dalloc(bytes);
+            mode_cap = bytes;
+        }
+        HIPCHK(hipMemcpyAsync(mode_buf, mode, bytes, hipMemcpyHostToDevice, st));
+        if (acc64)
+            hipLaunchKernelGGL(kh::mode_overlap_kernel<double>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2],
+                               fm[3], ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, scratch);
+        else
+            hipLaunchKernelGGL(kh::mode_overlap_kernel<T>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
+                               ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, scratch);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, scratch, (size_t)4 * nf * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
     void dft_norm(int id, double *out) override {
         require(id >= -1 && id < (int)dfts.size(), "bad dft id");
         double total = 0.0;
@@ -1695,6 +1734,15 @@ int kh_flux(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t
         CTX_OK;
         require(out != nullptr, "null argument");
         ctx->impl->flux(e1, e2, h1, h2, normal_axis, dA, out);
+    });
+}
+
+int kh_mode_overlap(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                    int32_t n1, int32_t n2, const double *mode, double dA, double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(mode != nullptr && out != nullptr, "null argument");
+        ctx->impl->mode_overlap(e1, e2, h1, h2, normal_axis, n1, n2, mode, dA, out);
     });
 }
 

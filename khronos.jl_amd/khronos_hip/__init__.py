@@ -7,8 +7,11 @@ hand-written gfx950 HIP library behind include/khronos_hip.h.
 from .grid import (Ex, Ey, Ez, Hx, Hy, Hz, Center, Field, Volume, GridVolume, grid_volume,
                    component_voxel_count, interpolation_weight)
 from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, UniformSource,
-                      ArraySource, GaussianProfileSource, PlaneWaveSource)
-from .monitors import DFTMonitor, FluxMonitor
+                      ArraySource, GaussianProfileSource, PlaneWaveSource,
+                      EquivalentSource, ModeSource)
+from .monitors import (DFTMonitor, FluxMonitor, ModeMonitor, compute_mode_amplitudes,
+                       get_mode_transmission)
+from .mode import ModeSpec
 from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
 from .geometry import Material, Object, Box, Sphere, Cylinder
 from .smoothing import SubpixelSmoothing, NoSmoothing, VolumeAveraging, AnisotropicSmoothing
@@ -21,7 +24,8 @@ __all__ = [
     "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Center", "Field", "Volume", "GridVolume", "grid_volume",
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
-    "GaussianProfileSource", "PlaneWaveSource", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
+    "GaussianProfileSource", "PlaneWaveSource", "EquivalentSource", "ModeSource", "ModeMonitor", "ModeSpec",
+    "compute_mode_amplitudes", "get_mode_transmission", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
     "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere", "SubpixelSmoothing", "NoSmoothing",
     "VolumeAveraging", "AnisotropicSmoothing",
     "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",

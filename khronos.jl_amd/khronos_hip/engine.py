@@ -242,6 +242,18 @@ class HipEngine:
         _lib.check(self.lib.kh_flux(self.ctx, e1, e2, h1, h2, int(normal_axis), float(dA), _ptr(out)))
         return out
 
+    def mode_overlap(self, ids, normal_axis: int, mode: np.ndarray, dA: float) -> np.ndarray:
+        """Overlap sums (n_freq, 2) complex [sum (s + t) dA, sum (s - t) dA] of
+        kh_mode_overlap; ``mode`` is complex (4, n_freq, n2, n1)."""
+        q, nf, n2, n1 = mode.shape
+        buf = np.ascontiguousarray(mode, dtype=np.complex128)
+        out = np.zeros(4 * nf, dtype=np.float64)
+        e1, e2, h1, h2 = (self.dft_ids[i] for i in ids)
+        _lib.check(self.lib.kh_mode_overlap(self.ctx, e1, e2, h1, h2, int(normal_axis), int(n1), int(n2),
+                                            _ptr(buf), float(dA), _ptr(out)))
+        o = out.reshape(nf, 4)
+        return np.stack([o[:, 0] + 1j * o[:, 1], o[:, 2] + 1j * o[:, 3]], axis=1)
+
     def dft_norms(self):
         out = []
         for did in self.dft_ids:

@@ -208,6 +208,67 @@ def GaussianProfileSource(*, time_profile, component, center, size, waist, ampli
                        amplitude=amplitude, center=center, size=size)
 
 
+class EquivalentSource(Source):
+    """Currents equivalent to given E and H on a plane
+    (src/Sources/SpatialSources.jl:155-257): J = n x H on the transverse E
+    components, M = -n x E on the transverse H components.  ``fields(name,
+    X, Y, Z)`` returns the complex field ``name`` ('Ex' ... 'Hz') at the
+    simulation points; n is the plane's +normal, the injection direction."""
+
+    def __init__(self, *, time_profile, fields: Callable, center=(0.0, 0.0, 0.0), size=(0.0, 0.0, 0.0),
+                 amplitude=1.0):
+        self.time_profile = time_profile
+        self.fields = fields
+        self.center = [float(v) for v in center]
+        self.size = [float(v) for v in size]
+        self.amplitude = amplitude
+        zeros = [a for a in range(3) if self.size[a] == 0.0]
+        if len(zeros) != 1:
+            raise ValueError(f"The specified volume is not a plane (size={self.size})")
+        self.normal_axis = zeros[0]
+
+    def components(self):
+        from .grid import Ex, Ey, Ez, Hx, Hy, Hz
+        trans = {0: [Ey(), Ez(), Hy(), Hz()], 1: [Ex(), Ez(), Hx(), Hz()], 2: [Ex(), Ey(), Hx(), Hy()]}
+        return trans[self.normal_axis]
+
+    def profile(self, x, y, z, component):
+        n = self.normal_axis
+        a = component.axis
+        # (n x F)_a = eps_{a n b} F_b with b the remaining axis
+        b = 3 - n - a
+        sign = 1.0 if (a, n, b) in ((0, 1, 2), (1, 2, 0), (2, 0, 1)) else -1.0
+        X, Y, Z = np.broadcast_arrays(x, y, z)
+        if component.electric:
+            return sign * np.asarray(self.fields("H" + "xyz"[b], X, Y, Z), dtype=np.complex128)
+        return -sign * np.asarray(self.fields("E" + "xyz"[b], X, Y, Z), dtype=np.complex128)
+
+
+def ModeSource(*, frequency, time_profile, mode_solver_resolution, mode_index, center, size,
+               solver_tolerance=1e-6, amplitude=1.0, geometry, target_neff=0.0):
+    """The mode_index-th mode of the plane (center, size) at ``frequency``,
+    launched along +normal as an equivalent source
+    (src/Sources/SpatialSources.jl:588-646)."""
+    from .mode import plane_modes
+    modes = plane_modes(center, size, geometry, float(frequency), mode_solver_resolution,
+                        num_modes=mode_index, target_neff=target_neff)
+    if len(modes) < mode_index:
+        raise ValueError(f"Mode solver found only {len(modes)} modes, mode index {mode_index} was requested")
+    mode = modes[mode_index - 1]
+    u, v, w = mode.axes
+
+    def fields(name, X, Y, Z):
+        P = (X, Y, Z)
+        ax = "xyz".index(name[1])
+        local = {u: "u", v: "v", w: "w"}[ax]
+        return mode.field(name[0] + local, P[u], P[v])
+
+    src = EquivalentSource(time_profile=time_profile, fields=fields, center=center, size=size,
+                           amplitude=amplitude)
+    src.mode = mode
+    return src
+
+
 class PlaneWaveSource(Source):
     """Plane-wave current sheet (src/Sources/SpatialSources.jl:412-586).
 
