@@ -252,6 +252,24 @@ int kh_flux(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2,
 int kh_mode_overlap(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
                     int32_t n1, int32_t n2, const double *mode, double dA, double *out);
 
+/* Near-to-far projection of a planar surface from its four tangential DFTs
+ * (e1, e2, h1, h2 as for kh_flux): the full dyadic Green's function (1/r,
+ * 1/r^2, 1/r^3 terms) of the equivalent currents J, M of every sample of the
+ * n1 x n2 plane (i1 along the lower tangential axis), summed at each
+ * observation point, in float64.  normal_sign is +1 or -1 (outward normal
+ * direction); base[4][3] is the position of sample (0, 0) of e1, e2, h1, h2;
+ * d1, d2 the tangential spacings (lower axis first) and dA = d1 * d2;
+ * freqs[n_freq] must be the DFTs' frequencies; obs[n_obs][3] the
+ * observation points.  out receives complex (re, im) (Ex, Ey, Ez, Hx, Hy, Hz)
+ * at index (k * 6 + j) * n_obs + o: the column-major (n_obs, 6, n_freq)
+ * array of the reference.  Synchronous.  Replaces: _compute_far_field_gpu
+ * and near2far_kernel_{x,y,z}! (src/Monitors/Near2Far.jl:103-246, :380-550),
+ * green3d! (:40-94). */
+int kh_near2far(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                double normal_sign, const double *base, double d1, double d2, double dA, int32_t n1,
+                int32_t n2, const double *freqs, int32_t n_freq, double medium_eps, double medium_mu,
+                const double *obs, int32_t n_obs, double *out);
+
 /* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
  * dft_id = -1 gives the sum of that over all monitors.
  * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */

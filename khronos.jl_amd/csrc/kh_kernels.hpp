@@ -1304,4 +1304,222 @@ __global__ __launch_bounds__(256) void sumsq_kernel(const ACC *a, int64_t n, dou
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ------------------------------------------------------------ near-to-far
+// Surface-equivalence projection of the tangential DFT fields on a plane to
+// observation points (src/Monitors/Near2Far.jl:103-246 kernels, :380-550
+// driver).  All arithmetic is FP64.
+
+struct N2FGeom {
+    double base[4][3];       // position of sample (0, 0) of e1, e2, h1, h2
+    double d1, d2, dA, ns;   // tangential spacings (lower axis first), cell area, normal sign
+    double eps, mu, Z;       // medium; Z = sqrt(mu / eps)
+};
+
+constexpr double N2F_TWO_PI = 6.283185307179586;    // Float64(2 pi)
+constexpr double N2F_FOUR_PI = 12.566370614359172;  // Float64(4 pi)
+constexpr double N2F_HALF_PI = 1.5707963267948966;  // Float64(pi) / 2
+
+// green3d! (src/Monitors/Near2Far.jl:40-94): adds to EH = (Ex, Ey, Ez, Hx, Hy,
+// Hz) as (re, im) pairs the field at o of a point current of component C
+// (0-2: Jx..Jz, 3-5: Mx..Mz) with amplitude (fr, fi) at s.  The operations
+// are the reference's, in its order (the library is built without FMA
+// contraction):
+//   1/(ikr) = (0, -1/kr), 1/(ikr)^2 = 1/(-(kr)^2)
+//   term1 = 1 - 1/(ikr) + 1/(ikr)^2, term2 = (-1 + 3/(ikr) - 3/(ikr)^2) (p.r),
+//   term3 = 1 - 1/(ikr), expfac = f (k n / (4 pi r)) exp(i (kr + pi/2)).
+// The unit direction p and r x p have zero entries whose products add exact
+// zeros; those terms are skipped.  UNIT (eps = mu = 1) skips the divisions by
+// eps, mu and Z, which are then exact identities.
+template <int C, bool UNIT>
+__device__ __forceinline__ void green3d(double (&EH)[12], double xo, double yo, double zo, double sx, double sy,
+                                        double sz, double k, double kn, const N2FGeom &g, double fr, double fi) {
+    const double rx = xo - sx, ry = yo - sy, rz = zo - sz;
+    const double r = sqrt((rx * rx + ry * ry) + rz * rz);
+    if (!(r >= 1e-20)) return;
+    const double h[3] = {rx / r, ry / r, rz / r};
+    const double kr = k * r;
+    double sn, cs;
+    sincos(kr + N2F_HALF_PI, &sn, &cs);
+    const double s = kn / (N2F_FOUR_PI * r);
+    const double ar = fr * s, ai = fi * s;
+    const double er = ar * cs - ai * sn, ei = ar * sn + ai * cs;
+    const double a1 = 1.0 / kr;
+    const double kr2 = -(kr * kr);
+    const double t1r = 1.0 + 1.0 / kr2, t1i = a1;
+    constexpr int c = C % 3;
+    const double pd = h[c];
+    const double t2r = (-1.0 - 3.0 / kr2) * pd, t2i = (3.0 * -a1) * pd;
+    double efr = er, efi = ei;
+    if (!UNIT) {
+        const double m = C < 3 ? g.eps : g.mu;
+        efr = er / m;
+        efi = ei / m;
+    }
+    constexpr int D = C < 3 ? 0 : 6;  // slots of the parallel part: E for J, H for M
+    constexpr int X = C < 3 ? 6 : 0;  // slots of the r x p part
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double wr = t2r * h[j], wi = t2i * h[j];
+        if (j == c) {
+            wr = t1r + wr;
+            wi = t1i + wi;
+        }
+        EH[D + 2 * j] += efr * wr - efi * wi;
+        EH[D + 2 * j + 1] += efr * wi + efi * wr;
+    }
+    // ef * term3 with term3 = (1, a1)
+    const double ur = efr - efi * a1, ui = efr * a1 + efi;
+    // (r x p)_{c+1} = h_{c+2}, (r x p)_{c+2} = -h_{c+1}
+    constexpr int j1 = (c + 1) % 3, j2 = (c + 2) % 3;
+    const double q1 = h[j2], q2 = -h[j1];
+    if (C < 3) {  // + ef term3 (r x p) / Z
+        double a = ur * q1, b = ui * q1, e = ur * q2, f = ui * q2;
+        if (!UNIT) {
+            a = a / g.Z;
+            b = b / g.Z;
+            e = e / g.Z;
+            f = f / g.Z;
+        }
+        EH[X + 2 * j1] += a;
+        EH[X + 2 * j1 + 1] += b;
+        EH[X + 2 * j2] += e;
+        EH[X + 2 * j2 + 1] += f;
+    } else {      // - ef term3 (r x p) Z
+        double a = -ur * q1, b = -ui * q1, e = -ur * q2, f = -ui * q2;
+        if (!UNIT) {
+            a = a * g.Z;
+            b = b * g.Z;
+            e = e * g.Z;
+            f = f * g.Z;
+        }
+        EH[X + 2 * j1] += a;
+        EH[X + 2 * j1 + 1] += b;
+        EH[X + 2 * j2] += e;
+        EH[X + 2 * j2 + 1] += f;
+    }
+}
+
+// The four equivalent currents of one surface sample, in the reference's
+// order for each normal axis (src/Monitors/Near2Far.jl:147-150, :192-195,
+// :238-241).  v[q] = (re, im) of e1, e2, h1, h2; p[q] = source positions.
+template <int AX, bool UNIT>
+__device__ __forceinline__ void n2f_sample(double (&EH)[12], double xo, double yo, double zo, const double (&v)[4][2],
+                                           const double (&p)[4][3], double k, double kn, const N2FGeom &g) {
+    const double ns = g.ns, mns = -ns, dA = g.dA;
+#define KH_N2F_CALL(CC, Q, SG) \
+    green3d<CC, UNIT>(EH, xo, yo, zo, p[Q][0], p[Q][1], p[Q][2], k, kn, g, ((SG) * v[Q][0]) * dA, ((SG) * v[Q][1]) * dA)
+    if (AX == 2) {         // J = (ns Hy, -ns Hx), M = (-ns Ey, ns Ex)
+        KH_N2F_CALL(0, 3, ns);
+        KH_N2F_CALL(1, 2, mns);
+        KH_N2F_CALL(3, 1, mns);
+        KH_N2F_CALL(4, 0, ns);
+    } else if (AX == 0) {  // Jy = ns Hz, Jz = -ns Hy, My = -ns Ez, Mz = ns Ey
+        KH_N2F_CALL(1, 3, ns);
+        KH_N2F_CALL(2, 2, mns);
+        KH_N2F_CALL(4, 1, mns);
+        KH_N2F_CALL(5, 0, ns);
+    } else {               // Jz = ns Hx, Jx = -ns Hz, Mz = -ns Ex, Mx = ns Ez
+        KH_N2F_CALL(2, 2, ns);
+        KH_N2F_CALL(0, 3, mns);
+        KH_N2F_CALL(5, 0, mns);
+        KH_N2F_CALL(3, 1, ns);
+    }
+#undef KH_N2F_CALL
+}
+
+// One thread per (observation point, frequency); the block's threads share a
+// frequency and walk the samples [chunk * blockIdx.z, + chunk) of the n1 x n2
+// plane (i1 fastest) through an LDS tile, each sample read once per block:
+// the four DFT values at that frequency, averaged over the (up to) two planes
+// across the normal in the accumulator's type (src/Monitors/Near2Far.jl:
+// 431-474), and the tangential offsets i1 d1, i2 d2.  The block's partial
+// field goes to out[((z nf + k) 6 + j) n_obs + o] (re, im).
+template <typename ACC, int AX, bool UNIT>
+__global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2, N2FGeom g,
+                                                       const double *obs, const double *freqs, int n_obs, int nf,
+                                                       int n1, int n2, int chunk, double *out) {
+    constexpr int T0 = AX == 0 ? 1 : 0, T1 = AX == 2 ? 1 : 2;
+    __shared__ double tile[256][10];
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y, z = blockIdx.z;
+    const int N = n1 * n2;
+    const int c0 = z * chunk, c1 = min(N, c0 + chunk);
+    double xo = 0.0, yo = 0.0, zo = 0.0;
+    if (o < n_obs) {
+        xo = obs[3 * o];
+        yo = obs[3 * o + 1];
+        zo = obs[3 * o + 2];
+    }
+    const double n = sqrt(g.eps * g.mu);
+    const double kk = N2F_TWO_PI * freqs[k] * n;
+    const double kn = kk * n;
+    double EH[12] = {};
+    const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
+    for (int b0 = c0; b0 < c1; b0 += blockDim.x) {
+        const int c = b0 + threadIdx.x;
+        if (c < c1) {
+            const int i1 = c % n1, i2 = c / n1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const FluxMon &M = *m[q];
+                const ACC *A = (const ACC *)M.acc;
+                const int64_t lin = (AX == 0 ? (int64_t)i1 * M.d[0] + (int64_t)i2 * M.d[0] * M.d[1]
+                                     : AX == 1 ? (int64_t)i1 + (int64_t)i2 * M.d[0] * M.d[1]
+                                               : (int64_t)i1 + (int64_t)i2 * M.d[0]);
+                const int64_t nc = (int64_t)M.d[0] * M.d[1] * M.d[2];
+                const ACC *p0 = A + 2 * (lin + nc * k);
+                ACC r = p0[0], im = p0[1];
+                if (M.d[AX] >= 2) {
+                    const int64_t step = AX == 0 ? 1 : (AX == 1 ? (int64_t)M.d[0] : (int64_t)M.d[0] * M.d[1]);
+                    r = (r + p0[2 * step]) / ACC(2);
+                    im = (im + p0[2 * step + 1]) / ACC(2);
+                }
+                tile[threadIdx.x][2 * q] = (double)r;
+                tile[threadIdx.x][2 * q + 1] = (double)im;
+            }
+            tile[threadIdx.x][8] = (double)i1 * g.d1;
+            tile[threadIdx.x][9] = (double)i2 * g.d2;
+        }
+        __syncthreads();
+        const int cnt = min((int)blockDim.x, c1 - b0);
+        if (o < n_obs) {
+            for (int s = 0; s < cnt; ++s) {
+                double v[4][2], p[4][3];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    v[q][0] = tile[s][2 * q];
+                    v[q][1] = tile[s][2 * q + 1];
+                    p[q][AX] = g.base[q][AX];
+                    p[q][T0] = g.base[q][T0] + tile[s][8];
+                    p[q][T1] = g.base[q][T1] + tile[s][9];
+                }
+                n2f_sample<AX, UNIT>(EH, xo, yo, zo, v, p, kk, kn, g);
+            }
+        }
+        __syncthreads();
+    }
+    if (o < n_obs) {
+        const int64_t base = ((int64_t)z * nf + k) * 6;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double *w = out + 2 * ((base + j) * n_obs + o);
+            w[0] = EH[2 * j];
+            w[1] = EH[2 * j + 1];
+        }
+    }
+}
+
+// Sum of the per-split partial fields in split order: out[i] = sum_z part[z m + i].
+__global__ __launch_bounds__(256) void n2f_reduce_kernel(const double *part, int splits, int64_t m, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double r = 0.0, im = 0.0;
+    for (int z = 0; z < splits; ++z) {
+        r += part[2 * ((int64_t)z * m + i)];
+        im += part[2 * ((int64_t)z * m + i) + 1];
+    }
+    out[2 * i] = r;
+    out[2 * i + 1] = im;
+}
+
 }  // namespace kh

@@ -16,6 +16,7 @@
 #include <memory>
 #include <new>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,9 @@ struct CtxBase {
     virtual void dft_norm(int id, double *out) = 0;
     virtual void mode_overlap(int e1, int e2, int h1, int h2, int ax, int n1, int n2, const double *mode,
                               double dA, double *out) = 0;
+    virtual void near2far(const int32_t *ids, int ax, double ns, const double *base, double d1, double d2,
+                          double dA, int n1, int n2, const double *freqs, int nf, double eps, double mu,
+                          const double *obs, int n_obs, double *out) = 0;
     virtual void reset() = 0;
     virtual void set_timing(int n) = 0;
     virtual void get_timing(double *out) = 0;
@@ -1427,6 +1431,117 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // growable device buffer owned by the context (the old one is freed)
+    void *regrow(void *old, size_t &cap, size_t bytes) {
+        if (bytes <= cap) return old;
+        if (old) {
+            HIPCHK(hipStreamSynchronize(st));
+            for (auto it = owned.begin(); it != owned.end(); ++it)
+                if (it->p == old) {
+                    owned.erase(it);
+                    break;
+                }
+            HIPCHK(hipFree(old));
+        }
+        cap = bytes;
+        return dalloc(bytes);
+    }
+
+    double *n2f_in = nullptr, *n2f_part = nullptr, *n2f_out = nullptr;  // near-to-far staging
+    size_t n2f_in_cap = 0, n2f_part_cap = 0, n2f_out_cap = 0;
+
+    template <int AX, bool UNIT>
+    void launch_near2far(const kh::FluxMon *fm, const kh::N2FGeom &geo, const double *obs, const double *fr,
+                         int n_obs, int nf, int n1, int n2, int chunk, int splits, int bs, double *dst) {
+        dim3 grid((n_obs + bs - 1) / bs, nf, splits);
+        if (acc64)
+            hipLaunchKernelGGL((kh::near2far_kernel<double, AX, UNIT>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2],
+                               fm[3], geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
+        else
+            hipLaunchKernelGGL((kh::near2far_kernel<T, AX, UNIT>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2], fm[3],
+                               geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
+    }
+
+    void near2far(const int32_t *ids, int ax1, double ns, const double *base, double d1, double d2, double dA,
+                  int n1, int n2, const double *freqs, int nf, double eps, double mu, const double *obs, int n_obs,
+                  double *out) override {
+        for (int q = 0; q < 4; ++q) require(ids[q] >= 0 && ids[q] < (int)dfts.size(), "bad dft id");
+        require(ax1 >= 1 && ax1 <= 3, "normal axis must be 1..3");
+        require(n_obs >= 1 && nf >= 1, "need at least one observation point and one frequency");
+        require(eps > 0 && mu > 0 && std::isfinite(eps) && std::isfinite(mu), "medium eps and mu must be > 0");
+        require(std::isfinite(d1) && std::isfinite(d2) && std::isfinite(dA) && std::isfinite(ns),
+                "near-to-far needs a 3D grid (finite spacings)");
+        for (int q = 0; q < 4; ++q) require(dfts[ids[q]].nf == nf, "near-to-far DFTs differ from n_freq");
+        const int ax = ax1 - 1;
+        const int t0 = ax == 0 ? 1 : 0, t1 = ax == 2 ? 1 : 2;
+        kh::FluxMon fm[4];
+        for (int q = 0; q < 4; ++q) {
+            Dft &m = dfts[ids[q]];
+            fm[q].acc = m.acc;
+            for (int a = 0; a < 3; ++a) fm[q].d[a] = m.d[a];
+            require(n1 >= 1 && n2 >= 1 && n1 <= m.d[t0] && n2 <= m.d[t1], "surface larger than a DFT plane");
+            require(m.d[ax] <= 2, "a near-to-far DFT spans more than two planes across the normal");
+        }
+        const int64_t N = (int64_t)n1 * n2;
+        const int64_t m_out = (int64_t)nf * 6 * n_obs;
+        require(N < (1LL << 31) && m_out < (1LL << 31), "near-to-far problem too large");
+        kh::N2FGeom geo;
+        for (int q = 0; q < 4; ++q)
+            for (int a = 0; a < 3; ++a) {
+                geo.base[q][a] = base[3 * q + a];
+                require(std::isfinite(geo.base[q][a]), "non-finite base position");
+            }
+        geo.d1 = d1;
+        geo.d2 = d2;
+        geo.dA = dA;
+        geo.ns = ns;
+        geo.eps = eps;
+        geo.mu = mu;
+        geo.Z = std::sqrt(mu / eps);
+        const bool unit = eps == 1.0 && mu == 1.0;
+        // block: one thread per observation point (64, 128 or 256 of them)
+        const int bs = n_obs <= 64 ? 64 : (n_obs <= 128 ? 128 : 256);
+        const int64_t blocks_xy = (int64_t)((n_obs + bs - 1) / bs) * nf;
+        const int64_t tiles = (N + bs - 1) / bs;
+        // split the samples until the grid holds ~8 waves per SIMD, keeping the
+        // partial fields within 512 MB
+        const int64_t want = (8192 / (bs / 64) + blocks_xy - 1) / blocks_xy;
+        const int64_t cap_mem = std::max<int64_t>(1, (512LL << 20) / (m_out * 16));
+        int splits = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(want, tiles), std::min<int64_t>(cap_mem, 65535)));
+        const int64_t tiles_per = (tiles + splits - 1) / splits;
+        splits = (int)((tiles + tiles_per - 1) / tiles_per);
+        const int chunk = (int)(tiles_per * bs);
+        const size_t in_bytes = ((size_t)3 * n_obs + nf) * sizeof(double);
+        n2f_in = (double *)regrow(n2f_in, n2f_in_cap, in_bytes);
+        HIPCHK(hipMemcpyAsync(n2f_in, obs, (size_t)3 * n_obs * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in + 3 * n_obs, freqs, (size_t)nf * sizeof(double), hipMemcpyHostToDevice, st));
+        const size_t out_bytes = (size_t)m_out * 2 * sizeof(double);
+        n2f_out = (double *)regrow(n2f_out, n2f_out_cap, out_bytes);
+        double *dst = n2f_out;
+        if (splits > 1) {
+            n2f_part = (double *)regrow(n2f_part, n2f_part_cap, out_bytes * splits);
+            dst = n2f_part;
+        }
+        const double *o_dev = n2f_in, *f_dev = n2f_in + 3 * n_obs;
+        auto go = [&](auto ax_c, auto unit_c) {
+            launch_near2far<decltype(ax_c)::value, decltype(unit_c)::value>(fm, geo, o_dev, f_dev, n_obs, nf, n1, n2,
+                                                                           chunk, splits, bs, dst);
+        };
+        using F = std::false_type;
+        using Tr = std::true_type;
+        if (ax == 0) unit ? go(std::integral_constant<int, 0>{}, Tr{}) : go(std::integral_constant<int, 0>{}, F{});
+        else if (ax == 1) unit ? go(std::integral_constant<int, 1>{}, Tr{}) : go(std::integral_constant<int, 1>{}, F{});
+        else unit ? go(std::integral_constant<int, 2>{}, Tr{}) : go(std::integral_constant<int, 2>{}, F{});
+        HIPCHK(hipGetLastError());
+        if (splits > 1) {
+            hipLaunchKernelGGL(kh::n2f_reduce_kernel, dim3((unsigned)((m_out + 255) / 256)), dim3(256), 0, st,
+                               (const double *)n2f_part, splits, m_out, n2f_out);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpyAsync(out, n2f_out, out_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
     void dft_norm(int id, double *out) override {
         require(id >= -1 && id < (int)dfts.size(), "bad dft id");
         double total = 0.0;
@@ -1743,6 +1858,19 @@ int kh_mode_overlap(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2,
         CTX_OK;
         require(mode != nullptr && out != nullptr, "null argument");
         ctx->impl->mode_overlap(e1, e2, h1, h2, normal_axis, n1, n2, mode, dA, out);
+    });
+}
+
+int kh_near2far(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                double normal_sign, const double *base, double d1, double d2, double dA, int32_t n1,
+                int32_t n2, const double *freqs, int32_t n_freq, double medium_eps, double medium_mu,
+                const double *obs, int32_t n_obs, double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(base != nullptr && freqs != nullptr && obs != nullptr && out != nullptr, "null argument");
+        const int32_t ids[4] = {e1, e2, h1, h2};
+        ctx->impl->near2far(ids, normal_axis, normal_sign, base, d1, d2, dA, n1, n2, freqs, n_freq, medium_eps,
+                            medium_mu, obs, n_obs, out);
     });
 }
 

@@ -12,6 +12,8 @@ from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, U
 from .monitors import (DFTMonitor, FluxMonitor, ModeMonitor, compute_mode_amplitudes,
                        get_mode_transmission)
 from .mode import ModeSpec
+from .near2far import (Near2FarMonitor, LayerSpec, LayerStack, compute_far_field, compute_far_field_power,
+                       compute_LEE)
 from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
 from .geometry import Material, Object, Box, Sphere, Cylinder
 from .smoothing import SubpixelSmoothing, NoSmoothing, VolumeAveraging, AnisotropicSmoothing
@@ -31,4 +33,5 @@ __all__ = [
     "Cylinder", "LorentzianSusceptibility", "DrudeSusceptibility", "compute_ade_coefficients",
     "eval_susceptibility", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
     "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",
+    "Near2FarMonitor", "LayerSpec", "LayerStack", "compute_far_field", "compute_far_field_power", "compute_LEE",
 ]

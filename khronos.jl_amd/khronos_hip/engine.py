@@ -254,6 +254,21 @@ class HipEngine:
         o = out.reshape(nf, 4)
         return np.stack([o[:, 0] + 1j * o[:, 1], o[:, 2] + 1j * o[:, 3]], axis=1)
 
+    def near2far(self, ids, normal_axis: int, normal_sign: float, base, d1: float, d2: float, dA: float,
+                 n1: int, n2: int, freqs, eps: float, mu: float, obs) -> np.ndarray:
+        """Far field (n_obs, 6, n_freq) complex of kh_near2far; ``base`` is the
+        (4, 3) position of sample (0, 0) of e1, e2, h1, h2, ``obs`` (n_obs, 3)."""
+        b = np.ascontiguousarray(base, dtype=np.float64).reshape(4, 3)
+        f = np.ascontiguousarray(freqs, dtype=np.float64).ravel()
+        o = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+        nf, n_obs = f.size, o.shape[0]
+        out = np.zeros((nf, 6, n_obs), dtype=np.complex128)
+        e1, e2, h1, h2 = (self.dft_ids[i] for i in ids)
+        _lib.check(self.lib.kh_near2far(self.ctx, e1, e2, h1, h2, int(normal_axis), float(normal_sign), _ptr(b),
+                                        float(d1), float(d2), float(dA), int(n1), int(n2), _ptr(f), nf,
+                                        float(eps), float(mu), _ptr(o), n_obs, _ptr(out)))
+        return np.transpose(out, (2, 1, 0))
+
     def dft_norms(self):
         out = []
         for did in self.dft_ids:

@@ -6,6 +6,7 @@
 * Poynting flux readout                     -- src/Monitors/FluxMonitor.jl:92-172
 * mode monitor = 4 tangential DFTs + modes  -- src/Monitors/ModeMonitor.jl:191-331
 * mode amplitudes a+- (overlap on device)   -- src/Monitors/ModeMonitor.jl:345-520
+* near-to-far surface = 4 tangential DFTs   -- src/Monitors/Near2Far.jl:1137-1200 (near2far.py)
 
 The accumulation itself (``F += dt * exp(i 2 pi f t) * field`` every D-th
 step, H at t and E at t + dt/2) runs in the HIP library.
@@ -19,6 +20,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from .grid import Ex, Ey, Ez, Hx, Hy, Hz, Field, Volume, grid_volume
+from .near2far import Near2FarMonitor, init_near2far_monitor
 
 
 @dataclass
@@ -159,6 +161,8 @@ def expand_monitors(sim, monitors):
             else:
                 m.monitor_data = FluxMonitorData(*args)
             dfts.extend(te + th)
+        elif isinstance(m, Near2FarMonitor):
+            dfts.extend(init_near2far_monitor(sim, m))
         else:
             raise TypeError(f"unsupported monitor {type(m).__name__}")
     return dfts

@@ -785,3 +785,154 @@ int orc_read_dft(const orc_sim *s, int id, double *out) {
 }
 
 int orc_real_size(void) { return (int)sizeof(real); }
+
+/* ------------------------------------------------------------------------
+ * Near-to-far projection (double precision whatever REAL is).
+ *
+ *   green3d! ..................... src/Monitors/Near2Far.jl:40-94
+ *   equivalent currents per axis . src/Monitors/Near2Far.jl:147-150, :192-195,
+ *                                  :238-241 (= CPU path :334-357)
+ *   plane slicing, normal average  src/Monitors/Near2Far.jl:409-474
+ *
+ * Complex numbers are (re, im) pairs combined with Julia's rules, in its
+ * order: z*w = (a c - b d, a d + b c); real*z and z/real act on both parts;
+ * real/z = real*inv(z), inv((0, y)) = (0, -1/y).  The unit direction p and
+ * r_hat x p are formed in full, as in the reference.
+ * ------------------------------------------------------------------------ */
+typedef struct { double re, im; } orc_cx;
+
+static inline orc_cx cx(double a, double b) { orc_cx z = {a, b}; return z; }
+static inline orc_cx cmul(orc_cx a, orc_cx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline orc_cx cscale(orc_cx a, double s) { return cx(a.re * s, a.im * s); }
+static inline orc_cx cdivr(orc_cx a, double s) { return cx(a.re / s, a.im / s); }
+static inline orc_cx cadd(orc_cx a, orc_cx b) { return cx(a.re + b.re, a.im + b.im); }
+
+#ifdef ORC_PLANT_GREEN_TERM2
+#define ORC_T2_COEF 2.0 /* planted: 2/(ikr) where the Green's function has 3/(ikr) */
+#else
+#define ORC_T2_COEF 3.0
+#endif
+
+void orc_green3d(double *EH, const double *x, double freq, double eps, double mu, const double *x0, int c0,
+                 double f0re, double f0im) {
+    const double rv[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]};
+    const double r = sqrt((rv[0] * rv[0] + rv[1] * rv[1]) + rv[2] * rv[2]);
+    if (!(r >= 1e-20)) return;
+    const double rh[3] = {rv[0] / r, rv[1] / r, rv[2] / r};
+    const double n = sqrt(eps * mu);
+    const double k = 6.283185307179586 * freq * n; /* 2 pi */
+    const double Z = sqrt(mu / eps);
+    const double kr = k * r;
+    const orc_cx inv_ikr = cx(0.0, -(1.0 / kr)); /* 1/(i k r) */
+    const double ikr2 = -(kr * kr);
+    const double ph = kr + 1.5707963267948966;    /* k r + pi/2 */
+    const orc_cx expfac = cmul(cscale(cx(f0re, f0im), k * n / (12.566370614359172 * r)), cx(cos(ph), sin(ph)));
+    const int pc = (c0 - 1) % 3;
+    const double p[3] = {pc == 0 ? 1.0 : 0.0, pc == 1 ? 1.0 : 0.0, pc == 2 ? 1.0 : 0.0};
+    const double pd = p[0] * rh[0] + p[1] * rh[1] + p[2] * rh[2];
+    const double rxp[3] = {rh[1] * p[2] - rh[2] * p[1], rh[2] * p[0] - rh[0] * p[2], rh[0] * p[1] - rh[1] * p[0]};
+    /* term1 = 1 - 1/ikr + 1/ikr2 */
+    const orc_cx term1 = cx((1.0 - inv_ikr.re) + 1.0 / ikr2, -inv_ikr.im);
+    /* term2 = (-1 + 3/ikr - 3/ikr2) * (p . r_hat) */
+    const orc_cx t3ikr = cscale(inv_ikr, ORC_T2_COEF);
+    const orc_cx term2 = cscale(cx((-1.0 + t3ikr.re) - 3.0 / ikr2, t3ikr.im), pd);
+    /* term3 = 1 - 1/ikr */
+    const orc_cx term3 = cx(1.0 - inv_ikr.re, -inv_ikr.im);
+    orc_cx add[6];
+    if (c0 <= 3) {
+        const orc_cx ef = cdivr(expfac, eps);
+        for (int j = 0; j < 3; ++j) add[j] = cmul(ef, cadd(cscale(term1, p[j]), cscale(term2, rh[j])));
+        for (int j = 0; j < 3; ++j) add[3 + j] = cdivr(cscale(cmul(ef, term3), rxp[j]), Z);
+    } else {
+        const orc_cx ef = cdivr(expfac, mu);
+        const orc_cx mef = cx(-ef.re, -ef.im);
+        for (int j = 0; j < 3; ++j) add[j] = cscale(cscale(cmul(mef, term3), rxp[j]), Z);
+        for (int j = 0; j < 3; ++j) add[3 + j] = cmul(ef, cadd(cscale(term1, p[j]), cscale(term2, rh[j])));
+    }
+    for (int j = 0; j < 6; ++j) {
+        EH[2 * j] += add[j].re;
+        EH[2 * j + 1] += add[j].im;
+    }
+}
+
+/* Value of one tangential plane at (i1, i2) for frequency kf: the sample at
+ * normal index 0, averaged with index 1 when the DFT spans two planes. */
+static orc_cx n2f_plane(const double *a, const int *d, int ax, int t0, int t1, int i1, int i2, int kf) {
+    int ii[3];
+    ii[ax] = 0;
+    ii[t0] = i1;
+    ii[t1] = i2;
+    const size_t nc = (size_t)d[0] * d[1] * d[2];
+    const size_t lin = (size_t)ii[0] + (size_t)d[0] * ((size_t)ii[1] + (size_t)d[1] * ii[2]);
+    const double *p0 = a + 2 * (lin + nc * kf);
+    orc_cx v = cx(p0[0], p0[1]);
+    if (d[ax] >= 2) {
+        const size_t st = ax == 0 ? 1 : (ax == 1 ? (size_t)d[0] : (size_t)d[0] * d[1]);
+        v = cx((v.re + p0[2 * st]) / 2, (v.im + p0[2 * st + 1]) / 2);
+    }
+    return v;
+}
+
+/* Far field out[((k * 6 + j) * n_obs + o)] (re, im) of one planar surface:
+ * planes e1, e2, h1, h2 complex (d0, d1, d2, nf) x fastest, dims[4][3];
+ * base[4][3] the position of sample (0, 0) of each; i1 runs along the lower
+ * tangential axis.  Samples are summed in the reference's order (i1
+ * fastest), one (observation point, frequency) per OpenMP iteration. */
+int orc_near2far(const double *e1, const double *e2, const double *h1, const double *h2, const int *dims, int nf,
+                 int normal_axis, double ns, const double *base, double d1, double d2, double dA, int n1, int n2,
+                 const double *freqs, double eps, double mu, const double *obs, int n_obs, double *out) {
+    if (normal_axis < 1 || normal_axis > 3 || n1 < 1 || n2 < 1 || nf < 1 || n_obs < 1) return 1;
+    const int ax = normal_axis - 1;
+    const int t0 = ax == 0 ? 1 : 0, t1 = ax == 2 ? 1 : 2;
+    const double *pl[4] = {e1, e2, h1, h2};
+    for (int q = 0; q < 4; ++q)
+        if (n1 > dims[3 * q + t0] || n2 > dims[3 * q + t1] || dims[3 * q + ax] > 2) return 2;
+    /* (plane, source component 1..6, sign) in the reference's call order */
+    int qs[4], cs[4];
+    double sg[4];
+#ifdef ORC_PLANT_N2F_MSIGN
+    const double msg = -1.0; /* planted: magnetic currents with the opposite sign */
+#else
+    const double msg = 1.0;
+#endif
+    if (ax == 2) {        /* Jx = ns Hy, Jy = -ns Hx, Mx = -ns Ey, My = ns Ex */
+        qs[0] = 3; cs[0] = 1; sg[0] = ns;
+        qs[1] = 2; cs[1] = 2; sg[1] = -ns;
+        qs[2] = 1; cs[2] = 4; sg[2] = -ns * msg;
+        qs[3] = 0; cs[3] = 5; sg[3] = ns * msg;
+    } else if (ax == 0) { /* Jy = ns Hz, Jz = -ns Hy, My = -ns Ez, Mz = ns Ey */
+        qs[0] = 3; cs[0] = 2; sg[0] = ns;
+        qs[1] = 2; cs[1] = 3; sg[1] = -ns;
+        qs[2] = 1; cs[2] = 5; sg[2] = -ns * msg;
+        qs[3] = 0; cs[3] = 6; sg[3] = ns * msg;
+    } else {              /* Jz = ns Hx, Jx = -ns Hz, Mz = -ns Ex, Mx = ns Ez */
+        qs[0] = 2; cs[0] = 3; sg[0] = ns;
+        qs[1] = 3; cs[1] = 1; sg[1] = -ns;
+        qs[2] = 0; cs[2] = 6; sg[2] = -ns * msg;
+        qs[3] = 1; cs[3] = 4; sg[3] = ns * msg;
+    }
+    const int64_t work = (int64_t)nf * n_obs;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t w = 0; w < work; ++w) {
+        const int kf = (int)(w / n_obs), o = (int)(w % n_obs);
+        double EH[12] = {0};
+        for (int i2 = 0; i2 < n2; ++i2)
+            for (int i1 = 0; i1 < n1; ++i1) {
+                for (int c = 0; c < 4; ++c) {
+                    const int q = qs[c];
+                    const orc_cx v = n2f_plane(pl[q], dims + 3 * q, ax, t0, t1, i1, i2, kf);
+                    double x0[3];
+                    x0[ax] = base[3 * q + ax];
+                    x0[t0] = base[3 * q + t0] + (double)i1 * d1;
+                    x0[t1] = base[3 * q + t1] + (double)i2 * d2;
+                    const orc_cx f0 = cscale(cscale(v, sg[c]), dA);
+                    orc_green3d(EH, obs + 3 * o, freqs[kf], eps, mu, x0, cs[c], f0.re, f0.im);
+                }
+            }
+        for (int j = 0; j < 6; ++j) {
+            out[2 * (((int64_t)kf * 6 + j) * n_obs + o)] = EH[2 * j];
+            out[2 * (((int64_t)kf * 6 + j) * n_obs + o) + 1] = EH[2 * j + 1];
+        }
+    }
+    return 0;
+}
