@@ -116,6 +116,74 @@ def serving_bench(args):
     return 0
 
 
+def near2far_bench(args):
+    """Near-to-far projection: the far field of a 256 x 256-sample z-normal
+    surface (a metasurface-scale aperture at resolution 32) at 91 x 73
+    (theta, phi) directions and 5 frequencies, from FP32 fields with float64
+    DFT accumulators; FP64 Green's-function sums on the device."""
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    import khronos_hip as kh
+    from khronos_hip import near2far as NF
+    n, res = 256, 32
+    W = n / res
+    theta = np.radians(np.linspace(0.0, 90.0, 91))
+    phi = np.radians(np.linspace(0.0, 360.0, 73))
+    freqs = [0.9, 0.95, 1.0, 1.05, 1.1]
+    mon = kh.Near2FarMonitor(center=[0.0, 0.0, 0.25], size=[W - 1.0 / res, W - 1.0 / res, 0.0], frequencies=freqs,
+                             theta=theta, phi=phi, r=1e6)
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=1.0, fwidth=0.5), component=kh.Ex(),
+                            center=[0.0, 0.0, 0.0], size=[W, W, 0.0])]
+    sim = kh.Simulation(cell_size=[W + 0.75, W + 0.75, 1.5], resolution=res, boundaries=[[0.375, 0.375]] * 3,
+                        sources=src, monitors=[mon], precision="f32")
+    sim.step(60)
+    sim.engine.sync()
+    md = mon.monitor_data
+    base, n1, n2, d1, d2, dA = NF.surface_layout(md)
+    obs = NF.observation_points(md)
+    owner, ids = NF._owner(md)
+    call = lambda: owner.engine.near2far(ids, 3, md.normal_sign, base, d1, d2, dA, n1, n2, md.frequencies,
+                                         md.medium_eps, md.medium_mu, obs)
+    for _ in range(max(1, args.warmup)):
+        call()
+    k = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        EH = call()
+    wall = (time.perf_counter() - t0) / k
+    evals = obs.shape[0] * n1 * n2 * len(freqs)
+    line = {
+        "metric": "near-to-far surface-sample evaluations/s (full dyadic Green's function, FP64)",
+        "value": round(evals / wall / 1e9, 3),
+        "unit": "G sample-evals/s",
+        "n_gpus": 1, "steps": k, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"z-normal surface {n1}x{n2} samples, {obs.shape[0]} directions (91 theta x 73 phi) "
+                               f"at r = 1e6, {len(freqs)} frequencies; a step is one kh_near2far call "
+                               "(host copies of 0.1 MB in, 1.6 MB out included)",
+                   "green3d_per_s": round(4 * evals / wall / 1e9, 3)},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import n2f_oracle as N  # the checker, used here only as the CPU baseline
+        mons = md.tangential_E_monitors + md.tangential_H_monitors
+        planes = [sim.read_dft(x.monitor_data)[..., :1] for x in mons]
+        sub = obs[::2]
+        t0 = time.perf_counter()
+        N.near2far(planes, 3, md.normal_sign, base, d1, d2, n1, n2, freqs[:1], sub, dA=dA)
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        line["cpu_baseline"] = {"value": round(sub.shape[0] * n1 * n2 / dt / 1e9, 4), "unit": "G sample-evals/s",
+                                "cores": cores, "kind": "port",
+                                "sample": f"{sub.shape[0]} directions x 1 frequency of the same surface "
+                                          f"(C oracle orc_near2far, -O3 -fopenmp, {cores} threads)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,12 +195,15 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (counter calibration)")
     ap.add_argument("--serving", action="store_true", help="serving form instead of the headline benchmark")
+    ap.add_argument("--near2far", action="store_true", help="near-to-far projection instead of the headline benchmark")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--max-active", type=int, default=6)
     ap.add_argument("--chunk", type=int, default=256)
     args = ap.parse_args(argv)
     if args.serving:
         return serving_bench(args)
+    if args.near2far:
+        return near2far_bench(args)
     if args.calibrate_copy:
         import torch
         src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")

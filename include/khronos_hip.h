@@ -270,6 +270,18 @@ int kh_near2far(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int
                 int32_t n2, const double *freqs, int32_t n_freq, double medium_eps, double medium_mu,
                 const double *obs, int32_t n_obs, double *out);
 
+/* Spectral samples of the four tangential DFT planes of a surface (as for
+ * kh_near2far) at frequency index freq_index, for the layered projection:
+ * out[8 * b + 2 * q ..] = sum_{i1 < n1, i2 < n2} F_q(i1, i2)
+ * exp(-2 pi i (m1 i1 / pad1 + m2 i2 / pad2)) as (re, im), q = e1, e2, h1, h2,
+ * for bin b = (m1, m2) = bins[2 * b], bins[2 * b + 1] (integer frequencies,
+ * any sign).  These are the bins of the zero-padded pad1 x pad2 FFT the
+ * reference interpolates.  Synchronous.  Replaces: the padded FFTs of
+ * compute_far_field_layered (src/Monitors/Near2Far.jl:805-840). */
+int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                         int32_t n1, int32_t n2, int32_t freq_index, int32_t pad1, int32_t pad2,
+                         const int32_t *bins, int32_t n_bins, double *out);
+
 /* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
  * dft_id = -1 gives the sum of that over all monitors.
  * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */

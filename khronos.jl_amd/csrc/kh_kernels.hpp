@@ -1399,47 +1399,187 @@ __device__ __forceinline__ void green3d(double (&EH)[12], double xo, double yo, 
     }
 }
 
+// sin and cos of x >= 0 for the FAST projection: x = n pi/2 + t by a
+// two-constant Cody-Waite reduction done with FMAs (each step rounds once, so
+// |t| <= pi/4 carries an absolute error of about 1e-16), then the minimax
+// kernels of fdlibm (Sun Microsystems, k_sin.c / k_cos.c, public domain) on
+// [-pi/4, pi/4], evaluated with FMAs, and the quadrant of n.  x >= 2^31 pi/2
+// (and non-finite x) go to the library sincos.
+__device__ __forceinline__ void n2f_sincos(double x, double *sn, double *cs) {
+    if (!(x < 3.0e9)) {
+        sincos(x, sn, cs);
+        return;
+    }
+    const double n = __builtin_rint(x * 0.63661977236758134308);   // 2 / pi
+    double t = __builtin_fma(-n, 1.5707963267948966, x);              // pi/2, leading double
+    t = __builtin_fma(-n, 6.123233995736766e-17, t);                  // pi/2, next double
+    const double z = t * t;
+    double ps = __builtin_fma(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08);
+    ps = __builtin_fma(z, ps, 2.75573137070700676789e-06);
+    ps = __builtin_fma(z, ps, -1.98412698298579493134e-04);
+    ps = __builtin_fma(z, ps, 8.33333333332248946124e-03);
+    ps = __builtin_fma(z, ps, -1.66666666666666324348e-01);
+    const double s = __builtin_fma(z * t, ps, t);
+    double pc = __builtin_fma(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09);
+    pc = __builtin_fma(z, pc, -2.75573143513906633035e-07);
+    pc = __builtin_fma(z, pc, 2.48015872894767294178e-05);
+    pc = __builtin_fma(z, pc, -1.38888888888741095749e-03);
+    pc = __builtin_fma(z, pc, 4.16666666666666019037e-02);
+    const double hz = 0.5 * z, w = 1.0 - hz;
+    const double c = w + __builtin_fma(z * z, pc, (1.0 - w) - hz);
+    const int q = (int)n;
+    const double s1 = (q & 1) ? c : s, c1 = (q & 1) ? s : c;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// sqrt(x) for x > 1e-200 (no denormal scaling needed) from the hardware
+// reciprocal square root by the Newton/Goldschmidt sequence whose last two
+// corrections round correctly (the compiler's own lowering of sqrt for
+// double), so r matches an IEEE sqrt; the refined half-reciprocal gives 1/r,
+// finished by one more Newton step (relative error ~1e-16).
+__device__ __forceinline__ double n2f_sqrt_rcp(double x, double *inv) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double g = x * y0, h = 0.5 * y0;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    const double q = 2.0 * h;
+    *inv = __builtin_fma(q, __builtin_fma(-g, q, 1.0), q);
+    return g;
+}
+
+// green3d! with the same terms, evaluated for speed: r, kr and the phase
+// argument kr + pi/2 are formed exactly as in green3d (at r ~ 1e6 one ulp of r
+// moves the phase by ~1e-9); the amplitudes use 1/r, powers of 1/(kr) by
+// products, the inverses of eps, mu, Z, and FMAs.  The source amplitude
+// (fr, fi) already carries ns dA and the current's sign.  Per-frequency
+// constants: k, kn4 = k n / (4 pi), ik = 1/k; ieps, imu, iZ = 1/eps, 1/mu,
+// 1/Z.
+struct N2FFast {
+    double ieps, imu, iZ;
+};
+
+template <int C, bool UNIT>
+__device__ __forceinline__ void green3d_fast(double (&EH)[12], double xo, double yo, double zo, double sx, double sy,
+                                             double sz, double k, double kn4, double ik, const N2FGeom &g,
+                                             const N2FFast &f, double fr, double fi) {
+    const double rx = xo - sx, ry = yo - sy, rz = zo - sz;
+    const double r2 = (rx * rx + ry * ry) + rz * rz;
+    if (!(r2 >= 1e-40)) return;
+    double ir;
+    double r;
+    if (r2 > 1e-200) {
+        r = n2f_sqrt_rcp(r2, &ir);
+    } else {
+        r = sqrt(r2);
+        ir = 1.0 / r;
+    }
+    const double h[3] = {rx * ir, ry * ir, rz * ir};
+    const double kr = k * r;
+    double sn, cs;
+    n2f_sincos(kr + N2F_HALF_PI, &sn, &cs);
+    const double s = kn4 * ir;
+    const double ar = fr * s, ai = fi * s;
+    double er = __builtin_fma(ar, cs, -(ai * sn)), ei = __builtin_fma(ar, sn, ai * cs);
+    if (!UNIT) {
+        const double m = C < 3 ? f.ieps : f.imu;
+        er *= m;
+        ei *= m;
+    }
+    const double a1 = ir * ik;       // 1/(kr)
+    const double b = a1 * a1;        // 1/(kr)^2
+    constexpr int c = C % 3;
+    const double pd = h[c];
+    const double t2r = __builtin_fma(3.0, b, -1.0) * pd, t2i = (-3.0 * a1) * pd;
+    constexpr int D = C < 3 ? 0 : 6;
+    constexpr int X = C < 3 ? 6 : 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double wr, wi;
+        if (j == c) {
+            wr = __builtin_fma(t2r, h[j], 1.0 - b);
+            wi = __builtin_fma(t2i, h[j], a1);
+        } else {
+            wr = t2r * h[j];
+            wi = t2i * h[j];
+        }
+        EH[D + 2 * j] = __builtin_fma(er, wr, __builtin_fma(-ei, wi, EH[D + 2 * j]));
+        EH[D + 2 * j + 1] = __builtin_fma(er, wi, __builtin_fma(ei, wr, EH[D + 2 * j + 1]));
+    }
+    double ur = __builtin_fma(-ei, a1, er), ui = __builtin_fma(er, a1, ei);
+    if (!UNIT) {
+        const double zf = C < 3 ? f.iZ : -g.Z;
+        ur *= zf;
+        ui *= zf;
+    } else if (C >= 3) {
+        ur = -ur;
+        ui = -ui;
+    }
+    constexpr int j1 = (c + 1) % 3, j2 = (c + 2) % 3;
+    EH[X + 2 * j1] = __builtin_fma(ur, h[j2], EH[X + 2 * j1]);
+    EH[X + 2 * j1 + 1] = __builtin_fma(ui, h[j2], EH[X + 2 * j1 + 1]);
+    EH[X + 2 * j2] = __builtin_fma(-ur, h[j1], EH[X + 2 * j2]);
+    EH[X + 2 * j2 + 1] = __builtin_fma(-ui, h[j1], EH[X + 2 * j2 + 1]);
+}
+
 // The four equivalent currents of one surface sample, in the reference's
 // order for each normal axis (src/Monitors/Near2Far.jl:147-150, :192-195,
 // :238-241).  v[q] = (re, im) of e1, e2, h1, h2; p[q] = source positions.
-template <int AX, bool UNIT>
+template <int AX, bool UNIT, bool FAST>
 __device__ __forceinline__ void n2f_sample(double (&EH)[12], double xo, double yo, double zo, const double (&v)[4][2],
-                                           const double (&p)[4][3], double k, double kn, const N2FGeom &g) {
+                                           const double (&p)[4][3], double k, double kn, double kn4, double ik,
+                                           const N2FGeom &g, const N2FFast &f) {
     const double ns = g.ns, mns = -ns, dA = g.dA;
-#define KH_N2F_CALL(CC, Q, SG) \
-    green3d<CC, UNIT>(EH, xo, yo, zo, p[Q][0], p[Q][1], p[Q][2], k, kn, g, ((SG) * v[Q][0]) * dA, ((SG) * v[Q][1]) * dA)
+    // SGN = +1: amplitude ns v dA, -1: -ns v dA.  FAST tiles hold v ns dA already.
+#define KH_N2F_CALL(CC, Q, SGN)                                                                                   \
+    if (FAST)                                                                                                     \
+        green3d_fast<CC, UNIT>(EH, xo, yo, zo, p[Q][0], p[Q][1], p[Q][2], k, kn4, ik, g, f,                       \
+                               (SGN) > 0 ? v[Q][0] : -v[Q][0], (SGN) > 0 ? v[Q][1] : -v[Q][1]);                   \
+    else                                                                                                          \
+        green3d<CC, UNIT>(EH, xo, yo, zo, p[Q][0], p[Q][1], p[Q][2], k, kn, g,                                    \
+                          (((SGN) > 0 ? ns : mns) * v[Q][0]) * dA, (((SGN) > 0 ? ns : mns) * v[Q][1]) * dA)
     if (AX == 2) {         // J = (ns Hy, -ns Hx), M = (-ns Ey, ns Ex)
-        KH_N2F_CALL(0, 3, ns);
-        KH_N2F_CALL(1, 2, mns);
-        KH_N2F_CALL(3, 1, mns);
-        KH_N2F_CALL(4, 0, ns);
+        KH_N2F_CALL(0, 3, 1);
+        KH_N2F_CALL(1, 2, -1);
+        KH_N2F_CALL(3, 1, -1);
+        KH_N2F_CALL(4, 0, 1);
     } else if (AX == 0) {  // Jy = ns Hz, Jz = -ns Hy, My = -ns Ez, Mz = ns Ey
-        KH_N2F_CALL(1, 3, ns);
-        KH_N2F_CALL(2, 2, mns);
-        KH_N2F_CALL(4, 1, mns);
-        KH_N2F_CALL(5, 0, ns);
+        KH_N2F_CALL(1, 3, 1);
+        KH_N2F_CALL(2, 2, -1);
+        KH_N2F_CALL(4, 1, -1);
+        KH_N2F_CALL(5, 0, 1);
     } else {               // Jz = ns Hx, Jx = -ns Hz, Mz = -ns Ex, Mx = ns Ez
-        KH_N2F_CALL(2, 2, ns);
-        KH_N2F_CALL(0, 3, mns);
-        KH_N2F_CALL(5, 0, mns);
-        KH_N2F_CALL(3, 1, ns);
+        KH_N2F_CALL(2, 2, 1);
+        KH_N2F_CALL(0, 3, -1);
+        KH_N2F_CALL(5, 0, -1);
+        KH_N2F_CALL(3, 1, 1);
     }
 #undef KH_N2F_CALL
 }
 
-// One thread per (observation point, frequency); the block's threads share a
-// frequency and walk the samples [chunk * blockIdx.z, + chunk) of the n1 x n2
-// plane (i1 fastest) through an LDS tile, each sample read once per block:
-// the four DFT values at that frequency, averaged over the (up to) two planes
-// across the normal in the accumulator's type (src/Monitors/Near2Far.jl:
-// 431-474), and the tangential offsets i1 d1, i2 d2.  The block's partial
-// field goes to out[((z nf + k) 6 + j) n_obs + o] (re, im).
-template <typename ACC, int AX, bool UNIT>
-__global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2, N2FGeom g,
-                                                       const double *obs, const double *freqs, int n_obs, int nf,
-                                                       int n1, int n2, int chunk, double *out) {
+// One thread per (observation point, frequency); FAST selects green3d_fast.
+// The block's threads share a frequency and walk the samples
+// [chunk * blockIdx.z, + chunk) of the n1 x n2 plane (i1 fastest) through an
+// LDS tile, each sample read once per block: the four DFT values at that
+// frequency, averaged over the (up to) two planes across the normal in the
+// accumulator's type (src/Monitors/Near2Far.jl:431-474) -- scaled by ns dA
+// for FAST -- and the positions base_q + i1 d1 + i2 d2 of the four currents.
+// The geometry sits in LDS too, so the sample loop keeps few scalars live.
+// The block's partial field goes to out[((z nf + k) 6 + j) n_obs + o] (re, im).
+template <typename ACC, int AX, bool UNIT, bool FAST>
+__global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2,
+                                                       const N2FGeom *gp, const double *obs, const double *freqs,
+                                                       int n_obs, int nf, int n1, int n2, int chunk, double *out) {
     constexpr int T0 = AX == 0 ? 1 : 0, T1 = AX == 2 ? 1 : 2;
-    __shared__ double tile[256][10];
+    __shared__ double tile[256][20];
+    __shared__ N2FGeom sg;
+    if (threadIdx.x == 0) sg = *gp;
+    __syncthreads();
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y, z = blockIdx.z;
     const int N = n1 * n2;
@@ -1450,15 +1590,20 @@ __global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, F
         yo = obs[3 * o + 1];
         zo = obs[3 * o + 2];
     }
+    const N2FGeom g = sg;
     const double n = sqrt(g.eps * g.mu);
     const double kk = N2F_TWO_PI * freqs[k] * n;
     const double kn = kk * n;
+    const double kn4 = kn * (1.0 / N2F_FOUR_PI), ik = 1.0 / kk;
+    const N2FFast ff = {1.0 / g.eps, 1.0 / g.mu, 1.0 / g.Z};
     double EH[12] = {};
     const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
     for (int b0 = c0; b0 < c1; b0 += blockDim.x) {
         const int c = b0 + threadIdx.x;
         if (c < c1) {
             const int i1 = c % n1, i2 = c / n1;
+            const double sc = FAST ? sg.ns * sg.dA : 1.0;
+            const double o1 = (double)i1 * sg.d1, o2 = (double)i2 * sg.d2;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const FluxMon &M = *m[q];
@@ -1474,11 +1619,12 @@ __global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, F
                     r = (r + p0[2 * step]) / ACC(2);
                     im = (im + p0[2 * step + 1]) / ACC(2);
                 }
-                tile[threadIdx.x][2 * q] = (double)r;
-                tile[threadIdx.x][2 * q + 1] = (double)im;
+                tile[threadIdx.x][2 * q] = FAST ? (double)r * sc : (double)r;
+                tile[threadIdx.x][2 * q + 1] = FAST ? (double)im * sc : (double)im;
+                tile[threadIdx.x][8 + 3 * q + AX] = sg.base[q][AX];
+                tile[threadIdx.x][8 + 3 * q + T0] = sg.base[q][T0] + o1;
+                tile[threadIdx.x][8 + 3 * q + T1] = sg.base[q][T1] + o2;
             }
-            tile[threadIdx.x][8] = (double)i1 * g.d1;
-            tile[threadIdx.x][9] = (double)i2 * g.d2;
         }
         __syncthreads();
         const int cnt = min((int)blockDim.x, c1 - b0);
@@ -1489,11 +1635,11 @@ __global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, F
                 for (int q = 0; q < 4; ++q) {
                     v[q][0] = tile[s][2 * q];
                     v[q][1] = tile[s][2 * q + 1];
-                    p[q][AX] = g.base[q][AX];
-                    p[q][T0] = g.base[q][T0] + tile[s][8];
-                    p[q][T1] = g.base[q][T1] + tile[s][9];
+                    p[q][0] = tile[s][8 + 3 * q];
+                    p[q][1] = tile[s][9 + 3 * q];
+                    p[q][2] = tile[s][10 + 3 * q];
                 }
-                n2f_sample<AX, UNIT>(EH, xo, yo, zo, v, p, kk, kn, g);
+                n2f_sample<AX, UNIT, FAST>(EH, xo, yo, zo, v, p, kk, kn, kn4, ik, g, ff);
             }
         }
         __syncthreads();
@@ -1520,6 +1666,57 @@ __global__ __launch_bounds__(256) void n2f_reduce_kernel(const double *part, int
     }
     out[2 * i] = r;
     out[2 * i + 1] = im;
+}
+
+// Spectral samples of the four tangential planes for the layered projection
+// (src/Monitors/Near2Far.jl:805-840): the reference zero-pads each plane to
+// p1 x p2 (4x), FFTs it and interpolates the sorted spectrum bilinearly.  Only
+// the bins next to the observation directions are read, so they are summed
+// directly: out[b][q] = sum_{i1 < n1, i2 < n2} F_q(i1, i2)
+// exp(-2 pi i (m1 i1 / p1 + m2 i2 / p2)) for bin b = (m1, m2), F_q the plane
+// averaged across the normal in the accumulator's type.  One block per bin.
+template <typename ACC, int AX>
+__global__ __launch_bounds__(256) void n2f_spectrum_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2,
+                                                           int n1, int n2, int k, int p1, int p2, const int *bins,
+                                                           double *out) {
+    const int b = blockIdx.x;
+    const int m1 = bins[2 * b], m2 = bins[2 * b + 1];
+    const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
+    double acc[8] = {};
+    const int N = n1 * n2;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        const int i1 = c % n1, i2 = c / n1;
+        // exact integer phase reduction: (m i) mod p in [0, p)
+        int64_t r1 = ((int64_t)m1 * i1) % p1, r2 = ((int64_t)m2 * i2) % p2;
+        if (r1 < 0) r1 += p1;
+        if (r2 < 0) r2 += p2;
+        double sn, cs;
+        sincospi(-2.0 * ((double)r1 / p1 + (double)r2 / p2), &sn, &cs);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const FluxMon &M = *m[q];
+            const ACC *A = (const ACC *)M.acc;
+            const int64_t lin = (AX == 0 ? (int64_t)i1 * M.d[0] + (int64_t)i2 * M.d[0] * M.d[1]
+                                 : AX == 1 ? (int64_t)i1 + (int64_t)i2 * M.d[0] * M.d[1]
+                                           : (int64_t)i1 + (int64_t)i2 * M.d[0]);
+            const int64_t nc = (int64_t)M.d[0] * M.d[1] * M.d[2];
+            const ACC *p0 = A + 2 * (lin + nc * k);
+            ACC r = p0[0], im = p0[1];
+            if (M.d[AX] >= 2) {
+                const int64_t step = AX == 0 ? 1 : (AX == 1 ? (int64_t)M.d[0] : (int64_t)M.d[0] * M.d[1]);
+                r = (r + p0[2 * step]) / ACC(2);
+                im = (im + p0[2 * step + 1]) / ACC(2);
+            }
+            const double vr = (double)r, vi = (double)im;
+            acc[2 * q] += vr * cs - vi * sn;
+            acc[2 * q + 1] += vr * sn + vi * cs;
+        }
+    }
+    for (int q = 0; q < 8; ++q) {
+        const double v = block_sum(acc[q]);
+        if (threadIdx.x == 0) out[8 * b + q] = v;
+        __syncthreads();   // block_sum's LDS partials are reused by the next sum
+    }
 }
 
 }  // namespace kh

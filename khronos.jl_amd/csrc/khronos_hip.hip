@@ -128,6 +128,8 @@ struct CtxBase {
     virtual void near2far(const int32_t *ids, int ax, double ns, const double *base, double d1, double d2,
                           double dA, int n1, int n2, const double *freqs, int nf, double eps, double mu,
                           const double *obs, int n_obs, double *out) = 0;
+    virtual void near2far_spectrum(const int32_t *ids, int ax, int n1, int n2, int k, int p1, int p2,
+                                   const int32_t *bins, int n_bins, double *out) = 0;
     virtual void reset() = 0;
     virtual void set_timing(int n) = 0;
     virtual void get_timing(double *out) = 0;
@@ -1450,15 +1452,15 @@ struct Ctx final : CtxBase {
     double *n2f_in = nullptr, *n2f_part = nullptr, *n2f_out = nullptr;  // near-to-far staging
     size_t n2f_in_cap = 0, n2f_part_cap = 0, n2f_out_cap = 0;
 
-    template <int AX, bool UNIT>
-    void launch_near2far(const kh::FluxMon *fm, const kh::N2FGeom &geo, const double *obs, const double *fr,
+    template <int AX, bool UNIT, bool FAST>
+    void launch_near2far(const kh::FluxMon *fm, const kh::N2FGeom *geo, const double *obs, const double *fr,
                          int n_obs, int nf, int n1, int n2, int chunk, int splits, int bs, double *dst) {
         dim3 grid((n_obs + bs - 1) / bs, nf, splits);
         if (acc64)
-            hipLaunchKernelGGL((kh::near2far_kernel<double, AX, UNIT>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2],
+            hipLaunchKernelGGL((kh::near2far_kernel<double, AX, UNIT, FAST>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2],
                                fm[3], geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
         else
-            hipLaunchKernelGGL((kh::near2far_kernel<T, AX, UNIT>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2], fm[3],
+            hipLaunchKernelGGL((kh::near2far_kernel<T, AX, UNIT, FAST>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2], fm[3],
                                geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
     }
 
@@ -1511,10 +1513,14 @@ struct Ctx final : CtxBase {
         const int64_t tiles_per = (tiles + splits - 1) / splits;
         splits = (int)((tiles + tiles_per - 1) / tiles_per);
         const int chunk = (int)(tiles_per * bs);
-        const size_t in_bytes = ((size_t)3 * n_obs + nf) * sizeof(double);
+        // staging: geometry (padded to 256 B), observation points, frequencies
+        constexpr size_t geo_slots = (sizeof(kh::N2FGeom) + 255) / 256 * 256 / sizeof(double);
+        const size_t in_bytes = (geo_slots + (size_t)3 * n_obs + nf) * sizeof(double);
         n2f_in = (double *)regrow(n2f_in, n2f_in_cap, in_bytes);
-        HIPCHK(hipMemcpyAsync(n2f_in, obs, (size_t)3 * n_obs * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(n2f_in + 3 * n_obs, freqs, (size_t)nf * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in, &geo, sizeof(geo), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in + geo_slots, obs, (size_t)3 * n_obs * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in + geo_slots + 3 * n_obs, freqs, (size_t)nf * sizeof(double),
+                              hipMemcpyHostToDevice, st));
         const size_t out_bytes = (size_t)m_out * 2 * sizeof(double);
         n2f_out = (double *)regrow(n2f_out, n2f_out_cap, out_bytes);
         double *dst = n2f_out;
@@ -1522,10 +1528,18 @@ struct Ctx final : CtxBase {
             n2f_part = (double *)regrow(n2f_part, n2f_part_cap, out_bytes * splits);
             dst = n2f_part;
         }
-        const double *o_dev = n2f_in, *f_dev = n2f_in + 3 * n_obs;
+        const double *o_dev = n2f_in + geo_slots, *f_dev = n2f_in + geo_slots + 3 * n_obs;
+        const kh::N2FGeom *g_dev = (const kh::N2FGeom *)n2f_in;
+        // EXACT contexts evaluate green3d! in the reference's operation order;
+        // FAST ones use green3d_fast (same phase argument, amplitudes to ~1 ulp)
+        const bool fast = mode == KH_MODE_FAST;
         auto go = [&](auto ax_c, auto unit_c) {
-            launch_near2far<decltype(ax_c)::value, decltype(unit_c)::value>(fm, geo, o_dev, f_dev, n_obs, nf, n1, n2,
-                                                                           chunk, splits, bs, dst);
+            constexpr int A = decltype(ax_c)::value;
+            constexpr bool U = decltype(unit_c)::value;
+            if (fast)
+                launch_near2far<A, U, true>(fm, g_dev, o_dev, f_dev, n_obs, nf, n1, n2, chunk, splits, bs, dst);
+            else
+                launch_near2far<A, U, false>(fm, g_dev, o_dev, f_dev, n_obs, nf, n1, n2, chunk, splits, bs, dst);
         };
         using F = std::false_type;
         using Tr = std::true_type;
@@ -1539,6 +1553,48 @@ struct Ctx final : CtxBase {
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(out, n2f_out, out_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    int32_t *n2f_bins = nullptr;
+    double *n2f_spec = nullptr;
+    size_t n2f_bins_cap = 0, n2f_spec_cap = 0;
+
+    void near2far_spectrum(const int32_t *ids, int ax1, int n1, int n2, int k, int p1, int p2, const int32_t *bins,
+                           int n_bins, double *out) override {
+        for (int q = 0; q < 4; ++q) require(ids[q] >= 0 && ids[q] < (int)dfts.size(), "bad dft id");
+        require(ax1 >= 1 && ax1 <= 3, "normal axis must be 1..3");
+        require(n_bins >= 1, "need at least one bin");
+        require(p1 >= n1 && p2 >= n2 && p1 >= 1 && p2 >= 1, "padded size smaller than the plane");
+        const int ax = ax1 - 1;
+        const int t0 = ax == 0 ? 1 : 0, t1 = ax == 2 ? 1 : 2;
+        kh::FluxMon fm[4];
+        for (int q = 0; q < 4; ++q) {
+            Dft &m = dfts[ids[q]];
+            require(k >= 0 && k < m.nf, "frequency index out of range");
+            fm[q].acc = m.acc;
+            for (int a = 0; a < 3; ++a) fm[q].d[a] = m.d[a];
+            require(n1 >= 1 && n2 >= 1 && n1 <= m.d[t0] && n2 <= m.d[t1], "surface larger than a DFT plane");
+            require(m.d[ax] <= 2, "a near-to-far DFT spans more than two planes across the normal");
+        }
+        require((int64_t)n1 * n2 < (1LL << 31) && n_bins < (1 << 30), "spectrum problem too large");
+        n2f_bins = (int32_t *)regrow(n2f_bins, n2f_bins_cap, (size_t)n_bins * 2 * sizeof(int32_t));
+        n2f_spec = (double *)regrow(n2f_spec, n2f_spec_cap, (size_t)n_bins * 8 * sizeof(double));
+        HIPCHK(hipMemcpyAsync(n2f_bins, bins, (size_t)n_bins * 2 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        auto go = [&](auto ax_c) {
+            constexpr int A = decltype(ax_c)::value;
+            if (acc64)
+                hipLaunchKernelGGL((kh::n2f_spectrum_kernel<double, A>), dim3(n_bins), dim3(256), 0, st, fm[0],
+                                   fm[1], fm[2], fm[3], n1, n2, k, p1, p2, (const int *)n2f_bins, n2f_spec);
+            else
+                hipLaunchKernelGGL((kh::n2f_spectrum_kernel<T, A>), dim3(n_bins), dim3(256), 0, st, fm[0], fm[1],
+                                   fm[2], fm[3], n1, n2, k, p1, p2, (const int *)n2f_bins, n2f_spec);
+        };
+        if (ax == 0) go(std::integral_constant<int, 0>{});
+        else if (ax == 1) go(std::integral_constant<int, 1>{});
+        else go(std::integral_constant<int, 2>{});
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, n2f_spec, (size_t)n_bins * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1871,6 +1927,17 @@ int kh_near2far(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int
         const int32_t ids[4] = {e1, e2, h1, h2};
         ctx->impl->near2far(ids, normal_axis, normal_sign, base, d1, d2, dA, n1, n2, freqs, n_freq, medium_eps,
                             medium_mu, obs, n_obs, out);
+    });
+}
+
+int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_t h2, int32_t normal_axis,
+                         int32_t n1, int32_t n2, int32_t freq_index, int32_t pad1, int32_t pad2,
+                         const int32_t *bins, int32_t n_bins, double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(bins != nullptr && out != nullptr, "null argument");
+        const int32_t ids[4] = {e1, e2, h1, h2};
+        ctx->impl->near2far_spectrum(ids, normal_axis, n1, n2, freq_index, pad1, pad2, bins, n_bins, out);
     });
 }
 

@@ -49,6 +49,9 @@ SIGNATURES = {
     "kh_near2far": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p,
                               C.c_int32, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p]),
+    "kh_near2far_spectrum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                       C.c_int32, C.c_void_p]),
     "kh_set_chi3": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kh_add_pole": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "kh_read_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),

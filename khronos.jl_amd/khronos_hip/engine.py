@@ -269,6 +269,17 @@ class HipEngine:
                                         float(eps), float(mu), _ptr(o), n_obs, _ptr(out)))
         return np.transpose(out, (2, 1, 0))
 
+    def near2far_spectrum(self, ids, normal_axis: int, n1: int, n2: int, k: int, pad1: int, pad2: int,
+                          bins) -> np.ndarray:
+        """(n_bins, 4) complex spectral samples of kh_near2far_spectrum; bins
+        (n_bins, 2) integer frequencies."""
+        b = np.ascontiguousarray(bins, dtype=np.int32).reshape(-1, 2)
+        out = np.zeros(b.shape[0] * 8, dtype=np.float64)
+        e1, e2, h1, h2 = (self.dft_ids[i] for i in ids)
+        _lib.check(self.lib.kh_near2far_spectrum(self.ctx, e1, e2, h1, h2, int(normal_axis), int(n1), int(n2),
+                                                 int(k), int(pad1), int(pad2), _ptr(b), b.shape[0], _ptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(-1, 4)
+
     def dft_norms(self):
         out = []
         for did in self.dft_ids:

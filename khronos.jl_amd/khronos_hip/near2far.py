@@ -226,5 +226,174 @@ def compute_LEE(power, theta, phi, cone_half_angle: float = math.radians(90.0)) 
     return cone / total if total > 0 else 0.0
 
 
-def compute_far_field_layered(md, layer_stack):
-    raise NotImplementedError("layered near-to-far projection")
+def _kz(eps, mu, w, kt2):
+    """kz with Im >= 0 (src/Monitors/Near2Far.jl:588-592), per direction."""
+    kz = np.sqrt((eps * mu * w ** 2 - kt2).astype(np.complex128))
+    return np.where(kz.imag < 0, -kz, kz)
+
+
+def _admittance(pol, eps, mu, w, kt2):
+    kz = _kz(eps, mu, w, kt2)
+    if pol == 0:                                   # TE: kz / (w mu)
+        return kz / (w * mu)
+    with np.errstate(divide="ignore", invalid="ignore"):   # TM: w eps / kz, inf at grazing
+        return np.where(np.abs(kz) < 1e-30, complex(math.inf, 0.0), (w * eps) / kz)
+
+
+def layer_transmission(layers, z_monitor, freq, kx, ky):
+    """Fresnel transmission (t_TE, t_TM) of tangential E from the monitor's
+    layer to the top layer by 2x2 transfer matrices in the (A+, A-) basis,
+    vectorised over directions (src/Monitors/Near2Far.jl:565-669).  layers:
+    [(z_min, z_max, eps, mu)] bottom to top."""
+    kx = np.atleast_1d(np.asarray(kx, dtype=np.float64))
+    ky = np.atleast_1d(np.asarray(ky, dtype=np.float64))
+    n = len(layers)
+    w = 2 * math.pi * freq
+    kt2 = kx ** 2 + ky ** 2
+    mon = next((i for i, L in enumerate(layers) if L[0] <= z_monitor <= L[1]), None)
+    if mon is None:
+        raise ValueError(f"Monitor z={z_monitor} is not inside any layer in the stack")
+    one = np.ones(kx.shape, dtype=np.complex128)
+    if mon == n - 1:
+        return one, one.copy()
+    res = []
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        for pol in (0, 1):
+            T = np.zeros(kx.shape + (2, 2), dtype=np.complex128)
+            T[..., 0, 0] = 1.0
+            T[..., 1, 1] = 1.0
+            alive = np.ones(kx.shape, dtype=bool)
+            for i in range(mon, n - 1):
+                z0, z1, e, m = layers[i]
+                kz_i = _kz(e, m, w, kt2)
+                p_i = _admittance(pol, e, m, w, kt2)
+                d = (z1 - z_monitor) if i == mon else (z1 - z0)
+                if d > 0:
+                    P = np.zeros_like(T)
+                    P[..., 0, 0] = np.exp(1j * kz_i * d)
+                    P[..., 1, 1] = np.exp(-1j * kz_i * d)
+                    T = T @ P
+                L1 = layers[i + 1]
+                p_n = _admittance(pol, L1[2], L1[3], w, kt2)
+                alive &= ~((np.abs(p_n) < 1e-30) | np.isinf(np.abs(p_n)) | np.isinf(np.abs(p_i)))
+                a = (p_i + p_n) / (2 * p_n)
+                b = (p_i - p_n) / (2 * p_n)
+                D = np.empty_like(T)
+                D[..., 0, 0] = a
+                D[..., 0, 1] = b
+                D[..., 1, 0] = b
+                D[..., 1, 1] = a
+                T = T @ D
+            p_mon = _admittance(pol, layers[mon][2], layers[mon][3], w, kt2)
+            p_top = _admittance(pol, layers[-1][2], layers[-1][3], w, kt2)
+            t22 = T[..., 1, 1]
+            bad = np.isinf(np.abs(p_top)) | (np.abs(p_top) < 1e-30) | (np.abs(t22) < 1e-30)
+            res.append(np.where(alive & ~bad, (p_mon / p_top) / t22, 0j))
+    return res[0], res[1]
+
+
+def compute_far_field_layered(md, layer_stack, spectrum=None) -> np.ndarray:
+    """Far field (n_obs, 6, n_freq) of a z-normal surface inside a planar
+    layer stack, seen from the top layer (src/Monitors/Near2Far.jl:723-985):
+    spectra of the four tangential fields at the zero-padded (4x) FFT bins
+    around each direction (k_t = k_top sin(theta) (cos phi, sin phi)),
+    bilinearly interpolated, corrected for the in-plane Yee offsets, split
+    into TE/TM upward amplitudes (E and H half a cell apart along z), carried
+    through the stack by transfer matrices and projected by stationary
+    phase.  As in the reference, the phase is that of an aperture whose first
+    sample sits at the origin (the exp(i k.x0) of the surface position is not
+    applied).  The spectral sums run on the device (kh_near2far_spectrum);
+    ``spectrum(k, pad1, pad2, bins) -> (n_bins, 4)`` replaces them when
+    given."""
+    md = md.monitor_data if isinstance(md, Near2FarMonitor) else md
+    if md.normal_axis != 3:
+        raise ValueError("Layered far-field projection currently only supports z-normal monitors (normal_axis=3)")
+    if md.theta is None or md.phi is None:
+        raise ValueError("Layered far-field projection requires theta/phi angular specification")
+    base, n1, n2, d1, d2, dA = surface_layout(md)
+    if spectrum is None:
+        owner, ids = _owner(md)
+        spectrum = lambda k, p1, p2, bins: owner.engine.near2far_spectrum(ids, 3, n1, n2, k, p1, p2, bins)
+    layers = [(float(L.z_min), float(L.z_max), float(L.eps), float(L.mu)) for L in layer_stack.layers]
+    e1b, e2b, h1b, h2b = base
+    z_mon = (e1b[2] + e2b[2] + h1b[2] + h2b[2]) / 4.0
+    mon = next((i for i, L in enumerate(layers) if L[0] <= z_mon <= L[1]), None)
+    if mon is None:
+        raise ValueError(f"Monitor z={z_mon} is not inside any layer")
+    eps_m, mu_m = layers[mon][2], layers[mon][3]
+    eps_t, mu_t = layers[-1][2], layers[-1][3]
+    Z_top = math.sqrt(mu_t / eps_t)
+    dz = (e1b[2] + e2b[2]) / 2.0 - (h1b[2] + h2b[2]) / 2.0
+    dx, dy = md.dx, md.dy
+    ns = md.normal_sign
+    th = np.asarray(md.theta, dtype=np.float64)
+    ph = np.asarray(md.phi, dtype=np.float64)
+    PH, TH = np.meshgrid(ph, th, indexing="ij")           # phi outer, theta fastest
+    TH, PH = TH.ravel(), PH.ravel()
+    st, ct, sp, cp = np.sin(TH), np.cos(TH), np.sin(PH), np.cos(PH)
+    out = np.zeros((TH.size, 6, len(md.frequencies)), dtype=np.complex128)
+    p1, p2 = 4 * n1, 4 * n2
+
+    def grid(p, d):
+        # sorted fftfreq(p, 2 pi / d): m * ((2 pi / d) / p), m = -p/2 .. p/2 - 1
+        mult = (2 * math.pi / d) / p
+        g0 = (-(p // 2)) * mult
+        return g0, ((-(p // 2) + 1) * mult) - g0
+
+    gx0, dkx = grid(p1, dx)
+    gy0, dky = grid(p2, dy)
+    for k, freq in enumerate(md.frequencies):
+        w = 2 * math.pi * freq
+        k_top = w * math.sqrt(eps_t * mu_t)
+        kx = k_top * st * cp
+        ky = k_top * st * sp
+        kt2 = kx ** 2 + ky ** 2
+        fx = (kx - gx0) / dkx + 1.0
+        fy = (ky - gy0) / dky + 1.0
+        ix = np.floor(fx).astype(np.int64)
+        iy = np.floor(fy).astype(np.int64)
+        ok = (ix >= 1) & (ix < p1) & (iy >= 1) & (iy < p2)
+        wx, wy = fx - ix, fy - iy
+        vals = np.zeros((TH.size, 4), dtype=np.complex128)
+        if ok.any():
+            cx = np.stack([ix[ok], ix[ok] + 1, ix[ok], ix[ok] + 1], axis=1) - 1 - p1 // 2
+            cy = np.stack([iy[ok], iy[ok], iy[ok] + 1, iy[ok] + 1], axis=1) - 1 - p2 // 2
+            pairs = np.stack([cx.ravel(), cy.ravel()], axis=1)
+            uniq, inv = np.unique(pairs, axis=0, return_inverse=True)
+            S = spectrum(k, p1, p2, uniq) * (dx * dy)
+            C = S[inv.ravel()].reshape(-1, 4, 4)              # (obs, corner, field)
+            a, b = wx[ok][:, None], wy[ok][:, None]
+            vals[ok] = ((1 - a) * (1 - b) * C[:, 0] + a * (1 - b) * C[:, 1]
+                        + (1 - a) * b * C[:, 2] + a * b * C[:, 3])
+        ex = vals[:, 0] * np.exp(-1j * kx * (e1b[0] - e2b[0]))
+        ey = vals[:, 1] * np.exp(-1j * ky * (e2b[1] - e1b[1]))
+        hx = vals[:, 2] * np.exp(-1j * ky * (h1b[1] - h2b[1]))
+        hy = vals[:, 3] * np.exp(-1j * kx * (h2b[0] - h1b[0]))
+        e_te = -sp * ex + cp * ey
+        e_tm = cp * ex + sp * ey
+        h_p = cp * hx + sp * hy
+        h_s = -sp * hx + cp * hy
+        with np.errstate(divide="ignore", invalid="ignore"):
+            kz = _kz(eps_m, mu_m, w, kt2)
+            p_te = kz / (w * mu_m)
+            p_tm = np.where(np.abs(kz) > 1e-30, (w * eps_m) / kz, complex(math.inf, 0.0))
+            phz = np.exp(-1j * kz * dz)
+            ckz = np.cos(kz * dz)
+            ckz = np.where(np.abs(ckz) < 1e-30, 1e-30 + 0j, ckz)
+            a_te = np.where((np.abs(p_te) > 1e-30) & ~np.isinf(np.abs(p_te)),
+                            (e_te - ns * (h_p / p_te) * phz) / (2 * ckz), 0j)
+            a_tm = np.where((np.abs(p_tm) > 1e-30) & ~np.isinf(np.abs(p_tm)),
+                            (e_tm + ns * (h_s / p_tm) * phz) / (2 * ckz), 0j)
+        t_te, t_tm = layer_transmission(layers, z_mon, freq, kx, ky)
+        pref = -1j * k_top * np.exp(1j * k_top * md.r) / (2 * math.pi * md.r)
+        e_th = pref * (t_tm * a_tm)
+        e_ph = pref * ct * (t_te * a_te)
+        Ex = e_th * ct * cp + e_ph * (-sp)
+        Ey = e_th * ct * sp + e_ph * cp
+        Ez = e_th * (-st)
+        rx, ry, rz = st * cp, st * sp, ct
+        out[:, 0, k], out[:, 1, k], out[:, 2, k] = Ex, Ey, Ez
+        out[:, 3, k] = (ry * Ez - rz * Ey) / Z_top
+        out[:, 4, k] = (rz * Ex - rx * Ez) / Z_top
+        out[:, 5, k] = (rx * Ey - ry * Ex) / Z_top
+    return out

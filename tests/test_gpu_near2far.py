@@ -69,11 +69,14 @@ def _check(dev, ref, tol):
         assert err <= tol * scale, (k, err / scale)
 
 
-@pytest.mark.parametrize("precision,dft_accum", [("f64", "f64"), ("f32", "f64"), ("f32", "f32")])
-def test_near2far_matches_oracle(precision, dft_accum):
-    sim = _scene(precision, dft_accum)
+@pytest.mark.parametrize("precision,dft_accum,mode", [("f64", "f64", "fast"), ("f64", "f64", "exact"),
+                                                      ("f32", "f64", "fast"), ("f32", "f32", "fast")])
+def test_near2far_matches_oracle(precision, dft_accum, mode):
+    """FAST contexts project with green3d_fast, EXACT ones with the
+    reference's operation order; both within 1e-12 of the oracle."""
+    sim = _scene(precision, dft_accum, mode)
     n = int(12.0 / sim.dt_n)
-    print(f"[near2far] {precision}/{dft_accum} {sim.Nx}x{sim.Ny}x{sim.Nz}, {n} steps", flush=True)
+    print(f"[near2far] {precision}/{dft_accum}/{mode} {sim.Nx}x{sim.Ny}x{sim.Nz}, {n} steps", flush=True)
     sim.step(n)
     for m in sim.monitors:
         md = m.monitor_data
@@ -107,3 +110,49 @@ def test_near2far_repeat_and_shapes():
         owner.engine.near2far(ids, 3, 1.0, base, d1, d2, dA, n1, n2, md.frequencies[:2], 1.0, 1.0, obs)
     with pytest.raises(KhronosError):
         owner.engine.near2far(ids, 3, 1.0, base, d1, d2, dA, n1, n2, md.frequencies, -1.0, 1.0, obs)
+
+
+def test_layered_spectrum_and_far_field():
+    """Layered projection: the device's spectral bins equal numpy's FFT of
+    the zero-padded planes within 1e-12, and the whole far field the numpy
+    oracle (oracle/near2far_layered.py) within 1e-10, per frequency."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import near2far_layered as LY
+    stack = kh.LayerStack([kh.LayerSpec(-5.0, 0.05, 2.25), kh.LayerSpec(0.05, 0.2, 1.7, 1.1),
+                           kh.LayerSpec(0.2, 5.0, 1.0)])
+    mon = kh.Near2FarMonitor(center=[0.0, 0.0, -0.1875], size=[1.3, 1.1, 0.0], frequencies=FREQS,
+                             theta=np.radians(np.linspace(0.0, 85.0, 35)), phi=np.linspace(0, 2 * np.pi, 13)[:-1],
+                             r=1e4, layer_stack=stack)
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=1.0, fwidth=0.5), component=kh.Ex(),
+                            center=[0.05, -0.02, -0.5], size=[0, 0, 0])]
+    sim = kh.Simulation(cell_size=[2.0, 1.75, 1.5], resolution=8, boundaries=[[0.25, 0.25]] * 3, sources=src,
+                        monitors=[mon], precision="f32")
+    sim.step(int(10.0 / sim.dt_n))
+    md = mon.monitor_data
+    mons = md.tangential_E_monitors + md.tangential_H_monitors
+    planes = [sim.read_dft(x.monitor_data) for x in mons]
+    base, n1, n2, d1, d2, dA = NF.surface_layout(md)
+    owner, ids = NF._owner(md)
+    p1, p2 = 4 * n1, 4 * n2
+    rng = np.random.default_rng(1)
+    bins = np.stack([rng.integers(-p1 // 2, p1 // 2, 300), rng.integers(-p2 // 2, p2 // 2, 300)], 1)
+    for k in range(len(FREQS)):
+        dev = owner.engine.near2far_spectrum(ids, 3, n1, n2, k, p1, p2, bins)
+        for q, p in enumerate(planes):
+            pl = (p[:, :, 0, k] + p[:, :, 1, k]) / 2 if p.shape[2] >= 2 else p[:, :, 0, k]
+            pad = np.zeros((p1, p2), dtype=np.complex128)
+            pad[:n1, :n2] = pl[:n1, :n2]
+            F = np.fft.fft2(pad)
+            want = F[np.mod(bins[:, 0], p1), np.mod(bins[:, 1], p2)]
+            assert np.abs(dev[:, q] - want).max() <= 1e-12 * np.abs(F).max()
+    got = kh.compute_far_field(mon)
+    want = LY.far_field_layered(planes, base, md.dx, md.dy, md.frequencies, md.theta, md.phi, md.r,
+                                md.normal_sign, [(L.z_min, L.z_max, L.eps, L.mu) for L in stack.layers])
+    for k in range(len(FREQS)):
+        scale = np.abs(want[:, :, k]).max()
+        assert scale > 0
+        err = np.abs(got[:, :, k] - want[:, :, k]).max() / scale
+        print(f"[near2far] layered f={FREQS[k]}: max rel err {err:.2e}", flush=True)
+        assert err <= 1e-10
