@@ -1405,6 +1405,15 @@ __device__ __forceinline__ void green3d(double (&EH)[12], double xo, double yo, 
 // kernels of fdlibm (Sun Microsystems, k_sin.c / k_cos.c, public domain) on
 // [-pi/4, pi/4], evaluated with FMAs, and the quadrant of n.  x >= 2^31 pi/2
 // (and non-finite x) go to the library sincos.
+// a * b + c with c held in a scalar register pair: the VOP3 form, so a
+// constant c is not copied into the destination first (as the two-address
+// v_fmac the compiler prefers would need).
+__device__ __forceinline__ double n2f_fma_s(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
 __device__ __forceinline__ void n2f_sincos(double x, double *sn, double *cs) {
     if (!(x < 3.0e9)) {
         sincos(x, sn, cs);
@@ -1414,17 +1423,17 @@ __device__ __forceinline__ void n2f_sincos(double x, double *sn, double *cs) {
     double t = __builtin_fma(-n, 1.5707963267948966, x);              // pi/2, leading double
     t = __builtin_fma(-n, 6.123233995736766e-17, t);                  // pi/2, next double
     const double z = t * t;
-    double ps = __builtin_fma(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08);
-    ps = __builtin_fma(z, ps, 2.75573137070700676789e-06);
-    ps = __builtin_fma(z, ps, -1.98412698298579493134e-04);
-    ps = __builtin_fma(z, ps, 8.33333333332248946124e-03);
-    ps = __builtin_fma(z, ps, -1.66666666666666324348e-01);
+    double ps = n2f_fma_s(z, 1.58969099521155010221e-10, -2.50507602534068634195e-08);
+    ps = n2f_fma_s(z, ps, 2.75573137070700676789e-06);
+    ps = n2f_fma_s(z, ps, -1.98412698298579493134e-04);
+    ps = n2f_fma_s(z, ps, 8.33333333332248946124e-03);
+    ps = n2f_fma_s(z, ps, -1.66666666666666324348e-01);
     const double s = __builtin_fma(z * t, ps, t);
-    double pc = __builtin_fma(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09);
-    pc = __builtin_fma(z, pc, -2.75573143513906633035e-07);
-    pc = __builtin_fma(z, pc, 2.48015872894767294178e-05);
-    pc = __builtin_fma(z, pc, -1.38888888888741095749e-03);
-    pc = __builtin_fma(z, pc, 4.16666666666666019037e-02);
+    double pc = n2f_fma_s(z, -1.13596475577881948265e-11, 2.08757232129817482790e-09);
+    pc = n2f_fma_s(z, pc, -2.75573143513906633035e-07);
+    pc = n2f_fma_s(z, pc, 2.48015872894767294178e-05);
+    pc = n2f_fma_s(z, pc, -1.38888888888741095749e-03);
+    pc = n2f_fma_s(z, pc, 4.16666666666666019037e-02);
     const double hz = 0.5 * z, w = 1.0 - hz;
     const double c = w + __builtin_fma(z * z, pc, (1.0 - w) - hz);
     const int q = (int)n;
@@ -1562,27 +1571,66 @@ __device__ __forceinline__ void n2f_sample(double (&EH)[12], double xo, double y
 #undef KH_N2F_CALL
 }
 
-// One thread per (observation point, frequency); FAST selects green3d_fast.
-// The block's threads share a frequency and walk the samples
-// [chunk * blockIdx.z, + chunk) of the n1 x n2 plane (i1 fastest) through an
-// LDS tile, each sample read once per block: the four DFT values at that
-// frequency, averaged over the (up to) two planes across the normal in the
-// accumulator's type (src/Monitors/Near2Far.jl:431-474) -- scaled by ns dA
-// for FAST -- and the positions base_q + i1 d1 + i2 d2 of the four currents.
-// The geometry sits in LDS too, so the sample loop keeps few scalars live.
-// The block's partial field goes to out[((z nf + k) 6 + j) n_obs + o] (re, im).
-template <typename ACC, int AX, bool UNIT, bool FAST>
-__global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2,
-                                                       const N2FGeom *gp, const double *obs, const double *freqs,
-                                                       int n_obs, int nf, int n1, int n2, int chunk, double *out) {
+// Surface samples of the projection, gathered once per call (frequencies
+// k0 + blockIdx.y): for sample c = i1 + n1 i2 of the n1 x n2 plane,
+//   vals[(blockIdx.y N + c) 8 + 2q ..] = the DFT value of plane q (e1, e2,
+//     h1, h2) at frequency k0 + blockIdx.y,
+//     averaged over the (up to) two planes across the normal in the
+//     accumulator's type (src/Monitors/Near2Far.jl:431-474), times ns dA
+//     for FAST;
+//   pos[c 12 + 3q ..] = base_q + i1 d1 + i2 d2, the position of the current
+//     made from plane q (src/Monitors/Near2Far.jl:141-144).
+template <typename ACC, int AX, bool FAST>
+__global__ __launch_bounds__(256) void n2f_gather_kernel(FluxMon e1, FluxMon e2, FluxMon h1, FluxMon h2, N2FGeom g,
+                                                         int n1, int n2, int k0, double *vals, double *pos) {
     constexpr int T0 = AX == 0 ? 1 : 0, T1 = AX == 2 ? 1 : 2;
-    __shared__ double tile[256][20];
-    __shared__ N2FGeom sg;
-    if (threadIdx.x == 0) sg = *gp;
-    __syncthreads();
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y, z = blockIdx.z;
     const int N = n1 * n2;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = k0 + blockIdx.y;
+    if (c >= N) return;
+    const int i1 = c % n1, i2 = c / n1;
+    const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
+    const double sc = FAST ? g.ns * g.dA : 1.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const FluxMon &M = *m[q];
+        const ACC *A = (const ACC *)M.acc;
+        const int64_t lin = (AX == 0 ? (int64_t)i1 * M.d[0] + (int64_t)i2 * M.d[0] * M.d[1]
+                             : AX == 1 ? (int64_t)i1 + (int64_t)i2 * M.d[0] * M.d[1]
+                                       : (int64_t)i1 + (int64_t)i2 * M.d[0]);
+        const int64_t nc = (int64_t)M.d[0] * M.d[1] * M.d[2];
+        const ACC *p0 = A + 2 * (lin + nc * k);
+        ACC r = p0[0], im = p0[1];
+        if (M.d[AX] >= 2) {
+            const int64_t step = AX == 0 ? 1 : (AX == 1 ? (int64_t)M.d[0] : (int64_t)M.d[0] * M.d[1]);
+            r = (r + p0[2 * step]) / ACC(2);
+            im = (im + p0[2 * step + 1]) / ACC(2);
+        }
+        double *w = vals + 8 * ((int64_t)blockIdx.y * N + c) + 2 * q;
+        w[0] = FAST ? (double)r * sc : (double)r;
+        w[1] = FAST ? (double)im * sc : (double)im;
+        if (blockIdx.y == 0) {
+            double *x = pos + 12 * (int64_t)c + 3 * q;
+            x[AX] = g.base[q][AX];
+            x[T0] = g.base[q][T0] + (double)i1 * g.d1;
+            x[T1] = g.base[q][T1] + (double)i2 * g.d2;
+        }
+    }
+}
+
+// One thread per (observation point, frequency k0 + blockIdx.y); FAST selects
+// green3d_fast.  The block's threads share a frequency and walk the samples
+// [chunk * blockIdx.z, + chunk) of the gathered surface through an LDS tile
+// (20 doubles a sample: four values, four positions), each sample read once
+// per block.  The block's partial field goes to
+// out[((z nf + k) 6 + j) n_obs + o] (re, im).
+template <int AX, bool UNIT, bool FAST>
+__global__ __launch_bounds__(256) void near2far_kernel(const double *vals, const double *pos, N2FGeom g,
+                                                       const double *obs, const double *freqs, int n_obs, int nf,
+                                                       int k0, int N, int chunk, double *out) {
+    __shared__ double tile[256 * 20];
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = k0 + blockIdx.y, z = blockIdx.z;
     const int c0 = z * chunk, c1 = min(N, c0 + chunk);
     double xo = 0.0, yo = 0.0, zo = 0.0;
     if (o < n_obs) {
@@ -1590,54 +1638,32 @@ __global__ __launch_bounds__(256) void near2far_kernel(FluxMon e1, FluxMon e2, F
         yo = obs[3 * o + 1];
         zo = obs[3 * o + 2];
     }
-    const N2FGeom g = sg;
     const double n = sqrt(g.eps * g.mu);
     const double kk = N2F_TWO_PI * freqs[k] * n;
     const double kn = kk * n;
     const double kn4 = kn * (1.0 / N2F_FOUR_PI), ik = 1.0 / kk;
     const N2FFast ff = {1.0 / g.eps, 1.0 / g.mu, 1.0 / g.Z};
+    const double *vk = vals + 8 * (int64_t)blockIdx.y * N;
     double EH[12] = {};
-    const FluxMon *m[4] = {&e1, &e2, &h1, &h2};
     for (int b0 = c0; b0 < c1; b0 += blockDim.x) {
-        const int c = b0 + threadIdx.x;
-        if (c < c1) {
-            const int i1 = c % n1, i2 = c / n1;
-            const double sc = FAST ? sg.ns * sg.dA : 1.0;
-            const double o1 = (double)i1 * sg.d1, o2 = (double)i2 * sg.d2;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const FluxMon &M = *m[q];
-                const ACC *A = (const ACC *)M.acc;
-                const int64_t lin = (AX == 0 ? (int64_t)i1 * M.d[0] + (int64_t)i2 * M.d[0] * M.d[1]
-                                     : AX == 1 ? (int64_t)i1 + (int64_t)i2 * M.d[0] * M.d[1]
-                                               : (int64_t)i1 + (int64_t)i2 * M.d[0]);
-                const int64_t nc = (int64_t)M.d[0] * M.d[1] * M.d[2];
-                const ACC *p0 = A + 2 * (lin + nc * k);
-                ACC r = p0[0], im = p0[1];
-                if (M.d[AX] >= 2) {
-                    const int64_t step = AX == 0 ? 1 : (AX == 1 ? (int64_t)M.d[0] : (int64_t)M.d[0] * M.d[1]);
-                    r = (r + p0[2 * step]) / ACC(2);
-                    im = (im + p0[2 * step + 1]) / ACC(2);
-                }
-                tile[threadIdx.x][2 * q] = FAST ? (double)r * sc : (double)r;
-                tile[threadIdx.x][2 * q + 1] = FAST ? (double)im * sc : (double)im;
-                tile[threadIdx.x][8 + 3 * q + AX] = sg.base[q][AX];
-                tile[threadIdx.x][8 + 3 * q + T0] = sg.base[q][T0] + o1;
-                tile[threadIdx.x][8 + 3 * q + T1] = sg.base[q][T1] + o2;
-            }
+        const int cnt = min((int)blockDim.x, c1 - b0);
+        // coalesced copy of cnt samples: 8 values then 12 position doubles each
+        for (int t = threadIdx.x; t < cnt * 20; t += blockDim.x) {
+            const int s = t / 20, e = t - 20 * s;
+            tile[t] = e < 8 ? vk[8 * (int64_t)(b0 + s) + e] : pos[12 * (int64_t)(b0 + s) + (e - 8)];
         }
         __syncthreads();
-        const int cnt = min((int)blockDim.x, c1 - b0);
         if (o < n_obs) {
             for (int s = 0; s < cnt; ++s) {
+                const double *T = tile + 20 * s;
                 double v[4][2], p[4][3];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    v[q][0] = tile[s][2 * q];
-                    v[q][1] = tile[s][2 * q + 1];
-                    p[q][0] = tile[s][8 + 3 * q];
-                    p[q][1] = tile[s][9 + 3 * q];
-                    p[q][2] = tile[s][10 + 3 * q];
+                    v[q][0] = T[2 * q];
+                    v[q][1] = T[2 * q + 1];
+                    p[q][0] = T[8 + 3 * q];
+                    p[q][1] = T[9 + 3 * q];
+                    p[q][2] = T[10 + 3 * q];
                 }
                 n2f_sample<AX, UNIT, FAST>(EH, xo, yo, zo, v, p, kk, kn, kn4, ik, g, ff);
             }

@@ -1452,17 +1452,8 @@ struct Ctx final : CtxBase {
     double *n2f_in = nullptr, *n2f_part = nullptr, *n2f_out = nullptr;  // near-to-far staging
     size_t n2f_in_cap = 0, n2f_part_cap = 0, n2f_out_cap = 0;
 
-    template <int AX, bool UNIT, bool FAST>
-    void launch_near2far(const kh::FluxMon *fm, const kh::N2FGeom *geo, const double *obs, const double *fr,
-                         int n_obs, int nf, int n1, int n2, int chunk, int splits, int bs, double *dst) {
-        dim3 grid((n_obs + bs - 1) / bs, nf, splits);
-        if (acc64)
-            hipLaunchKernelGGL((kh::near2far_kernel<double, AX, UNIT, FAST>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2],
-                               fm[3], geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
-        else
-            hipLaunchKernelGGL((kh::near2far_kernel<T, AX, UNIT, FAST>), grid, dim3(bs), 0, st, fm[0], fm[1], fm[2], fm[3],
-                               geo, obs, fr, n_obs, nf, n1, n2, chunk, dst);
-    }
+    double *n2f_surf = nullptr;   // gathered surface: values (nf x N x 8), positions (N x 12)
+    size_t n2f_surf_cap = 0;
 
     void near2far(const int32_t *ids, int ax1, double ns, const double *base, double d1, double d2, double dA,
                   int n1, int n2, const double *freqs, int nf, double eps, double mu, const double *obs, int n_obs,
@@ -1486,7 +1477,7 @@ struct Ctx final : CtxBase {
         }
         const int64_t N = (int64_t)n1 * n2;
         const int64_t m_out = (int64_t)nf * 6 * n_obs;
-        require(N < (1LL << 31) && m_out < (1LL << 31), "near-to-far problem too large");
+        require(N < (1LL << 31) && m_out < (1LL << 31) && nf <= 65535, "near-to-far problem too large");
         kh::N2FGeom geo;
         for (int q = 0; q < 4; ++q)
             for (int a = 0; a < 3; ++a) {
@@ -1513,14 +1504,10 @@ struct Ctx final : CtxBase {
         const int64_t tiles_per = (tiles + splits - 1) / splits;
         splits = (int)((tiles + tiles_per - 1) / tiles_per);
         const int chunk = (int)(tiles_per * bs);
-        // staging: geometry (padded to 256 B), observation points, frequencies
-        constexpr size_t geo_slots = (sizeof(kh::N2FGeom) + 255) / 256 * 256 / sizeof(double);
-        const size_t in_bytes = (geo_slots + (size_t)3 * n_obs + nf) * sizeof(double);
+        const size_t in_bytes = ((size_t)3 * n_obs + nf) * sizeof(double);
         n2f_in = (double *)regrow(n2f_in, n2f_in_cap, in_bytes);
-        HIPCHK(hipMemcpyAsync(n2f_in, &geo, sizeof(geo), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(n2f_in + geo_slots, obs, (size_t)3 * n_obs * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(n2f_in + geo_slots + 3 * n_obs, freqs, (size_t)nf * sizeof(double),
-                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in, obs, (size_t)3 * n_obs * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(n2f_in + 3 * n_obs, freqs, (size_t)nf * sizeof(double), hipMemcpyHostToDevice, st));
         const size_t out_bytes = (size_t)m_out * 2 * sizeof(double);
         n2f_out = (double *)regrow(n2f_out, n2f_out_cap, out_bytes);
         double *dst = n2f_out;
@@ -1528,24 +1515,42 @@ struct Ctx final : CtxBase {
             n2f_part = (double *)regrow(n2f_part, n2f_part_cap, out_bytes * splits);
             dst = n2f_part;
         }
-        const double *o_dev = n2f_in + geo_slots, *f_dev = n2f_in + geo_slots + 3 * n_obs;
-        const kh::N2FGeom *g_dev = (const kh::N2FGeom *)n2f_in;
+        // gathered surface for up to nfc frequencies at a time (<= 512 MB)
+        const int nfc = (int)std::max<int64_t>(1, std::min<int64_t>(nf, ((512LL << 20) / (8 * (int64_t)N * 8)) - 2));
+        n2f_surf = (double *)regrow(n2f_surf, n2f_surf_cap, ((size_t)nfc * 8 + 12) * N * sizeof(double));
+        double *vals = n2f_surf, *pos = n2f_surf + (size_t)nfc * 8 * N;
+        const double *o_dev = n2f_in, *f_dev = n2f_in + 3 * n_obs;
         // EXACT contexts evaluate green3d! in the reference's operation order;
         // FAST ones use green3d_fast (same phase argument, amplitudes to ~1 ulp)
         const bool fast = mode == KH_MODE_FAST;
-        auto go = [&](auto ax_c, auto unit_c) {
+        auto go = [&](auto ax_c, auto unit_c, auto fast_c) {
             constexpr int A = decltype(ax_c)::value;
-            constexpr bool U = decltype(unit_c)::value;
-            if (fast)
-                launch_near2far<A, U, true>(fm, g_dev, o_dev, f_dev, n_obs, nf, n1, n2, chunk, splits, bs, dst);
-            else
-                launch_near2far<A, U, false>(fm, g_dev, o_dev, f_dev, n_obs, nf, n1, n2, chunk, splits, bs, dst);
+            constexpr bool U = decltype(unit_c)::value, F = decltype(fast_c)::value;
+            for (int k0 = 0; k0 < nf; k0 += nfc) {
+                const int nk = std::min(nfc, nf - k0);
+                const dim3 gg((unsigned)((N + 255) / 256), nk);
+                if (acc64)
+                    hipLaunchKernelGGL((kh::n2f_gather_kernel<double, A, F>), gg, dim3(256), 0, st, fm[0], fm[1],
+                                       fm[2], fm[3], geo, n1, n2, k0, vals, pos);
+                else
+                    hipLaunchKernelGGL((kh::n2f_gather_kernel<T, A, F>), gg, dim3(256), 0, st, fm[0], fm[1], fm[2],
+                                       fm[3], geo, n1, n2, k0, vals, pos);
+                HIPCHK(hipGetLastError());
+                dim3 grid((n_obs + bs - 1) / bs, nk, splits);
+                hipLaunchKernelGGL((kh::near2far_kernel<A, U, F>), grid, dim3(bs), 0, st, (const double *)vals,
+                                   (const double *)pos, geo, o_dev, f_dev, n_obs, nf, k0, (int)N, chunk, dst);
+                HIPCHK(hipGetLastError());
+            }
         };
-        using F = std::false_type;
+        using Fl = std::false_type;
         using Tr = std::true_type;
-        if (ax == 0) unit ? go(std::integral_constant<int, 0>{}, Tr{}) : go(std::integral_constant<int, 0>{}, F{});
-        else if (ax == 1) unit ? go(std::integral_constant<int, 1>{}, Tr{}) : go(std::integral_constant<int, 1>{}, F{});
-        else unit ? go(std::integral_constant<int, 2>{}, Tr{}) : go(std::integral_constant<int, 2>{}, F{});
+        auto go2 = [&](auto ax_c) {
+            if (unit) fast ? go(ax_c, Tr{}, Tr{}) : go(ax_c, Tr{}, Fl{});
+            else fast ? go(ax_c, Fl{}, Tr{}) : go(ax_c, Fl{}, Fl{});
+        };
+        if (ax == 0) go2(std::integral_constant<int, 0>{});
+        else if (ax == 1) go2(std::integral_constant<int, 1>{});
+        else go2(std::integral_constant<int, 2>{});
         HIPCHK(hipGetLastError());
         if (splits > 1) {
             hipLaunchKernelGGL(kh::n2f_reduce_kernel, dim3((unsigned)((m_out + 255) / 256)), dim3(256), 0, st,
