@@ -1496,9 +1496,10 @@ struct Ctx final : CtxBase {
         const int bs = n_obs <= 64 ? 64 : (n_obs <= 128 ? 128 : 256);
         const int64_t blocks_xy = (int64_t)((n_obs + bs - 1) / bs) * nf;
         const int64_t tiles = (N + bs - 1) / bs;
-        // split the samples until the grid holds ~8 waves per SIMD, keeping the
+        // split the samples until the grid holds ~32 waves per SIMD (about ten
+        // rounds of resident blocks, so the last round costs little), keeping the
         // partial fields within 512 MB
-        const int64_t want = (8192 / (bs / 64) + blocks_xy - 1) / blocks_xy;
+        const int64_t want = (32768 / (bs / 64) + blocks_xy - 1) / blocks_xy;
         const int64_t cap_mem = std::max<int64_t>(1, (512LL << 20) / (m_out * 16));
         int splits = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(want, tiles), std::min<int64_t>(cap_mem, 65535)));
         const int64_t tiles_per = (tiles + splits - 1) / splits;
