@@ -282,6 +282,36 @@ int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_
                          int32_t n1, int32_t n2, int32_t freq_index, int32_t pad1, int32_t pad2,
                          const int32_t *bins, int32_t n_bins, double *out);
 
+/* Geometry objects for kh_raster_component.  shape: KH_SHAPE_BOX (size =
+ * full extents), KH_SHAPE_SPHERE (size[0] = radius), KH_SHAPE_CYLINDER
+ * (size[0] = radius, size[1] = height, axis 0..2).  value[c]: relative
+ * permittivity of Ex, Ey, Ez (c = 0..2) or permeability of Hx, Hy, Hz
+ * (3..5) inside the object; NaN where the object does not set it. */
+enum kh_shape { KH_SHAPE_BOX = 0, KH_SHAPE_SPHERE = 1, KH_SHAPE_CYLINDER = 2 };
+typedef struct kh_raster_object {
+    int32_t shape;
+    int32_t axis;
+    double center[3];
+    double size[3];
+    double value[6];
+} kh_raster_object;
+
+/* Rasterise one material component on the current device: out (precision
+ * type, cnt[0] * cnt[1] * cnt[2] values, column-major, the reference geometry
+ * layout) receives 1 / value of the first object in list order whose
+ * bounding box and shape hold the component's grid point (xs[i], ys[j],
+ * zs[k]) and that sets the component, else 1 (the reference paints objects
+ * last to first).  smoothing (electric components only): 0 none,
+ * 1 volume averaging, 2 anisotropic (Farjadpour); half[3] = half the cell
+ * size.  A voxel differing (relative 1e-6) from a face neighbour takes the
+ * effective value of the plane through the nearest shape surface.
+ * Synchronous; allocates and frees its own device memory.  Replaces: the
+ * painting of init_geometry and smooth_eps!
+ * (src/Geometry.jl:450-663, :792-955). */
+int kh_raster_component(int32_t precision, int32_t component, const int32_t cnt[3], const double *xs,
+                        const double *ys, const double *zs, const kh_raster_object *objs, int32_t n_objs,
+                        int32_t smoothing, const double half[3], void *out);
+
 /* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
  * dft_id = -1 gives the sum of that over all monitors.
  * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */

@@ -1745,4 +1745,323 @@ __global__ __launch_bounds__(256) void n2f_spectrum_kernel(FluxMon e1, FluxMon e
     }
 }
 
+// ------------------------------------------------------------ material raster
+// Geometry rasterisation and subpixel smoothing on the device
+// (src/Geometry.jl:450-663 painting, :792-955 smoothing).  Shapes: 0 box
+// (full size), 1 sphere (radius p[0]), 2 cylinder (radius p[0], height p[1],
+// axis).  value[c] is the relative permittivity (c < 3) or permeability of
+// component c, NaN where the object does not set it.
+struct RasterObj {
+    int shape, axis;
+    double c[3], p[3];
+    double value[6];
+};
+
+// The object's bounding box [lo, hi] (the host painting's index range) and
+// the shape's own test, with the host's arithmetic.
+__device__ __forceinline__ bool raster_holds(const RasterObj &o, double x, double y, double z) {
+    const double P[3] = {x, y, z};
+    double lo[3], hi[3];
+    if (o.shape == 0) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = o.c[a] - o.p[a] / 2;
+            hi[a] = o.c[a] + o.p[a] / 2;
+        }
+    } else if (o.shape == 1) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = o.c[a] - o.p[0];
+            hi[a] = o.c[a] + o.p[0];
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            const double h = a == o.axis ? o.p[1] / 2 : o.p[0];
+            lo[a] = o.c[a] - h;
+            hi[a] = o.c[a] + h;
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!(P[a] >= lo[a] && P[a] <= hi[a])) return false;
+    if (o.shape == 0) {
+        for (int a = 0; a < 3; ++a)
+            if (!(fabs(P[a] - o.c[a]) <= o.p[a] / 2)) return false;
+        return true;
+    }
+    if (o.shape == 1) {
+        const double dx = x - o.c[0], dy = y - o.c[1], dz = z - o.c[2];
+        return (dx * dx + dy * dy) + dz * dz <= o.p[0] * o.p[0];
+    }
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a)
+        if (a != o.axis) {
+            const double d = P[a] - o.c[a];
+            r2 = r2 + d * d;
+        }
+    return r2 <= o.p[0] * o.p[0] && fabs(P[o.axis] - o.c[o.axis]) <= o.p[1] / 2;
+}
+
+// The shape's own test without the bounding box (the smoothing's
+// `contains`).
+__device__ __forceinline__ bool raster_contains(const RasterObj &o, const double *P) {
+    if (o.shape == 0) {
+        for (int a = 0; a < 3; ++a)
+            if (!(fabs(P[a] - o.c[a]) <= o.p[a] / 2)) return false;
+        return true;
+    }
+    if (o.shape == 1) {
+        const double dx = P[0] - o.c[0], dy = P[1] - o.c[1], dz = P[2] - o.c[2];
+        return (dx * dx + dy * dy) + dz * dz <= o.p[0] * o.p[0];
+    }
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a)
+        if (a != o.axis) {
+            const double d = P[a] - o.c[a];
+            r2 = r2 + d * d;
+        }
+    return r2 <= o.p[0] * o.p[0] && fabs(P[o.axis] - o.c[o.axis]) <= o.p[1] / 2;
+}
+
+// out[i + n0 (j + n1 k)] = 1 / value of the first object (list order) that
+// holds (xs[i], ys[j], zs[k]) and sets component comp, else 1 -- the
+// reference paints objects last to first.
+template <typename T>
+__global__ __launch_bounds__(256) void raster_kernel(const RasterObj *objs, int n_objs, int comp, int n0, int n1,
+                                                     int n2, const double *xs, const double *ys, const double *zs,
+                                                     T *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)n0 * n1 * n2;
+    if (t >= n) return;
+    const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = (int)(t / ((int64_t)n0 * n1));
+    const double x = xs[i], y = ys[j], z = zs[k];
+    T v = T(1);
+    for (int o = 0; o < n_objs; ++o) {
+        const double p = objs[o].value[comp];
+        if (p != p) continue;
+        if (raster_holds(objs[o], x, y, z)) {
+            v = T(1) / T(p);
+            break;
+        }
+    }
+    out[t] = v;
+}
+
+// Nearest surface point and outward normal of a shape (GeometryPrimitives'
+// surfpt_nearby, restated): box -- inside, the nearest face (first axis of
+// the largest |L| - h), outside, the clamp onto the box with the normal along
+// P - clamp; sphere -- radial; cylinder -- the side or a cap inside, the
+// clamp outside.
+__device__ __forceinline__ void raster_normalize(double *v) {
+    const double n = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    if (n > 0) {
+        v[0] = v[0] / n;
+        v[1] = v[1] / n;
+        v[2] = v[2] / n;
+    } else {
+        v[0] = 0.0;
+        v[1] = 0.0;
+        v[2] = 1.0;
+    }
+}
+
+__device__ __forceinline__ void raster_surfpt(const RasterObj &o, const double *P, double *sp, double *nout) {
+    if (o.shape == 0) {
+        double L[3], s[3], d[3], h[3];
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+            h[a] = o.p[a] / 2;
+            L[a] = P[a] - o.c[a];
+            s[a] = L[a] >= 0 ? 1.0 : -1.0;
+            d[a] = fabs(L[a]) - h[a];
+            inside = inside && d[a] <= 0;
+        }
+        if (inside) {
+            int k = 0;
+            for (int a = 1; a < 3; ++a)
+                if (d[a] > d[k]) k = a;
+            for (int a = 0; a < 3; ++a) {
+                sp[a] = P[a];
+                nout[a] = 0.0;
+            }
+            sp[k] = o.c[k] + s[k] * h[k];
+            nout[k] = s[k];
+        } else {
+            for (int a = 0; a < 3; ++a) {
+                const double lo = o.c[a] - h[a], hi = o.c[a] + h[a];
+                sp[a] = fmin(fmax(P[a], lo), hi);
+                nout[a] = P[a] - sp[a];
+            }
+            raster_normalize(nout);
+        }
+        return;
+    }
+    if (o.shape == 1) {
+        double v[3] = {P[0] - o.c[0], P[1] - o.c[1], P[2] - o.c[2]};
+        const double n = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        if (n > 0) {
+            nout[0] = v[0] / n;
+            nout[1] = v[1] / n;
+            nout[2] = v[2] / n;
+        } else {
+            nout[0] = 1.0;
+            nout[1] = 0.0;
+            nout[2] = 0.0;
+        }
+        for (int a = 0; a < 3; ++a) sp[a] = o.c[a] + o.p[0] * nout[a];
+        return;
+    }
+    const int ax = o.axis;
+    double v[3] = {P[0] - o.c[0], P[1] - o.c[1], P[2] - o.c[2]};
+    const double ell = v[ax];
+    double rho[3] = {v[0], v[1], v[2]};
+    rho[ax] = 0.0;
+    const double rn = sqrt((rho[0] * rho[0] + rho[1] * rho[1]) + rho[2] * rho[2]);
+    double rhat[3] = {0.0, 0.0, 0.0};
+    if (rn > 0) {
+        for (int a = 0; a < 3; ++a) rhat[a] = rho[a] / rn;
+    } else {
+        rhat[(ax + 1) % 3] = 1.0;
+    }
+    const double sgn = ell >= 0 ? 1.0 : -1.0;
+    const double d_side = rn - o.p[0];
+    const double d_cap = fabs(ell) - o.p[1] / 2;
+    double ncap[3] = {0.0, 0.0, 0.0};
+    ncap[ax] = sgn;
+    if (d_side <= 0 && d_cap <= 0) {
+        if (d_side >= d_cap) {
+            for (int a = 0; a < 3; ++a) {
+                sp[a] = P[a] - d_side * rhat[a];
+                nout[a] = rhat[a];
+            }
+        } else {
+            for (int a = 0; a < 3; ++a) {
+                sp[a] = P[a] - d_cap * ncap[a];
+                nout[a] = ncap[a];
+            }
+        }
+        return;
+    }
+    const double ds = fmax(d_side, 0.0), dc = fmax(d_cap, 0.0);
+    for (int a = 0; a < 3; ++a) {
+        const double delta = ds * rhat[a] + dc * ncap[a];
+        sp[a] = P[a] - delta;
+        nout[a] = delta;
+    }
+    raster_normalize(nout);
+}
+
+// Fraction of the box [lo, hi] with (x - r0) . n <= 0: with the box mapped to
+// [0, L] and the axes flipped so every used n_i > 0, the volume of
+// {n . x <= t} is sum over the corners v of the k used axes of
+// (-1)^|v| max(0, t - n . (v L))^k / (k! prod n_i L_i); an axis whose n_i L_i
+// is below 1e-9 of the largest is dropped (it contributes its length).
+__device__ __forceinline__ double raster_volfrac(const double *lo, const double *hi, const double *n,
+                                                 const double *r0) {
+    double L[3], an[3], span[3], t = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        L[a] = hi[a] - lo[a];
+        const double base = n[a] >= 0 ? lo[a] : hi[a];
+        t = a == 0 ? n[a] * (r0[a] - base) : t + n[a] * (r0[a] - base);
+        an[a] = fabs(n[a]);
+        span[a] = an[a] * fabs(L[a]);
+    }
+    const double smax = fmax(fmax(fmax(span[0], span[1]), span[2]), 1e-300);
+    int ks[3], k = 0;
+    for (int a = 0; a < 3; ++a)
+        if (span[a] > 1e-9 * smax) ks[k++] = a;
+    if (k == 0) return t >= 0 ? 1.0 : 0.0;
+    double acc = 0.0;
+    for (int bits = 0; bits < (1 << k); ++bits) {
+        double w = t, sign = 1.0;
+        for (int j = 0; j < k; ++j)
+            if (bits >> j & 1) {
+                w = w - an[ks[j]] * L[ks[j]];
+                sign = -sign;
+            }
+        const double m = w > 0 ? w : 0.0;
+        const double pw = k == 1 ? m : (k == 2 ? m * m : m * m * m);
+        acc = acc + sign * pw;
+    }
+    double pn = an[ks[0]], pl = L[ks[0]];
+    for (int j = 1; j < k; ++j) {
+        pn = pn * an[ks[j]];
+        pl = pl * L[ks[j]];
+    }
+    const double fact = k == 1 ? 1.0 : (k == 2 ? 2.0 : 6.0);
+    const double f = acc / ((fact * pn) * pl);
+    return fmin(fmax(f, 0.0), 1.0);
+}
+
+// Subpixel smoothing of electric component `axis` (0..2), reading the
+// unsmoothed raster `in`: a voxel differing (relative 1e-6) from a face
+// neighbour -- the first of +x, -x, +y, -y, +z, -z -- takes the effective
+// value of the plane through the nearest shape surface (first object on
+// ties): fill fraction f, <eps> = f e_in + (1 - f) e_out, <1/eps> likewise;
+// aniso: (1 - n_k^2) <1/eps> + n_k^2 / <eps>, else 1 / <eps>.
+template <typename T>
+__global__ __launch_bounds__(256) void smooth_kernel(const RasterObj *objs, int n_objs, int axis, int aniso, int n0,
+                                                     int n1, int n2, const double *xs, const double *ys,
+                                                     const double *zs, double hx, double hy, double hz, const T *in,
+                                                     T *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)n0 * n1 * n2;
+    if (t >= n) return;
+    const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = (int)(t / ((int64_t)n0 * n1));
+    const T c = in[t];
+    const T rtol = T(1e-6);
+    const int idx[3] = {i, j, k}, dims[3] = {n0, n1, n2};
+    const int64_t stride[3] = {1, n0, (int64_t)n0 * n1};
+    bool found = false;
+    T nb = T(0);
+    for (int q = 0; q < 6 && !found; ++q) {
+        const int a = q >> 1, st = (q & 1) ? -1 : 1;
+        const int m = idx[a] + st;
+        if (m < 0 || m >= dims[a]) continue;
+        const T b = in[t + st * stride[a]];
+        if (fabs(b - c) > rtol * (fabs(b) > fabs(c) ? fabs(b) : fabs(c))) {
+            found = true;
+            nb = b;
+        }
+    }
+    if (!found) {
+        out[t] = c;
+        return;
+    }
+    const double P[3] = {xs[i], ys[j], zs[k]};
+    double best = INFINITY, bsp[3] = {P[0], P[1], P[2]}, bn[3] = {0.0, 0.0, 1.0};
+    int bi = -1;
+    for (int o = 0; o < n_objs; ++o) {
+        double sp[3], no[3];
+        raster_surfpt(objs[o], P, sp, no);
+        const double e0 = sp[0] - P[0], e1 = sp[1] - P[1], e2 = sp[2] - P[2];
+        const double d2 = (e0 * e0 + e1 * e1) + e2 * e2;
+        if (d2 < best) {
+            best = d2;
+            for (int a = 0; a < 3; ++a) {
+                bsp[a] = sp[a];
+                bn[a] = no[a];
+            }
+            bi = o;
+        }
+    }
+    if (bi < 0) {
+        out[t] = c;
+        return;
+    }
+    raster_normalize(bn);
+    const double lo[3] = {P[0] - hx, P[1] - hy, P[2] - hz}, hi[3] = {P[0] + hx, P[1] + hy, P[2] + hz};
+    const double f = raster_volfrac(lo, hi, bn, bsp);
+    const double ec = (double)(T(1) / c), en = (double)(T(1) / nb);
+    const bool inside = raster_contains(objs[bi], P);
+    const double es = inside ? ec : en, eb = inside ? en : ec;
+    const double e_avg = f * es + (1 - f) * eb;
+    const double e_harm = f / es + (1 - f) / eb;
+    double val;
+    if (aniso) {
+        const double n2v = bn[axis] * bn[axis];
+        val = (1 - n2v) * e_harm + n2v / e_avg;
+    } else {
+        val = 1.0 / e_avg;
+    }
+    out[t] = (T)val;
+}
+
 }  // namespace kh

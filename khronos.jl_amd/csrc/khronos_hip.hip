@@ -1740,6 +1740,61 @@ struct kh_ctx {
     std::unique_ptr<CtxBase> impl;
 };
 
+namespace {
+template <typename T>
+void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_t *cnt, const double *xs,
+                const double *ys, const double *zs, int smoothing, const double *half, void *out) {
+    const int64_t n = (int64_t)cnt[0] * cnt[1] * cnt[2];
+    std::vector<kh::RasterObj> ho(std::max(1, n_objs));
+    for (int o = 0; o < n_objs; ++o) {
+        const kh_raster_object &r = objs[o];
+        require(r.shape >= 0 && r.shape <= 2, "shape must be KH_SHAPE_BOX, _SPHERE or _CYLINDER");
+        require(r.axis >= 0 && r.axis <= 2, "cylinder axis must be 0..2");
+        ho[o].shape = r.shape;
+        ho[o].axis = r.axis;
+        for (int a = 0; a < 3; ++a) {
+            ho[o].c[a] = r.center[a];
+            ho[o].p[a] = r.size[a];
+        }
+        for (int c = 0; c < 6; ++c) ho[o].value[c] = r.value[c];
+    }
+    const bool smooth = smoothing != 0 && comp < 3;
+    // device buffers: objects, coordinates, raster (+ smoothed copy)
+    const size_t ob = ho.size() * sizeof(kh::RasterObj);
+    const size_t cb = (size_t)(cnt[0] + cnt[1] + cnt[2]) * sizeof(double);
+    const size_t ab = (size_t)n * sizeof(T);
+    void *buf = nullptr;
+    const size_t total = ob + cb + ab * (smooth ? 2 : 1) + 512;
+    HIPCHK(hipMalloc(&buf, total));
+    struct Free {
+        void *p;
+        ~Free() { (void)hipFree(p); }
+    } guard{buf};
+    char *b = (char *)buf;
+    kh::RasterObj *d_obj = (kh::RasterObj *)b;
+    double *d_x = (double *)(b + ((ob + 255) / 256) * 256);
+    double *d_y = d_x + cnt[0], *d_z = d_y + cnt[1];
+    T *d_a = (T *)((char *)d_x + ((cb + 255) / 256) * 256);
+    T *d_s = d_a + n;
+    HIPCHK(hipMemcpy(d_obj, ho.data(), ob, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_x, xs, cnt[0] * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_y, ys, cnt[1] * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_z, zs, cnt[2] * sizeof(double), hipMemcpyHostToDevice));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(kh::raster_kernel<T>, dim3(blocks), dim3(256), 0, 0, (const kh::RasterObj *)d_obj, n_objs,
+                       comp, cnt[0], cnt[1], cnt[2], (const double *)d_x, (const double *)d_y, (const double *)d_z,
+                       d_a);
+    HIPCHK(hipGetLastError());
+    if (smooth) {
+        hipLaunchKernelGGL(kh::smooth_kernel<T>, dim3(blocks), dim3(256), 0, 0, (const kh::RasterObj *)d_obj,
+                           n_objs, comp, smoothing == 2 ? 1 : 0, cnt[0], cnt[1], cnt[2], (const double *)d_x,
+                           (const double *)d_y, (const double *)d_z, half[0], half[1], half[2], (const T *)d_a, d_s);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpy(out, smooth ? d_s : d_a, ab, hipMemcpyDeviceToHost));
+}
+}  // namespace
+
 extern "C" {
 
 const char *kh_last_error(void) { return g_last_error.c_str(); }
@@ -1944,6 +1999,26 @@ int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_
         require(bins != nullptr && out != nullptr, "null argument");
         const int32_t ids[4] = {e1, e2, h1, h2};
         ctx->impl->near2far_spectrum(ids, normal_axis, n1, n2, freq_index, pad1, pad2, bins, n_bins, out);
+    });
+}
+
+
+int kh_raster_component(int32_t precision, int32_t component, const int32_t cnt[3], const double *xs,
+                        const double *ys, const double *zs, const kh_raster_object *objs, int32_t n_objs,
+                        int32_t smoothing, const double half[3], void *out) {
+    return guarded([&] {
+        require(precision == KH_F32 || precision == KH_F64, "precision must be KH_F32 or KH_F64");
+        require(component >= 0 && component < 6, "component must be 0..5");
+        require(cnt != nullptr && xs != nullptr && ys != nullptr && zs != nullptr && out != nullptr, "null argument");
+        require(cnt[0] >= 1 && cnt[1] >= 1 && cnt[2] >= 1, "empty component array");
+        require((int64_t)cnt[0] * cnt[1] * cnt[2] < (1LL << 40), "component array too large");
+        require(n_objs >= 0 && (n_objs == 0 || objs != nullptr), "bad object list");
+        require(smoothing >= 0 && smoothing <= 2, "smoothing must be 0 (none), 1 (volume average), 2 (anisotropic)");
+        require(smoothing == 0 || component >= 3 || half != nullptr, "smoothing needs the half cell size");
+        if (precision == KH_F32)
+            raster_run<float>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out);
+        else
+            raster_run<double>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out);
     });
 }
 

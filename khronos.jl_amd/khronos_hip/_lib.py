@@ -16,6 +16,11 @@ KH_F32, KH_F64 = 0, 1
 KH_MODE_FAST, KH_MODE_EXACT = 0, 1
 
 
+class KhRasterObject(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("axis", C.c_int32), ("center", C.c_double * 3), ("size", C.c_double * 3),
+                ("value", C.c_double * 6)]
+
+
 class KhSimDesc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -52,6 +57,8 @@ SIGNATURES = {
     "kh_near2far_spectrum": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                        C.c_int32, C.c_void_p]),
+    "kh_raster_component": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "kh_set_chi3": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kh_add_pole": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "kh_read_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),

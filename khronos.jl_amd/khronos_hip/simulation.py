@@ -66,13 +66,17 @@ class Simulation:
     def __init__(self, *, cell_size, resolution, sources, cell_center=(0.0, 0.0, 0.0),
                  boundaries=None, boundary_conditions=None, geometry=None, monitors=None,
                  absorbers=None, subpixel_smoothing=None, Courant=0.5, precision="f64", mode="fast",
-                 dft_accum="f64", num_chunks=1, engine=None, setup_precision=None):
+                 dft_accum="f64", num_chunks=1, engine=None, setup_precision=None, raster="host"):
         if num_chunks not in (1, None):
             raise NotImplementedError("only the single-domain layout (num_chunks=1) is provided")
         if precision not in ("f32", "f64"):
             raise ValueError("precision must be 'f32' or 'f64'")
         if setup_precision not in (None, "f32", "f64"):
             raise ValueError("setup_precision must be 'f32' or 'f64'")
+        if raster not in ("host", "device"):
+            raise ValueError("raster must be 'host' or 'device'")
+        # "device": paint and smooth the material arrays with the HIP library
+        self.raster = raster
         # the precision form: setup_precision="f64" with precision="f32" keeps
         # the grid snapping, weights, profiles and waveforms of the FP64 run
         # and runs the time step in FP32
@@ -135,9 +139,13 @@ class Simulation:
     def build_spec(self) -> SimSpec:
         """Host setup of ``prepare_simulation!`` (src/Simulation.jl:92-283)."""
         sig = init_boundaries(self)
-        eps_inv, mu_inv = rasterize(self, self.geometry)
-        # subpixel smoothing of eps^-1 before anything reads it (src/Geometry.jl:602-603)
-        eps_inv = apply_subpixel_smoothing(self, self.geometry, eps_inv)
+        if self.raster == "device":
+            from .raster import raster_device
+            eps_inv, mu_inv = raster_device(self, self.geometry)
+        else:
+            eps_inv, mu_inv = rasterize(self, self.geometry)
+            # subpixel smoothing of eps^-1 before anything reads it (src/Geometry.jl:602-603)
+            eps_inv = apply_subpixel_smoothing(self, self.geometry, eps_inv)
         chi3 = rasterize_chi3(self, self.geometry)
         poles, eps_inv = init_polarization(self, self.geometry, eps_inv)
         # conductivity (src/Geometry.jl:467-511, :708-789)
