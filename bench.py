@@ -85,13 +85,24 @@ def serving_bench(args):
         sys.stderr.write("no GPU visible\n")
         return 1
     from khronos_hip import serving
-    reqs = serving.serving_trace(args.requests, seed=0)
+    reqs = serving.serving_trace(args.requests, seed=0, rate=args.rate)
     free, total = torch.cuda.mem_get_info()
     budget = 0.85 * free
     sys.stderr.write(f"serving {len(reqs)} requests ({sum(r.cells * r.steps for r in reqs) / 1e9:.1f} G cell-updates)\n")
-    served = serving.Server(budget_bytes=budget, max_active=args.max_active, chunk=args.chunk).run(reqs)
+
+    def progress(label):
+        every = max(1, len(reqs) // 16)
+
+        def report(done, total, t):
+            if done % every == 0 or done == total:
+                sys.stderr.write(f"{label}: {done}/{total} requests done at {t:.1f} s\n")
+                sys.stderr.flush()
+        return report
+
+    served = serving.Server(budget_bytes=budget, max_active=args.max_active, chunk=args.chunk,
+                            progress=progress("served")).run(reqs)
     sys.stderr.write(f"served: makespan {served.makespan:.2f} s; one at a time next\n")
-    solo = serving.run_one_at_a_time(reqs)
+    solo = serving.run_one_at_a_time(reqs, progress=progress("one at a time"))
     sys.stderr.write(f"one at a time: makespan {solo.makespan:.2f} s\n")
     same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
     line = {
@@ -102,7 +113,9 @@ def serving_bench(args):
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} requests, grids 64^3..256^3, 1000-3000 steps, all arrived at t=0",
+        "config": {"workload": f"{len(reqs)} requests, grids 64^3..256^3, 1000-3000 steps, "
+                               + ("all arrived at t=0" if math.isinf(args.rate) else
+                                  f"Poisson arrivals at {args.rate} requests/s (seed 0)"),
                    "max_active": args.max_active, "chunk_steps": args.chunk,
                    "budget_GB": round(budget / 1e9, 1), "cell_updates": served.cell_updates},
         "one_at_a_time": round(solo.throughput, 2),
@@ -199,6 +212,7 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--max-active", type=int, default=6)
     ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrival rate (requests/s)")
     args = ap.parse_args(argv)
     if args.serving:
         return serving_bench(args)

@@ -122,7 +122,9 @@ class Server:
     """Continuous-batching scheduler over libkhronos_hip contexts (one GPU)."""
 
     def __init__(self, budget_bytes: float, max_active: int = 6, chunk: int = 256, prep_threads: int = 2,
-                 graph_steps: int = 16):
+                 graph_steps: int = 16, progress=None):
+        # progress: optional callable(done, total, seconds) after each retired request
+        self.progress = progress
         self.budget = float(budget_bytes)
         self.max_active = max_active
         self.chunk = chunk
@@ -201,6 +203,8 @@ class Server:
                     sim.engine.close()
                     active.remove(item)
                     results.append(Result(r.rid, flux, r.arrival, ts, done, r.cells, r.steps))
+                    if self.progress is not None:
+                        self.progress(len(results), len(reqs), done)
                     used -= r.footprint
                     inflight -= 1
                     progressed = True
@@ -214,7 +218,7 @@ class Server:
                            float(sum(r.cells * r.steps for r in results)))
 
 
-def run_one_at_a_time(requests: List[Request], graph_steps: int = 16) -> ServeReport:
+def run_one_at_a_time(requests: List[Request], graph_steps: int = 16, progress=None) -> ServeReport:
     """Baseline: the same trace served one request at a time."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
@@ -233,6 +237,8 @@ def run_one_at_a_time(requests: List[Request], graph_steps: int = 16) -> ServeRe
         done = time.perf_counter() - t0
         sim.engine.close()
         results.append(Result(r.rid, flux, r.arrival, start, done, r.cells, r.steps))
+        if progress is not None:
+            progress(len(results), len(reqs), done)
     makespan = max(x.done for x in results) - min(x.arrival for x in results)
     return ServeReport(sorted(results, key=lambda x: x.rid), makespan,
                        float(sum(x.cells * x.steps for x in results)))
