@@ -282,6 +282,31 @@ int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_
                          int32_t n1, int32_t n2, int32_t freq_index, int32_t pad1, int32_t pad2,
                          const int32_t *bins, int32_t n_bins, double *out);
 
+/* Continuous-batching scheduler over contexts (one GPU).  Replaces:
+ * run_batch_concurrent and plan_batch (src/Batch.jl:142-263, :22-69), whose
+ * waves of tasks on separate streams become a rolling admission.
+ *   kh_server_admit: reserve a slot and `footprint` bytes of the budget
+ *     (*admitted = 1) if fewer than max_active requests are in flight and the
+ *     budget holds it, or nothing else is in flight; then build the context.
+ *   kh_server_submit: hand over the admitted request's context for n_steps
+ *     steps; the caller keeps ownership and must not touch the context until
+ *     the request is reported done.  kh_server_cancel returns an admission
+ *     that will not be submitted.
+ *   kh_server_poll: without blocking, gives every idle submitted context its
+ *     next chunk_steps steps (kh_step on its own stream) and retires those
+ *     with no steps left; done_ids receives up to cap finished request ids in
+ *     completion order, *n_done their count, *n_inflight (may be NULL) the
+ *     requests admitted and not yet reported.
+ * admit/submit/cancel may be called from other threads than poll; each
+ * context is stepped from the polling thread only. */
+typedef struct kh_server kh_server;
+int kh_server_create(double budget_bytes, int32_t max_active, int32_t chunk_steps, kh_server **out);
+int kh_server_destroy(kh_server *server);
+int kh_server_admit(kh_server *server, double footprint, int32_t *admitted);
+int kh_server_submit(kh_server *server, kh_ctx *ctx, int64_t n_steps, double footprint, int64_t request_id);
+int kh_server_cancel(kh_server *server, double footprint);
+int kh_server_poll(kh_server *server, int64_t *done_ids, int32_t cap, int32_t *n_done, int32_t *n_inflight);
+
 /* Geometry objects for kh_raster_component.  shape: KH_SHAPE_BOX (size =
  * full extents), KH_SHAPE_SPHERE (size[0] = radius), KH_SHAPE_CYLINDER
  * (size[0] = radius, size[1] = height, axis 0..2).  value[c]: relative

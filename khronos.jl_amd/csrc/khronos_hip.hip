@@ -19,6 +19,8 @@
 #include <type_traits>
 #include <string>
 #include <vector>
+#include <mutex>
+#include <deque>
 
 #include "../../include/khronos_hip.h"
 #include "kh_kernels.hpp"
@@ -1740,6 +1742,25 @@ struct kh_ctx {
     std::unique_ptr<CtxBase> impl;
 };
 
+// Continuous-batching scheduler over contexts (the serving form of
+// src/Batch.jl:142-263, admission by footprint as plan_batch, :22-69).
+// admit/submit may come from preparation threads while one thread polls;
+// every context is stepped from the polling thread only.
+struct kh_server {
+    struct Req {
+        kh_ctx *ctx;
+        int64_t remaining;
+        double footprint;
+        int64_t id;
+    };
+    double budget = 0.0, used = 0.0;
+    int max_active = 1, chunk = 256;
+    int reserved = 0;           // admitted, context not yet submitted
+    std::deque<Req> active;     // submitted, stepping or waiting for a chunk
+    std::deque<int64_t> done;   // finished, not yet reported
+    std::mutex mu;
+};
+
 namespace {
 template <typename T>
 void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_t *cnt, const double *xs,
@@ -1902,6 +1923,95 @@ int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph) {
     return guarded([&] {
         CTX_OK;
         ctx->impl->set_graph(steps_per_graph);
+    });
+}
+
+int kh_server_create(double budget_bytes, int32_t max_active, int32_t chunk_steps, kh_server **out) {
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        require(budget_bytes > 0 && std::isfinite(budget_bytes), "budget must be positive");
+        require(max_active >= 1 && chunk_steps >= 1, "max_active and chunk_steps must be >= 1");
+        std::unique_ptr<kh_server> s(new kh_server());
+        s->budget = budget_bytes;
+        s->max_active = max_active;
+        s->chunk = chunk_steps;
+        *out = s.release();
+    });
+}
+
+int kh_server_destroy(kh_server *s) {
+    return guarded([&] {
+        require(s != nullptr, "server is null");
+        delete s;
+    });
+}
+
+int kh_server_admit(kh_server *s, double footprint, int32_t *admitted) {
+    return guarded([&] {
+        require(s != nullptr && admitted != nullptr, "null argument");
+        require(footprint >= 0 && std::isfinite(footprint), "footprint must be >= 0");
+        std::lock_guard<std::mutex> lock(s->mu);
+        const int inflight = s->reserved + (int)s->active.size();
+        // a request larger than the budget still runs when it would run alone
+        const bool fits = s->used + footprint <= s->budget || inflight == 0;
+        *admitted = (inflight < s->max_active && fits) ? 1 : 0;
+        if (*admitted) {
+            s->reserved += 1;
+            s->used += footprint;
+        }
+    });
+}
+
+int kh_server_submit(kh_server *s, kh_ctx *ctx, int64_t n_steps, double footprint, int64_t request_id) {
+    return guarded([&] {
+        require(s != nullptr, "server is null");
+        CTX_OK;
+        require(n_steps >= 0, "n_steps must be >= 0");
+        std::lock_guard<std::mutex> lock(s->mu);
+        require(s->reserved > 0, "submit without a matching kh_server_admit");
+        s->reserved -= 1;
+        s->active.push_back({ctx, n_steps, footprint, request_id});
+    });
+}
+
+int kh_server_cancel(kh_server *s, double footprint) {
+    return guarded([&] {
+        require(s != nullptr, "server is null");
+        std::lock_guard<std::mutex> lock(s->mu);
+        require(s->reserved > 0, "cancel without a matching kh_server_admit");
+        s->reserved -= 1;
+        s->used -= footprint;
+    });
+}
+
+int kh_server_poll(kh_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done, int32_t *n_inflight) {
+    return guarded([&] {
+        require(s != nullptr && n_done != nullptr, "null argument");
+        require(cap >= 0 && (cap == 0 || done_ids != nullptr), "bad done buffer");
+        std::lock_guard<std::mutex> lock(s->mu);
+        for (auto it = s->active.begin(); it != s->active.end();) {
+            if (!it->ctx->impl->idle()) {
+                ++it;
+                continue;
+            }
+            if (it->remaining > 0) {
+                const int64_t n = std::min<int64_t>(s->chunk, it->remaining);
+                it->ctx->impl->step(n);  // asynchronous on the context's stream
+                it->remaining -= n;
+                ++it;
+                continue;
+            }
+            s->used -= it->footprint;
+            s->done.push_back(it->id);
+            it = s->active.erase(it);
+        }
+        int k = 0;
+        while (k < cap && !s->done.empty()) {
+            done_ids[k++] = s->done.front();
+            s->done.pop_front();
+        }
+        *n_done = k;
+        if (n_inflight) *n_inflight = s->reserved + (int)s->active.size();
     });
 }
 

@@ -59,6 +59,12 @@ SIGNATURES = {
                                        C.c_int32, C.c_void_p]),
     "kh_raster_component": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "kh_server_create": (C.c_int, [C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
+    "kh_server_destroy": (C.c_int, [C.c_void_p]),
+    "kh_server_admit": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    "kh_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_int64]),
+    "kh_server_cancel": (C.c_int, [C.c_void_p, C.c_double]),
+    "kh_server_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "kh_set_chi3": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kh_add_pole": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "kh_read_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),

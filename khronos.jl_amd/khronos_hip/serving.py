@@ -12,9 +12,10 @@ are admitted continuously instead of in waves:
 * preparation -- host setup and the device context of an admitted request
   are built on worker threads, so the GPU keeps stepping the others;
 * stepping -- every active request owns a context (its own stream and its
-  own captured step graphs); the scheduler keeps ``chunk`` steps queued on
-  each stream (``kh_step`` is asynchronous) and polls ``kh_query`` instead of
-  synchronizing, so up to ``max_active`` streams run concurrently;
+  own captured step graphs); the library's scheduler (``kh_server_poll``)
+  gives each idle context its next ``chunk`` steps (``kh_step`` is
+  asynchronous) instead of synchronizing, so up to ``max_active`` streams run
+  concurrently;
 * retirement -- a finished request's flux is reduced on the device, its
   context freed, and its slot and bytes go to the next request in arrival
   order.
@@ -132,17 +133,21 @@ class Server:
         self.graph_steps = graph_steps
 
     def run(self, requests: List[Request]) -> ServeReport:
+        """Serve ``requests`` through the library's scheduler
+        (kh_server_admit / kh_server_submit / kh_server_poll): admission and
+        stepping run natively, host setup on worker threads."""
+        import ctypes as C
         from . import _lib
-        _lib.load()  # load the library (and the HIP runtime) before the worker threads start
+        lib = _lib.load()  # load the library (and the HIP runtime) before the worker threads start
+        srv = C.c_void_p()
+        _lib.check(lib.kh_server_create(float(self.budget), int(self.max_active), int(self.chunk), C.byref(srv)))
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = list(reqs)
-        ready: "queue.Queue" = queue.Queue()
         jobs: "queue.Queue" = queue.Queue()
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        used = 0.0
-        inflight = 0          # admitted: preparing or stepping
-        active = []           # [req, sim, remaining, start]
+        sims = {}             # request id -> (request, simulation, start time)
+        lock = threading.Lock()
         results: List[Result] = []
         errors = []
 
@@ -155,64 +160,53 @@ class Server:
                     sim = r.make()
                     sim.prepare()
                     sim.engine.set_graph(self.graph_steps)
-                    ready.put((r, sim, now()))
+                    sim.engine._ensure_waveforms(r.steps)  # the scheduler steps the context natively
+                    with lock:
+                        sims[r.rid] = (r, sim, now())
+                    _lib.check(lib.kh_server_submit(srv, sim.engine.ctx, int(r.steps), float(r.footprint),
+                                                    int(r.rid)))
                 except Exception as e:  # surfaced on the scheduler thread
                     errors.append(e)
-                    ready.put((r, None, now()))
+                    _lib.check(lib.kh_server_cancel(srv, float(r.footprint)))
 
         threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.prep_threads)]
         for th in threads:
             th.start()
+        done_buf = (C.c_int64 * 64)()
+        n_done = C.c_int32()
+        inflight = C.c_int32(0)
+        admitted = C.c_int32()
         try:
-            while pending or inflight:
+            while True:
                 if errors:
                     raise errors[0]
                 t = now()
                 # admit in arrival order while the budget and the slots allow
-                while pending and pending[0].arrival <= t and inflight < self.max_active:
-                    r = pending[0]
-                    if r.footprint > 0 and used + r.footprint > self.budget and inflight > 0:
+                while pending and pending[0].arrival <= t:
+                    _lib.check(lib.kh_server_admit(srv, float(pending[0].footprint), C.byref(admitted)))
+                    if not admitted.value:
                         break
-                    pending.pop(0)
-                    used += r.footprint
-                    inflight += 1
-                    jobs.put(r)
-                # contexts ready to step
-                while True:
-                    try:
-                        r, sim, ts = ready.get_nowait()
-                    except queue.Empty:
-                        break
-                    if sim is None:
-                        continue
-                    active.append([r, sim, r.steps, ts])
-                # keep every stream fed; retire the finished
-                progressed = False
-                for item in list(active):
-                    r, sim, remaining, ts = item
-                    if not sim.engine.idle():
-                        continue
-                    if remaining > 0:
-                        n = min(self.chunk, remaining)
-                        sim.step(n)
-                        item[2] -= n
-                        progressed = True
-                        continue
+                    jobs.put(pending.pop(0))
+                _lib.check(lib.kh_server_poll(srv, done_buf, 64, C.byref(n_done), C.byref(inflight)))
+                for k in range(n_done.value):
+                    with lock:
+                        r, sim, ts = sims.pop(int(done_buf[k]))
+                    sim.engine.ts = r.steps
+                    sim.timestep = r.steps
                     flux = _flux_of(sim)
                     done = now()
                     sim.engine.close()
-                    active.remove(item)
                     results.append(Result(r.rid, flux, r.arrival, ts, done, r.cells, r.steps))
                     if self.progress is not None:
                         self.progress(len(results), len(reqs), done)
-                    used -= r.footprint
-                    inflight -= 1
-                    progressed = True
-                if not progressed:
+                if not pending and inflight.value == 0:
+                    break
+                if n_done.value == 0:
                     time.sleep(0.0002)
         finally:
             for _ in threads:
                 jobs.put(None)
+            lib.kh_server_destroy(srv)
         makespan = max(r.done for r in results) - min(r.arrival for r in results)
         return ServeReport(sorted(results, key=lambda r: r.rid), makespan,
                            float(sum(r.cells * r.steps for r in results)))
