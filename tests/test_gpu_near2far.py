@@ -2,7 +2,8 @@
 
 Parity: the far fields of four surfaces (all three normal axes, both normal
 signs, one- and two-plane DFTs across the normal, a non-vacuum medium,
-far (r = 1e6) and near observation points, split and unsplit sample sums)
+far (r = 1e6 and 2e9) and near observation points, split and unsplit
+sample sums)
 equal the oracle's surface sum (oracle/khronos_oracle.c orc_near2far, pinned
 in tests/test_near2far.py) on the same DFT arrays within 1e-12 of the
 largest field component of each observation set and frequency.
@@ -36,7 +37,11 @@ def _scene(precision="f64", dft_accum="f64", mode="fast"):
                                theta=np.linspace(0.2, 2.9, 300), phi=np.linspace(0.0, 6.0, 9), r=50.0,
                                medium_eps=2.25, medium_mu=1.1),
             kh.Near2FarMonitor(center=[0, 0, -0.375], size=[0.2, 0.15, 0], frequencies=FREQS,
-                               observation_points=near[:5], normal_dir="-")]
+                               observation_points=near[:5], normal_dir="-"),
+            # k r ~ 1e10: the FAST sincos hands these arguments to the library
+            kh.Near2FarMonitor(center=[0, 0, 0.4375], size=[1.0, 0.9, 0], frequencies=FREQS,
+                               observation_points=2e9 * np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8],
+                                                                  [0.0, -0.6, 0.8], [0.48, 0.36, 0.8]]))]
     return kh.Simulation(cell_size=[2.0, 1.75, 1.5], resolution=8, boundaries=[[0.25, 0.25]] * 3, sources=srcs,
                          monitors=mons, precision=precision, dft_accum=dft_accum, mode=mode)
 
