@@ -346,6 +346,35 @@ int kh_dft_norm(kh_ctx *ctx, int32_t dft_id, double *out);
  * Replaces: reset_fields! (src/Simulation.jl:571-637). */
 int kh_reset(kh_ctx *ctx);
 
+/* Forward-field checkpoints for the time-reversed adjoint: every `every`
+ * steps of the run (from timestep 0) the whole field state is copied in
+ * device memory; 0 turns them off and frees them.  Only before the first
+ * step or right after kh_reset (KH_E_STATE otherwise).  Writing fields,
+ * permittivity or poles afterwards invalidates the checkpoints taken so far.
+ * Serves the reverse rule of simulate (src/Adjoint/ChainRulesIntegration.jl:
+ * 38-60), whose gradient the reference gets from a second, frequency-domain
+ * run instead (src/Adjoint/OptimizationProblem.jl:169-330). */
+int kh_set_checkpoints(kh_ctx *ctx, int32_t every);
+
+/* Gradient dJ/d(eps^-1) of a real objective J(F) of the listed DFT monitors
+ * (E or H components) after the steps run since timestep 0, by the discrete
+ * transpose of those steps: each checkpoint interval is re-run from its
+ * checkpoint (monitors off) and walked backwards through the transposed H/E
+ * half steps, CPML stages included.
+ * `cotangent`: per listed monitor in order, dJ/dRe(F) + i dJ/dIm(F) as
+ * (re, im) pairs in kh_read_dft's layout.  Design boxes, one per E component
+ * (a = 0..2): first grid index box_lo[3a..3a+2] (1-based, cells 1..N) and
+ * extent box_n[3a..3a+2] (any 0: component skipped).  `grad`: float64, the
+ * boxes in order, each
+ * dense with x fastest.  Real fields without dispersive, chi3 or conductive
+ * media and without periodic axes.  The fields, timestep and monitors are
+ * those of the forward run again on return.
+ * Replaces: the pullback of simulate (src/Adjoint/ChainRulesIntegration.jl:
+ * 38-60) with calculate_gradient! (src/Adjoint/GradientKernel.jl:35-109),
+ * as dJ/d(eps^-1) per voxel before the design-grid contraction. */
+int kh_adjoint_gradient(kh_ctx *ctx, int32_t n_monitors, const int32_t *monitor_ids, const double *cotangent,
+                        const int32_t box_lo[9], const int32_t box_n[9], double *grad);
+
 /* Kernel timing for measurement: while enabled (enable > 0), every update
  * and DFT launch is bracketed by HIP events on the context's stream; `enable`
  * event pairs are created up front so that timing allocates nothing while it

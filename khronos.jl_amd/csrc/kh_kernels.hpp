@@ -1095,6 +1095,15 @@ struct DftArgs {
     int64_t sy, sz;
 };
 
+// 2 pi t of a DFT sample at step ts: E monitors sample at t + dt/2, H at t;
+// time_fac = complex_backend_number(im * 2 * pi * time) (src/Monitors/Monitors.jl:287)
+template <typename T>
+__device__ __forceinline__ double dft_theta(int64_t ts, T dt, int electric) {
+    double t = step_time(ts, dt);
+    if (electric) t = t + (double)(dt / T(2));
+    return (double)(T)(6.283185307179586 * t);
+}
+
 // All DFT monitors of one half step in one launch; a monitor not due this
 // step (ts % decimation != 0) exits (src/Kernels/Kernels.jl:465-497).
 template <typename T, typename ACC>
@@ -1106,10 +1115,7 @@ __global__ __launch_bounds__(256) void dft_kernel(DftArgs<T> p) {
     const DftMon<T> &m = p.mon[q];
     const int64_t ts = *p.clk + p.koff;
     if (m.dec > 1 && (ts % m.dec) != 0) return;
-    // time_fac = complex_backend_number(im * 2 * pi * time) (src/Monitors/Monitors.jl:287)
-    double t = step_time(ts, p.dt);
-    if (p.electric) t = t + (double)(p.dt / T(2));
-    const double theta = (double)(T)(6.283185307179586 * t);
+    const double theta = dft_theta(ts, p.dt, p.electric);
     const double dts = (double)p.dt;
     for (int k = threadIdx.x; k < m.nf; k += blockDim.x) {
         double sn, cs;
@@ -2062,6 +2068,198 @@ __global__ __launch_bounds__(256) void smooth_kernel(const RasterObj *objs, int 
         val = 1.0 / e_avg;
     }
     out[t] = (T)val;
+}
+
+// ------------------------------------------------- time-reversed adjoint
+// Discrete transpose of one half step of the sigma-cascade update
+// (src/Kernels/Helpers.jl:113-139 curl stage, :323-340 material stage, the
+// curls of src/Kernels/Helpers.jl:286-298): with K = dt curl(F) and
+// (o, n, p) the own/next/prev-axis sigma of a component,
+//   U' = ((1-n) U + K) / (1+n)          (n != 0; else U' - U reads as K)
+//   T' = ((1-p) T + U' - U) / (1+p)
+//   W' = m T'                            (plus sources, which the transpose drops)
+//   A' = A + (1+o) W' - (1-o) W          (o != 0; else A' = W')
+// Run backwards, the adjoints (lA, lT, lU, lW) of the state after the half
+// step become those of the state before it, and Kbar = dJ/dK is written for
+// the other group's transposed curl.  That curl is read from the other
+// group's Kbar of the previous transposed half step: the transpose of a
+// forward-difference curl is minus the backward-difference curl of the same
+// shape, and the other way round.  Cells 1..N of every axis, as the forward
+// update; ghost entries of Kbar stay zero.
+template <typename T>
+struct AdjArgs {
+    T *lA[3], *lT[3], *lU[3], *lW[3];
+    T *Kout[3];
+    const T *Kin[3];
+    const T *sig[3];
+    const T *minv[3];
+    T minv_s, dt;
+    T inv[3];
+    int n[3];
+    int64_t sy, sz;
+    // E half step: dJ/d(eps^-1) over a box per component, from m times the
+    // forward net drive of the same step (dense over the box, x fastest)
+    int glo[3][3], gn[3][3];
+    const T *fwd[3];
+    double *grad[3];
+};
+
+template <typename T, bool ELEC>
+__global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncell = (int64_t)p.n[0] * p.n[1] * p.n[2];
+    if (c >= ncell) return;
+    const int gx = 1 + (int)(c % p.n[0]);
+    const int64_t r = c / p.n[0];
+    const int gy = 1 + (int)(r % p.n[1]);
+    const int gz = 1 + (int)(r / p.n[1]);
+    const int64_t sy = p.sy, sz = p.sz;
+    const size_t idx = (size_t)gx + (size_t)gy * sy + (size_t)gz * sz;
+    const T *Kx = p.Kin[0], *Ky = p.Kin[1], *Kz = p.Kin[2];
+    const T kx = Kx[idx], ky = Ky[idx], kz = Kz[idx];
+    T add[3];
+    if constexpr (ELEC) {
+        // minus the E update's backward-difference curl of the H group's Kbar
+        add[0] = -(p.dt * (p.inv[2] * (Ky[idx - sz] - ky) - p.inv[1] * (Kz[idx - sy] - kz)));
+        add[1] = -(p.dt * (p.inv[0] * (Kz[idx - 1] - kz) - p.inv[2] * (Kx[idx - sz] - kx)));
+        add[2] = -(p.dt * (p.inv[1] * (Kx[idx - sy] - kx) - p.inv[0] * (Ky[idx - 1] - ky)));
+    } else {
+        // minus the H update's forward-difference curl of the E group's Kbar
+        add[0] = -(p.dt * (p.inv[2] * (Ky[idx + sz] - ky) - p.inv[1] * (Kz[idx + sy] - kz)));
+        add[1] = -(p.dt * (p.inv[0] * (Kz[idx + 1] - kz) - p.inv[2] * (Kx[idx + sz] - kx)));
+        add[2] = -(p.dt * (p.inv[1] * (Kx[idx + sy] - kx) - p.inv[0] * (Ky[idx + 1] - ky)));
+    }
+    const T sg[3] = {p.sig[0][gx], p.sig[1][gy], p.sig[2][gz]};
+    const int gc[3] = {gx, gy, gz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T so = sg[a], sn = sg[(a + 1) % 3], sp = sg[(a + 2) % 3];
+        const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
+        const T ap = p.lA[a][idx] + add[a];
+        T wh;
+        if (so == T(0)) {
+            wh = ap;
+            p.lA[a][idx] = T(0);
+        } else {
+            wh = p.lW[a][idx] + (T(1) + so) * ap;
+            p.lW[a][idx] = -((T(1) - so) * ap);
+            p.lA[a][idx] = ap;
+        }
+        if constexpr (ELEC) {
+            // m enters as W' = m net: dJ/dm += What * net, with fwd = m net
+            // (A' where o = 0, W' where o != 0)
+            if (p.grad[a]) {
+                int bi = 0, str = 1;
+                bool in = true;
+                for (int k = 0; k < 3; ++k) {
+                    const int o = gc[k] - p.glo[a][k];
+                    in = in && o >= 0 && o < p.gn[a][k];
+                    bi += o * str;
+                    str *= p.gn[a][k];
+                }
+                if (in) p.grad[a][bi] += (double)wh * ((double)p.fwd[a][bi] / (double)m);
+            }
+        }
+        const T th = p.lT[a][idx] + m * wh;
+        p.lT[a][idx] = ((T(1) - sp) / (T(1) + sp)) * th;
+        const T dl = th / (T(1) + sp);
+        T kb;
+        if (sn == T(0)) {
+            kb = dl;
+        } else {
+            const T uh = p.lU[a][idx] + dl;
+            p.lU[a][idx] = ((T(1) - sn) / (T(1) + sn)) * uh - dl;
+            kb = uh / (T(1) + sn);
+        }
+        p.Kout[a][idx] = kb;
+    }
+}
+
+// Transpose of the DFT update at step ts: lambda += Re(conj(G_k) w_k) summed
+// over frequencies, with w_k = dt exp(i f_k theta) the weight the DFT kernel
+// applied and G = dJ/dRe(F) + i dJ/dIm(F) per accumulator entry (same
+// layout as the accumulator).  Cells 1..N only (ghost samples are constant).
+template <typename T>
+struct AdjSeed {
+    const double *g;
+    T *lam;
+    const double *freqs;
+    int s[3], d[3];
+    int nf, dec;
+    int first;
+};
+
+template <typename T>
+struct AdjSeedArgs {
+    int nmon;
+    AdjSeed<T> mon[MAX_DFT];
+    int64_t ts;
+    T dt;
+    int electric;
+    int n0, n1, n2;
+    int64_t sy, sz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void adjoint_seed_kernel(AdjSeedArgs<T> p) {
+    extern __shared__ double wsh[];
+    int q = 0;
+    const int b = blockIdx.x;
+    while (q + 1 < p.nmon && b >= p.mon[q + 1].first) ++q;
+    const AdjSeed<T> &m = p.mon[q];
+    if (m.dec > 1 && (p.ts % m.dec) != 0) return;
+    const double theta = dft_theta(p.ts, p.dt, p.electric);
+    const double dts = (double)p.dt;
+    for (int k = threadIdx.x; k < m.nf; k += blockDim.x) {
+        double sn, cs;
+        sincos(m.freqs[k] * theta, &sn, &cs);
+        wsh[2 * k] = dts * cs;
+        wsh[2 * k + 1] = dts * sn;
+    }
+    __syncthreads();
+    const int64_t ncell = (int64_t)m.d[0] * m.d[1] * m.d[2];
+    const int64_t cell = (int64_t)(b - m.first) * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    const int ix = (int)(cell % m.d[0]);
+    const int64_t r = cell / m.d[0];
+    const int iy = (int)(r % m.d[1]);
+    const int iz = (int)(r / m.d[1]);
+    const int gx = m.s[0] + ix, gy = m.s[1] + iy, gz = m.s[2] + iz;
+    if (gx < 1 || gy < 1 || gz < 1 || gx > p.n0 || gy > p.n1 || gz > p.n2) return;
+    double v = 0.0;
+    const double *gp = m.g + 2 * cell;
+    for (int k = 0; k < m.nf; ++k, gp += 2 * ncell) v += gp[0] * wsh[2 * k] + gp[1] * wsh[2 * k + 1];
+    const size_t fi = (size_t)gx + (size_t)gy * p.sy + (size_t)gz * p.sz;
+    m.lam[fi] = (T)((double)m.lam[fi] + v);
+}
+
+// Copy of a box of up to three fields (blockIdx.y = component), dense x
+// fastest; where the component's own-axis sigma is non-zero the W stage is
+// copied instead (both hold m times the net drive of the step there).
+template <typename T>
+struct BoxCopy {
+    const T *f[3];
+    const T *w[3];
+    const T *sig[3];
+    T *out[3];
+    int lo[3][3], n[3][3];
+    int64_t sy, sz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void box_copy_kernel(BoxCopy<T> p) {
+    const int a = blockIdx.y;
+    if (!p.out[a]) return;
+    const int64_t ncell = (int64_t)p.n[a][0] * p.n[a][1] * p.n[a][2];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int ix = (int)(c % p.n[a][0]);
+    const int64_t r = c / p.n[a][0];
+    const int iy = (int)(r % p.n[a][1]);
+    const int iz = (int)(r / p.n[a][1]);
+    const int g[3] = {p.lo[a][0] + ix, p.lo[a][1] + iy, p.lo[a][2] + iz};
+    const size_t fi = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+    p.out[a][c] = p.sig[a][g[a]] != T(0) ? p.w[a][fi] : p.f[a][fi];
 }
 
 }  // namespace kh

@@ -133,6 +133,9 @@ struct CtxBase {
     virtual void near2far_spectrum(const int32_t *ids, int ax, int n1, int n2, int k, int p1, int p2,
                                    const int32_t *bins, int n_bins, double *out) = 0;
     virtual void reset() = 0;
+    virtual void set_checkpoints(int every) = 0;
+    virtual void adjoint_gradient(int nmon, const int32_t *ids, const double *cot, const int32_t *lo,
+                                  const int32_t *n, double *grad) = 0;
     virtual void set_timing(int n) = 0;
     virtual void get_timing(double *out) = 0;
     virtual void *stream() = 0;
@@ -352,6 +355,7 @@ struct Ctx final : CtxBase {
 
     void set_eps_inv(int comp, const void *a) override {
         require(comp >= 0 && comp < 3, "component must be 0..2");
+        drop_checkpoints();
         require(a != nullptr, "eps_inv array is null");
         require(epsi[comp] != nullptr, "the context has no per-voxel permittivity (create it with eps_inv arrays)");
         HIPCHK(hipStreamSynchronize(st));
@@ -410,6 +414,7 @@ struct Ctx final : CtxBase {
         for (auto &e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto &e : pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto &b : owned) (void)hipFree(b.p);
+        for (void *p : ckpt) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -587,6 +592,7 @@ struct Ctx final : CtxBase {
         pole_io(id, which, comp, host, bytes, true);
     }
     void write_pole(int id, int which, int comp, const void *host, size_t bytes) override {
+        drop_checkpoints();
         pole_io(id, which, comp, const_cast<void *>(host), bytes, false);
     }
 
@@ -1252,11 +1258,14 @@ struct Ctx final : CtxBase {
         stepped = true;
         int64_t done = 0;
         while (done < n) {
+            if (ckpt_every > 0 && ts % ckpt_every == 0) save_checkpoint();
             begin_step();
             const bool src_now = active || last_active;
             // a graph starts on a multiple of the DFT period, so the due
-            // monitors of each captured position are the same on every replay
-            if (graph_len > 0 && !timing && !graph_failed && n - done >= graph_len && ts % dft_period == 0) {
+            // monitors of each captured position are the same on every replay;
+            // with checkpoints on, a graph block stays inside one interval
+            if (graph_len > 0 && !timing && !graph_failed && n - done >= graph_len && ts % dft_period == 0 &&
+                (ckpt_every <= 0 || ts % ckpt_every + graph_len <= ckpt_every)) {
                 if (!graph_exec || graph_src != src_now) {
                     if (!capture_graph()) graph_failed = true;
                 }
@@ -1327,6 +1336,7 @@ struct Ctx final : CtxBase {
 
     void write_field(int comp, const void *host, size_t bytes) override {
         require(comp >= 0 && comp < 6, "component out of range");
+        drop_checkpoints();
         int cnt[3];
         comp_count(g.n, comp, cnt);
         int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
@@ -1628,6 +1638,295 @@ struct Ctx final : CtxBase {
         *out = total;
     }
 
+    // ------------------------------------------- checkpoints and the adjoint
+    // Every ckpt_every steps of a run from timestep 0 the whole field state
+    // (E, H, D, B, the PML stages) is copied to a checkpoint; the adjoint
+    // re-runs each interval from its checkpoint and walks it backwards.
+    int ckpt_every = 0;
+    std::vector<void *> ckpt;
+    struct CkHost {
+        int64_t ts = -1;
+        bool active = true, last_active = false, valid = false;
+    };
+    std::vector<CkHost> ck_host;
+    int64_t clk_stage = 0;
+
+    std::vector<T *> state_arrays() const {
+        std::vector<T *> v;
+        for (int part = 0; part < nparts; ++part) {
+            const Part &f = fp[part];
+            for (int a = 0; a < 3; ++a) {
+                T *arrs[10] = {f.E[a], f.H[a], f.D[a], f.B[a], f.UD[a], f.UB[a], f.WD[a], f.WB[a], f.CD[a], f.CB[a]};
+                for (T *q : arrs)
+                    if (q) v.push_back(q);
+            }
+        }
+        return v;
+    }
+
+    void copy_state(void *dst, const void *src_state, bool save) {
+        const auto arrs = state_arrays();
+        const size_t fb = (size_t)g.total * sizeof(T);
+        for (size_t i = 0; i < arrs.size(); ++i) {
+            char *buf = (char *)(save ? dst : const_cast<void *>(src_state)) + i * fb;
+            if (save) HIPCHK(hipMemcpyAsync(buf, arrs[i] - g.front, fb, hipMemcpyDeviceToDevice, st));
+            else HIPCHK(hipMemcpyAsync(arrs[i] - g.front, buf, fb, hipMemcpyDeviceToDevice, st));
+        }
+    }
+
+    size_t state_bytes() const { return state_arrays().size() * (size_t)g.total * sizeof(T); }
+
+    void save_checkpoint() {
+        const size_t k = (size_t)(ts / ckpt_every);
+        while (ckpt.size() <= k) {
+            void *q = nullptr;
+            HIPCHK(hipMalloc(&q, state_bytes()));
+            ckpt.push_back(q);
+            ck_host.push_back(CkHost{});
+        }
+        copy_state(ckpt[k], nullptr, true);
+        ck_host[k] = CkHost{ts, active, last_active, true};
+    }
+
+    // state written from outside the time step: a re-run from an earlier
+    // checkpoint would no longer reproduce the run, so none is kept (the
+    // checkpoint of the current timestep is taken when stepping resumes)
+    void drop_checkpoints() {
+        for (auto &c : ck_host) c.valid = false;
+    }
+
+    void restore_state(const void *src, const CkHost &h) {
+        copy_state(nullptr, src, false);
+        clk_stage = h.ts;
+        HIPCHK(hipMemcpyAsync(clk, &clk_stage, sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        ts = h.ts;
+        active = h.active;
+        last_active = h.last_active;
+    }
+
+    void set_checkpoints(int every) override {
+        require(every >= 0, "checkpoint interval must be >= 0");
+        require(every == 0 || ts == 0, "enable checkpoints before the first step or after kh_reset", KH_E_STATE);
+        HIPCHK(hipStreamSynchronize(st));
+        for (auto &c : ck_host) c.valid = false;
+        if (every == 0) {
+            for (void *q : ckpt) HIPCHK(hipFree(q));
+            ckpt.clear();
+            ck_host.clear();
+        }
+        ckpt_every = every;
+    }
+
+    // adjoint state, per group (0: H, 1: E) and component: the adjoints of
+    // A (E/H), T (D/B), U, W and Kbar; allocated on the first adjoint call
+    T *adjA[2][3]{}, *adjT[2][3]{}, *adjU[2][3]{}, *adjW[2][3]{}, *adjK[2][3]{};
+    void *adj_final = nullptr;
+    double *adj_cot = nullptr, *adj_grad = nullptr;
+    T *adj_seg = nullptr;
+    size_t adj_final_cap = 0, adj_cot_cap = 0, adj_grad_cap = 0, adj_seg_cap = 0;
+
+    void launch_adj_seed(const std::vector<const Dft *> &mons, const std::vector<size_t> &off, bool electric,
+                         int64_t j) {
+        std::vector<size_t> list;
+        for (size_t i = 0; i < mons.size(); ++i)
+            if ((mons[i]->comp < 3) == electric && (mons[i]->dec <= 1 || j % mons[i]->dec == 0)) list.push_back(i);
+        for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
+            kh::AdjSeedArgs<T> q{};
+            int blocks = 0, nfmax = 1;
+            for (size_t i = i0; i < list.size() && q.nmon < kh::MAX_DFT; ++i) {
+                const Dft &m = *mons[list[i]];
+                kh::AdjSeed<T> &x = q.mon[q.nmon++];
+                x.g = adj_cot + off[list[i]];
+                x.lam = adjA[electric ? 1 : 0][m.comp % 3];
+                x.freqs = m.freqs;
+                for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
+                x.nf = m.nf;
+                x.dec = m.dec;
+                x.first = blocks;
+                blocks += (int)((m.ncell + 255) / 256);
+                nfmax = std::max(nfmax, m.nf);
+            }
+            q.ts = j;
+            q.dt = dt;
+            q.electric = electric ? 1 : 0;
+            q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
+            q.sy = g.sy();
+            q.sz = g.sz();
+            if (blocks == 0) continue;
+            hipLaunchKernelGGL((kh::adjoint_seed_kernel<T>), dim3(blocks), dim3(256), (size_t)nfmax * 2 * sizeof(double),
+                               st, q);
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    void launch_adj_half(int e, const int (*glo)[3], const int (*gn)[3], const size_t *goff, const T *seg) {
+        kh::AdjArgs<T> q{};
+        for (int a = 0; a < 3; ++a) {
+            q.lA[a] = adjA[e][a];
+            q.lT[a] = adjT[e][a];
+            q.lU[a] = adjU[e][a];
+            q.lW[a] = adjW[e][a];
+            q.Kout[a] = adjK[e][a];
+            q.Kin[a] = adjK[1 - e][a];
+            q.sig[a] = e ? sigD[a] : sigB[a];
+            q.minv[a] = e ? epsi[a] : mui[a];
+            q.inv[a] = T(1) / dx[a];
+            q.n[a] = g.n[a];
+            if (e) {
+                const bool has = gn[a][0] > 0 && gn[a][1] > 0 && gn[a][2] > 0;
+                for (int k = 0; k < 3; ++k) { q.glo[a][k] = glo[a][k]; q.gn[a][k] = gn[a][k]; }
+                q.fwd[a] = has ? seg + goff[a] : nullptr;
+                q.grad[a] = has ? adj_grad + goff[a] : nullptr;
+            }
+        }
+        q.minv_s = e ? eps_s : mu_s;
+        q.dt = dt;
+        q.sy = g.sy();
+        q.sz = g.sz();
+        const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
+        const unsigned blocks = (unsigned)((ncell + 255) / 256);
+        if (e) hipLaunchKernelGGL((kh::adjoint_half_kernel<T, true>), dim3(blocks), dim3(256), 0, st, q);
+        else hipLaunchKernelGGL((kh::adjoint_half_kernel<T, false>), dim3(blocks), dim3(256), 0, st, q);
+        HIPCHK(hipGetLastError());
+    }
+
+    // dJ/d(eps^-1) of a real objective J of DFT monitors, by the discrete
+    // transpose of the steps run since timestep 0 (checkpointed), for the
+    // three E components over one box each.
+    void adjoint_gradient(int nmon, const int32_t *ids, const double *cot, const int32_t *lo, const int32_t *bn,
+                          double *grad) override {
+        require(ckpt_every > 0, "checkpoints are off: call kh_set_checkpoints before the forward run", KH_E_STATE);
+        require(ts > 0, "no steps to differentiate", KH_E_STATE);
+        require(nparts == 1, "the adjoint needs real fields (no complex / Bloch fields)");
+        require(!mat, "the adjoint does not cover dispersive or chi3 media");
+        for (int c = 0; c < 6; ++c) require(!msig[c], "the adjoint does not cover conductive media");
+        for (int a = 0; a < 3; ++a) require(!periodic[a], "the adjoint does not cover periodic boundaries");
+        require(nmon >= 0 && (nmon == 0 || (ids && cot)), "null monitor list or cotangent");
+        const int64_t N = ts, K = ckpt_every;
+        const int64_t nseg = (N + K - 1) / K;
+        require((int64_t)ck_host.size() >= nseg, "a checkpoint of this run is missing", KH_E_STATE);
+        for (int64_t k = 0; k < nseg; ++k)
+            require(ck_host[k].valid && ck_host[k].ts == k * K,
+                    "a checkpoint of this run is missing (the run did not start at timestep 0 with checkpoints on, "
+                    "or state was written during it)",
+                    KH_E_STATE);
+        std::vector<const Dft *> mons;
+        std::vector<size_t> off;
+        size_t cot_len = 0;
+        for (int i = 0; i < nmon; ++i) {
+            require(ids[i] >= 0 && ids[i] < (int)dfts.size(), "monitor id out of range");
+            const Dft &m = dfts[ids[i]];
+            require(m.comp < 6, "the adjoint takes objectives of E and H monitors (not D or B)");
+            mons.push_back(&m);
+            off.push_back(cot_len);
+            cot_len += (size_t)m.ncell * m.nf * 2;
+        }
+        // design boxes: inside cells 1..N
+        HIPCHK(hipStreamSynchronize(st));
+        int glo[3][3], gn[3][3];
+        size_t goff[3], gtot = 0;
+        for (int a = 0; a < 3; ++a) {
+            bool has = true;
+            for (int k = 0; k < 3; ++k) {
+                glo[a][k] = lo ? lo[3 * a + k] : 1;
+                gn[a][k] = bn ? bn[3 * a + k] : 0;
+                require(gn[a][k] >= 0, "design box extent must be >= 0");
+                has = has && gn[a][k] > 0;
+            }
+            goff[a] = gtot;
+            if (!has) {
+                for (int k = 0; k < 3; ++k) gn[a][k] = 0;
+                continue;
+            }
+            for (int k = 0; k < 3; ++k)
+                require(glo[a][k] >= 1 && glo[a][k] + gn[a][k] - 1 <= g.n[k], "design box outside cells 1..N");
+            gtot += (size_t)gn[a][0] * gn[a][1] * gn[a][2];
+        }
+        if (!adjA[0][0]) {
+            for (int e = 0; e < 2; ++e)
+                for (int a = 0; a < 3; ++a) {
+                    adjA[e][a] = field_alloc();
+                    adjT[e][a] = field_alloc();
+                    adjU[e][a] = field_alloc();
+                    adjW[e][a] = field_alloc();
+                    adjK[e][a] = field_alloc();
+                }
+        }
+        const size_t fb = (size_t)g.total * sizeof(T);
+        for (int e = 0; e < 2; ++e)
+            for (int a = 0; a < 3; ++a) {
+                T *arrs[5] = {adjA[e][a], adjT[e][a], adjU[e][a], adjW[e][a], adjK[e][a]};
+                for (T *q : arrs) HIPCHK(hipMemsetAsync(q - g.front, 0, fb, st));
+            }
+        adj_cot = (double *)regrow(adj_cot, adj_cot_cap, std::max<size_t>(cot_len, 1) * sizeof(double));
+        if (cot_len) HIPCHK(hipMemcpyAsync(adj_cot, cot, cot_len * sizeof(double), hipMemcpyHostToDevice, st));
+        adj_grad = (double *)regrow(adj_grad, adj_grad_cap, std::max<size_t>(gtot, 1) * sizeof(double));
+        HIPCHK(hipMemsetAsync(adj_grad, 0, std::max<size_t>(gtot, 1) * sizeof(double), st));
+        adj_seg = (T *)regrow(adj_seg, adj_seg_cap, std::max<size_t>(gtot * (size_t)K, 1) * sizeof(T));
+        adj_final = regrow(adj_final, adj_final_cap, state_bytes());
+        copy_state(adj_final, nullptr, true);
+        const CkHost fin{ts, active, last_active, true};
+        HIPCHK(hipStreamSynchronize(st));
+
+        // re-runs skip the monitors; the forward state and monitors are put
+        // back however the sweep ends
+        std::vector<Dft> saved;
+        saved.swap(dfts);
+        drop_graph();
+        struct Back {
+            Ctx *c;
+            std::vector<Dft> *saved;
+            const CkHost *fin;
+            ~Back() {
+                try {
+                    c->restore_state(c->adj_final, *fin);
+                } catch (...) {
+                }
+                c->dfts.swap(*saved);
+                c->plan_graph_len();
+                c->drop_graph();
+            }
+        } back{this, &saved, &fin};
+
+        kh::BoxCopy<T> bc{};
+        unsigned bmax = 1;
+        for (int a = 0; a < 3; ++a) {
+            bc.f[a] = fp[0].E[a];
+            bc.w[a] = fp[0].WD[a];
+            bc.sig[a] = sigD[a];
+            for (int k = 0; k < 3; ++k) { bc.lo[a][k] = glo[a][k]; bc.n[a][k] = gn[a][k]; }
+            const size_t cnt = (size_t)gn[a][0] * gn[a][1] * gn[a][2];
+            bmax = std::max(bmax, (unsigned)((cnt + 255) / 256));
+        }
+        bc.sy = g.sy();
+        bc.sz = g.sz();
+        for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
+            const int64_t s0 = sgi * K, s1 = std::min(N, s0 + K);
+            restore_state(ckpt[(size_t)sgi], ck_host[(size_t)sgi]);
+            for (int64_t j = s0; j < s1; ++j) {
+                begin_step();
+                launch_step(0);
+                launch_tick(1);
+                end_step();
+                if (gtot) {
+                    for (int a = 0; a < 3; ++a)
+                        bc.out[a] = gn[a][0] > 0 ? adj_seg + (size_t)(j - s0) * gtot + goff[a] : nullptr;
+                    hipLaunchKernelGGL((kh::box_copy_kernel<T>), dim3(bmax, 3), dim3(256), 0, st, bc);
+                    HIPCHK(hipGetLastError());
+                }
+            }
+            for (int64_t j = s1 - 1; j >= s0; --j) {
+                launch_adj_seed(mons, off, true, j);
+                launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)(j - s0) * gtot);
+                launch_adj_seed(mons, off, false, j);
+                launch_adj_half(0, glo, gn, goff, nullptr);
+            }
+        }
+        if (gtot) HIPCHK(hipMemcpyAsync(grad, adj_grad, gtot * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
     void reset() override {
         HIPCHK(hipStreamSynchronize(st));
         size_t fb = (size_t)g.total * sizeof(T);
@@ -1652,6 +1951,7 @@ struct Ctx final : CtxBase {
         clear_material_state();
         HIPCHK(hipMemsetAsync(clk, 0, 256, st));
         HIPCHK(hipStreamSynchronize(st));
+        for (auto &c : ck_host) c.valid = false;
         ts = 0;
         active = true;
         last_active = false;
@@ -2144,6 +2444,22 @@ int kh_reset(kh_ctx *ctx) {
     return guarded([&] {
         CTX_OK;
         ctx->impl->reset();
+    });
+}
+
+int kh_set_checkpoints(kh_ctx *ctx, int32_t every) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->set_checkpoints(every);
+    });
+}
+
+int kh_adjoint_gradient(kh_ctx *ctx, int32_t n_monitors, const int32_t *monitor_ids, const double *cotangent,
+                        const int32_t box_lo[9], const int32_t box_n[9], double *grad) {
+    return guarded([&] {
+        CTX_OK;
+        require(box_lo != nullptr && box_n != nullptr, "null design box");
+        ctx->impl->adjoint_gradient(n_monitors, monitor_ids, cotangent, box_lo, box_n, grad);
     });
 }
 

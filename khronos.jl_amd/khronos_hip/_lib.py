@@ -82,6 +82,9 @@ SIGNATURES = {
                           C.c_double, C.c_void_p]),
     "kh_dft_norm": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "kh_reset": (C.c_int, [C.c_void_p]),
+    "kh_set_checkpoints": (C.c_int, [C.c_void_p, C.c_int32]),
+    "kh_adjoint_gradient": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     "kh_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "kh_get_timing": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kh_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -22,6 +22,13 @@ interpolation (src/Adjoint/GradientKernel.jl:35-109).
 Both runs step on the GPU through libkhronos_hip (the engine of the
 simulation); the objective's derivative uses torch autograd on the real and
 imaginary parts, as the reference uses ForwardDiff.
+
+``method="time_reversed"`` replaces the second run: the forward run keeps
+field checkpoints every ``checkpoint_every`` steps on the device, and the
+library's discrete transpose of those steps (``kh_adjoint_gradient``) returns
+dJ/d(eps^-1) per voxel of the design region, exact for the discrete objective
+(multi-frequency objectives included), which is then contracted onto the
+design grid with the same interpolation weights.
 """
 from __future__ import annotations
 
@@ -217,7 +224,7 @@ class OptimizationProblem:
                  objective_arguments: Sequence[FourierFieldsObjective], design_regions: Sequence[DesignRegion],
                  frequencies: Sequence[float], decay_by: float = 1e-6, minimum_run_time: float = 0.0,
                  maximum_run_time: float = math.inf, until: Optional[float] = None,
-                 reference_scale: bool = False):
+                 reference_scale: bool = False, method: str = "frequency", checkpoint_every: int = 100):
         self.sim = sim
         self.objective_functions = list(objective_functions)
         self.objective_arguments = list(objective_arguments)
@@ -228,6 +235,13 @@ class OptimizationProblem:
         self.maximum_run_time = maximum_run_time
         self.until = until
         self.reference_scale = reference_scale
+        if method not in ("frequency", "time_reversed"):
+            raise ValueError("method must be 'frequency' or 'time_reversed'")
+        if method == "time_reversed" and int(checkpoint_every) < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.method = method
+        self.checkpoint_every = int(checkpoint_every)
+        self._fwd_sim = None
         self.base = sim.build_spec()
         R = self.base.real
         for a, C in enumerate((G.Ex, G.Ey, G.Ez)):
@@ -253,8 +267,9 @@ class OptimizationProblem:
                                       frequencies=self.frequencies))
         return out
 
-    def _run(self, spec):
-        """Step a copy of the simulation carrying ``spec`` on its engine."""
+    def _run(self, spec, checkpoints: int = 0):
+        """Step a copy of the simulation carrying ``spec`` on its engine
+        (with field checkpoints every ``checkpoints`` steps if > 0)."""
         s = copy.copy(self.sim)
         s.spec = spec
         s.source_data = spec.sources
@@ -268,6 +283,8 @@ class OptimizationProblem:
         s.is_prepared = True
         for md in spec.dfts:
             md.owner = s
+        if checkpoints > 0:
+            s.engine.set_checkpoints(checkpoints)
         if self.until is not None:
             run(s, until=self.until)
         else:
@@ -302,9 +319,11 @@ class OptimizationProblem:
             oq.monitor = DFTMonitor(component=oq.component, center=list(oq.volume.center),
                                     size=list(oq.volume.size), frequencies=self.frequencies)
             obj_mons.append(oq.monitor)
-        design = self._design_monitors()
+        tr = self.method == "time_reversed"
+        design = [] if tr else self._design_monitors()
         spec = self._spec_with(self.base.sources, obj_mons + design, rho=self.rho)
-        s = self._run(spec)
+        self._close_forward()
+        s = self._run(spec, checkpoints=self.checkpoint_every if tr else 0)
         n_obj = len(obj_mons)
         self.results_list = []
         for i, oq in enumerate(self.objective_arguments):
@@ -314,7 +333,10 @@ class OptimizationProblem:
         self.fwd_design = [s.engine.read_dft(n_obj + j) for j in range(len(design))]
         self._t_end = s.round_time()
         self._fwd_spec = spec
-        s.engine.close() if hasattr(s.engine, "close") else None
+        if tr:
+            self._fwd_sim = s   # its checkpoints serve the adjoint sweep
+        else:
+            s.engine.close() if hasattr(s.engine, "close") else None
         f0 = [_eval_objective(f, self.results_list) for f in self.objective_functions]
         self.f0 = f0[0] if len(f0) == 1 else f0
         self.f_bank.append(self.f0)
@@ -378,9 +400,20 @@ class OptimizationProblem:
                 srcs.append(SourceData(component=oq.component, gv=oq.md.gv, amplitude=amp, time_profile=fwd_tp))
         return srcs
 
+    def _close_forward(self):
+        if self._fwd_sim is not None:
+            eng = self._fwd_sim.engine
+            eng.close() if hasattr(eng, "close") else None
+            self._fwd_sim = None
+
     def adjoint_run(self):
         """src/Adjoint/OptimizationProblem.jl:169-330: one adjoint run per
-        frequency (each reads its own frequency's D fields)."""
+        frequency (each reads its own frequency's D fields).  The
+        time-reversed method has no separate run: its sweep happens in
+        ``calculate_gradient``."""
+        if self.method == "time_reversed":
+            self.current_state = "ADJ"
+            return
         self.adj_design = []
         for fi in range(len(self.frequencies)):
             srcs = self._adjoint_sources(fi)
@@ -393,8 +426,14 @@ class OptimizationProblem:
             s.engine.close() if hasattr(s.engine, "close") else None
         self.current_state = "ADJ"
 
-    def calculate_gradient(self):
-        """src/Adjoint/GradientKernel.jl:35-109 (first design region)."""
+    def calculate_gradient(self, weights=None):
+        """src/Adjoint/GradientKernel.jl:35-109 (first design region):
+        (n_design, n_freq), one column per frequency.  Time-reversed method:
+        (n_design, 1), the gradient of sum_i weights[i] * objective_i (all
+        weights 1 by default) over all frequencies at once."""
+        if self.method == "time_reversed":
+            self.gradient = self._time_reversed_gradient(weights)[:, None]
+            return self.gradient
         dr = self.design_regions[0]
         nd = len(dr.design_parameters)
         nf = len(self.frequencies)
@@ -404,6 +443,63 @@ class OptimizationProblem:
                 grad[:, fi] = self._gradient_column(dr, fi)
         self.gradient = grad
         return grad
+
+    def design_boxes(self, dr):
+        """Per E component, the grid-index box ((lo), (n)) that update_design
+        writes (clamped to cells 1..N and the array), or None."""
+        N = self.base.N
+        boxes = []
+        for C in (G.Ex, G.Ey, G.Ez):
+            gv = dr.gv[C.name]
+            cnt = G.component_voxel_count(self.sim.Nx, self.sim.Ny, self.sim.Nz, C())
+            st = gv.start_idx
+            en = [st[0] + gv.Nx - 1, st[1] + gv.Ny - 1, st[2] + gv.Nz - 1]
+            lo = [max(st[k], 1) for k in range(3)]
+            hi = [min(en[k], N[k], cnt[k]) for k in range(3)]
+            boxes.append(None if any(hi[k] < lo[k] for k in range(3))
+                         else (lo, [hi[k] - lo[k] + 1 for k in range(3)]))
+        return boxes
+
+    def _time_reversed_gradient(self, weights=None):
+        """dJ/drho from the library's dJ/d(eps^-1) per design voxel: summed
+        over the extruded z range, times d(eps^-1)/dr = -(eps_max - eps_min) /
+        eps^2, through the transpose of the bilinear interpolation."""
+        if self._fwd_sim is None:
+            raise RuntimeError("no forward run to differentiate (call forward_run first)")
+        dr = self.design_regions[0]
+        funcs = self.objective_functions
+        w = np.ones(len(funcs)) if weights is None else np.atleast_1d(np.asarray(weights, dtype=np.float64))
+        if len(w) != len(funcs):
+            raise ValueError("one weight per objective function")
+        cots = []
+        for mi, oq in enumerate(self.objective_arguments):
+            acc = np.zeros(oq.eval_data.shape, dtype=np.complex128)
+            for fw, func in zip(w, funcs):
+                dJ = wirtinger_derivative(func, self.results_list, mi)
+                acc += fw * 2.0 * dJ.reshape(oq.eval_data.shape, order="C")
+            cots.append(acc)
+        boxes = self.design_boxes(dr)
+        grads = self._fwd_sim.engine.adjoint_gradient(list(range(len(cots))), cots, boxes)
+        de = dr.ε_max - dr.ε_min
+        out = np.zeros(len(dr.design_parameters))
+        for a, C in enumerate((G.Ex, G.Ey, G.Ez)):
+            if grads[a] is None:
+                continue
+            gv = dr.gv[C.name]
+            lo, n = boxes[a]
+            st = gv.start_idx
+            r = interpolate_to_yee(dr.design_parameters, dr.weights[C.name], gv.Nx * gv.Ny)
+            deps_inv = -de / (dr.ε_min + r * de) ** 2
+            gxy = grads[a].sum(axis=2)
+            ix = np.arange(n[0]) + (lo[0] - st[0])
+            iy = np.arange(n[1]) + (lo[1] - st[1])
+            yi = iy[None, :] * gv.Nx + ix[:, None]
+            gy = np.zeros(gv.Nx * gv.Ny)
+            np.add.at(gy, yi.ravel(), gxy.ravel())
+            gy *= deps_inv
+            yee, des, wts = dr.weights[C.name]
+            np.add.at(out, des, gy[yee] * wts)
+        return out
 
     def _gradient_column(self, dr, fi):
         nd = len(dr.design_parameters)
@@ -481,8 +577,11 @@ def simulate_torch(opt: OptimizationProblem, designs):
         @staticmethod
         def backward(ctx, g):
             opt.adjoint_run()
-            grad = opt.calculate_gradient()  # (n_design, n_freq)
             gw = g.detach().cpu().numpy()
+            if opt.method == "time_reversed":
+                # one sweep for the cotangent-weighted sum of the objectives
+                return torch.as_tensor(opt.calculate_gradient(weights=gw.ravel())[:, 0])
+            grad = opt.calculate_gradient()  # (n_design, n_freq)
             contracted = grad.sum(axis=1) * gw[0] if gw.size == 1 else grad @ gw
             return torch.as_tensor(contracted)
 

@@ -292,6 +292,54 @@ class HipEngine:
         _lib.check(self.lib.kh_reset(self.ctx))
         self.ts = 0
 
+    # ------------------------------------------------------------- adjoint
+    def set_checkpoints(self, every: int):
+        """Checkpoint the field state every ``every`` steps of the coming run
+        (0: off), for ``adjoint_gradient``."""
+        _lib.check(self.lib.kh_set_checkpoints(self.ctx, int(every)))
+
+    def adjoint_gradient(self, monitors, cotangents, boxes):
+        """dJ/d(eps^-1) by the time-reversed adjoint of the steps run so far.
+
+        monitors: indices into ``spec.dfts``; cotangents: per monitor, a
+        complex array shaped like ``read_dft(i)`` holding dJ/dRe F + i dJ/dIm F;
+        boxes: per E component, None or ((lo_x, lo_y, lo_z), (n_x, n_y, n_z))
+        in 1-based grid indices.  Returns per component None or the gradient
+        as a float64 array of shape n (Fortran order)."""
+        ids = np.array([self.dft_ids[i] for i in monitors] or [0], dtype=np.int32)
+        parts = []
+        for i, g in zip(monitors, cotangents):
+            md = self.spec.dfts[i]
+            shp = tuple(md.gv.dims) + (len(md.frequencies),)
+            z = np.asarray(g, dtype=np.complex128).reshape(shp)
+            flat = z.ravel(order="F")
+            pair = np.empty(2 * flat.size, dtype=np.float64)
+            pair[0::2] = flat.real
+            pair[1::2] = flat.imag
+            parts.append(pair)
+        cot = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1))
+        lo = np.ones(9, dtype=np.int32)
+        n = np.zeros(9, dtype=np.int32)
+        total = 0
+        for a, b in enumerate(boxes):
+            if b is None:
+                continue
+            lo[3 * a:3 * a + 3] = b[0]
+            n[3 * a:3 * a + 3] = b[1]
+            total += int(np.prod(b[1]))
+        grad = np.zeros(max(total, 1), dtype=np.float64)
+        _lib.check(self.lib.kh_adjoint_gradient(self.ctx, len(monitors), _ptr(ids), _ptr(cot), _ptr(lo), _ptr(n),
+                                                _ptr(grad)))
+        out, o = [], 0
+        for b in boxes:
+            if b is None or int(np.prod(b[1])) == 0:
+                out.append(None)
+                continue
+            c = int(np.prod(b[1]))
+            out.append(grad[o:o + c].reshape(tuple(b[1]), order="F"))
+            o += c
+        return out
+
     # -------------------------------------------------------------- timing
     def set_timing(self, on: bool):
         _lib.check(self.lib.kh_set_timing(self.ctx, 1 if on else 0))

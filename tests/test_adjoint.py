@@ -42,7 +42,8 @@ def slab_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.0,
     return A.OptimizationProblem(sim, [f], [obj], [dr], list(freqs), until=until)
 
 
-def slab2d_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.0, nd=(6, 4), freqs=(0.5,)):
+def slab2d_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.0, nd=(6, 4), freqs=(0.5,),
+                   **sim_kw):
     """The slab problem as a true 2D simulation (cell_size z = 0, dz = Inf),
     the setting of the reference's 2D adjoint tests (test/test_adjoint_gradient.jl:79-84)."""
     pml, pad, lx, cy = 1.0, 0.5, 1.0, 2.6
@@ -51,7 +52,8 @@ def slab2d_problem(engine=OracleEngine, precision="f64", mode="exact", until=60.
     src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ez(),
                             center=[-(lx / 2 + pad), 0.0, 0.0], size=[0.0, cy - 2 * pml, 0.0])]
     sim = kh.Simulation(cell_size=[cx, cy, 0.0], resolution=10, sources=src, geometry=geo,
-                        boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode, engine=engine)
+                        boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode, engine=engine,
+                        **sim_kw)
     dr = A.DesignRegion(volume=Volume([0, 0, 0], [lx, cy - 2 * pml, 0.0]),
                         design_parameters=np.full(nd[0] * nd[1], 0.5), grid_size=nd, ε_min=1.0, ε_max=4.0)
     obj = A.FourierFieldsObjective(volume=Volume([lx / 2 + pad / 2, 0, 0], [0, cy - 2 * pml, 0.0]), component=kh.Ez())

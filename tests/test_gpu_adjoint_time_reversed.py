@@ -1,0 +1,196 @@
+"""Time-reversed adjoint on the GPU (kh_set_checkpoints + kh_adjoint_gradient).
+
+The gradient is the exact derivative of the discrete objective, so it is
+pinned by central finite differences of device forward runs to far below the
+reference's own acceptance (median < 10 %, test/test_adjoint_gradient.jl:255):
+float64 directional derivatives agree to 1e-6 relative, in 2D (PML on x, y)
+and in 3D with PML on every axis (every sigma combination of the cascade,
+all three E components in the design box), in FAST and EXACT modes.  The
+checkpoint interval changes nothing (bit-identical gradients), the forward
+state and monitors are back on return, the torch reverse rule returns the
+same gradient, the frequency-domain adjoint agrees to its own accuracy, and
+float32 agrees with float64.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import khronos_hip as kh
+from khronos_hip import adjoint as A
+from khronos_hip.grid import Volume
+from khronos_hip._lib import KhronosError
+from test_adjoint import slab2d_problem, slab_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _time_reversed(opt, every):
+    opt.method = "time_reversed"
+    opt.checkpoint_every = every
+    return opt
+
+
+def box3d_problem(precision="f64", mode="fast", until=30.0, freqs=(0.45, 0.55)):
+    """3D box with PML on all six faces: Ez sheet source, 5x4 design block,
+    Ez/Ey objective plane behind it."""
+    pml = 0.5
+    cell = [3.0, 2.6, 1.6]
+    geo = [kh.Object(kh.Box([0, 0, 0], [10.0, 10.0, 10.0]), kh.Material(ε=1.0))]
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ez(),
+                            center=[-0.9, 0.0, 0.0], size=[0.0, 1.2, 0.5])]
+    sim = kh.Simulation(cell_size=cell, resolution=10, sources=src, geometry=geo,
+                        boundaries=[[pml, pml], [pml, pml], [pml, pml]], precision=precision, mode=mode)
+    dr = A.DesignRegion(volume=Volume([0, 0, 0], [1.0, 1.2, 0.5]), design_parameters=np.full(20, 0.5),
+                        grid_size=(5, 4), ε_min=1.0, ε_max=4.0)
+    objs = [A.FourierFieldsObjective(volume=Volume([0.8, 0, 0], [0, 1.2, 0.5]), component=kh.Ez()),
+            A.FourierFieldsObjective(volume=Volume([0.8, 0, 0], [0, 1.2, 0.5]), component=kh.Ey())]
+    f = lambda E1, E2: torch.sum(torch.abs(E1) ** 2) + 0.5 * torch.sum(torch.abs(E2) ** 2)
+    return A.OptimizationProblem(sim, [f], objs, [dr], list(freqs), until=until)
+
+
+def _directional(opt, rho, n_dir=3, h=1e-4, seed=7):
+    f0, g = opt(rho)
+    g = g[:, 0]
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_dir):
+        d = rng.standard_normal(len(rho))
+        d /= np.linalg.norm(d)
+        fp, _ = opt(rho + h * d, need_gradient=False)
+        fm, _ = opt(rho - h * d, need_gradient=False)
+        fd = (fp - fm) / (2 * h)
+        ad = float(g @ d)
+        out.append((ad, fd, abs(ad - fd) / max(abs(ad), abs(fd), 1e-300)))
+    return out, g
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_2d_gradient_equals_finite_differences(mode):
+    opt = _time_reversed(slab2d_problem(engine=None, precision="f64", mode=mode, freqs=(0.45, 0.55)), 37)
+    rho = 0.3 + 0.4 * np.random.default_rng(42).random(24)
+    _note(f"[adjoint-tr] 2D {mode}")
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] 2D {mode} (adjoint, fd, rel): {res}")
+    assert np.any(g != 0)
+    assert max(r[2] for r in res) < 1e-6, res
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_3d_pml_every_axis_gradient_equals_finite_differences(mode):
+    opt = _time_reversed(box3d_problem(mode=mode), 50)
+    rho = 0.3 + 0.4 * np.random.default_rng(5).random(20)
+    _note(f"[adjoint-tr] 3D {mode}")
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] 3D {mode} (adjoint, fd, rel): {res}")
+    assert max(r[2] for r in res) < 1e-6, res
+
+
+def test_3d_per_pixel_finite_differences():
+    """The reference's check: central differences at sampled pixels
+    (src/Adjoint/GradientKernel.jl:117-152)."""
+    opt = _time_reversed(box3d_problem(), 64)
+    rho = 0.3 + 0.4 * np.random.default_rng(11).random(20)
+    _, g = opt(rho)
+    fd, idx = opt.calculate_fd_gradient(rho, db=1e-4, n_samples=6, seed=3)
+    rel = np.abs(fd[:, 0] - g[idx, 0]) / np.max(np.abs(g))
+    _note(f"[adjoint-tr] per-pixel rel errors {rel}")
+    assert np.max(rel) < 1e-6, (fd[:, 0], g[idx, 0])
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    rho = 0.3 + 0.4 * np.random.default_rng(2).random(24)
+    grads = []
+    for every in (1000000, 100, 7):
+        opt = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast"), every)
+        grads.append(opt(rho)[1])
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def test_forward_state_and_monitors_restored():
+    opt = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast"), 50)
+    rho = 0.3 + 0.4 * np.random.default_rng(4).random(24)
+    opt.update_design(rho)
+    opt.forward_run()
+    eng = opt._fwd_sim.engine
+    ts = eng.timestep()
+    before = [eng.read_dft(0)] + [eng.read_field(c) for c in range(6)]
+    opt.adjoint_run()
+    opt.calculate_gradient()
+    after = [eng.read_dft(0)] + [eng.read_field(c) for c in range(6)]
+    assert eng.timestep() == ts
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    # the restored context keeps stepping as the original run would
+    eng.step(10)
+    ref = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast"), 50)
+    ref.update_design(rho)
+    ref.forward_run()
+    ref._fwd_sim.engine.step(10)
+    assert np.array_equal(eng.read_dft(0), ref._fwd_sim.engine.read_dft(0))
+
+
+def test_torch_reverse_rule_time_reversed():
+    opt = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.55)), 40)
+    rho = torch.tensor(0.3 + 0.4 * np.random.default_rng(1).random(24), requires_grad=True)
+    f = A.simulate_torch(opt, rho)
+    (3.0 * f.sum()).backward()
+    assert opt.gradient.shape == (24, 1)
+    # the sweep ran with the cotangent as objective weight; an unweighted
+    # sweep of the same forward run gives a third of it
+    g1 = opt.calculate_gradient()[:, 0]
+    assert np.allclose(rho.grad.numpy(), 3.0 * g1, rtol=1e-12, atol=0)
+
+
+def test_agrees_with_frequency_domain_adjoint():
+    """Both gradients of the same objective: the frequency-domain one is
+    approximate for a finite run (its FD acceptance here is a 3 % median),
+    the time-reversed one exact."""
+    rho = 0.3 + 0.4 * np.random.default_rng(8).random(24)
+    fr = slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.55))
+    g_f = fr(rho)[1].sum(axis=1)
+    tr = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.55)), 100)
+    g_t = tr(rho)[1][:, 0]
+    cos = float(g_f @ g_t / (np.linalg.norm(g_f) * np.linalg.norm(g_t)))
+    rel = np.linalg.norm(g_f - g_t) / np.linalg.norm(g_t)
+    _note(f"[adjoint-tr] frequency vs time-reversed: cos {cos:.6f} rel {rel:.3e}")
+    assert cos > 0.99 and rel < 0.1
+
+
+def test_float32_agrees_with_float64():
+    """FP32 fields on the FP64 run's setup (the precision form: the same
+    discrete problem) against the FP64 gradient; and the reference-style FP32
+    run (FP32 setup, a slightly different discrete problem) against its own
+    frequency-domain gradient."""
+    rho = 0.3 + 0.4 * np.random.default_rng(9).random(24)
+    g64 = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast"), 100)(rho)[1][:, 0]
+    g32 = _time_reversed(slab2d_problem(engine=None, precision="f32", mode="fast", setup_precision="f64"),
+                         100)(rho)[1][:, 0]
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    _note(f"[adjoint-tr] f32 fields (f64 setup) vs f64 rel {rel:.3e}")
+    gt = _time_reversed(slab2d_problem(engine=None, precision="f32", mode="fast"), 100)(rho)[1][:, 0]
+    gf = slab2d_problem(engine=None, precision="f32", mode="fast")(rho)[1].sum(axis=1)
+    rel_f = np.linalg.norm(gt - gf) / np.linalg.norm(gt)
+    _note(f"[adjoint-tr] f32 time-reversed vs f32 frequency-domain rel {rel_f:.3e}")
+    assert rel < 2e-3
+    assert rel_f < 0.1
+
+
+def test_rejects_what_it_does_not_cover():
+    # periodic z (the quasi-1D slab): refused before any sweep
+    opt = _time_reversed(slab_problem(engine=None, precision="f64", mode="fast"), 50)
+    with pytest.raises(KhronosError):
+        opt(0.5 * np.ones(24))
+    # no checkpoints: the plain forward engine refuses the sweep
+    opt2 = slab2d_problem(engine=None, precision="f64", mode="fast")
+    opt2.update_design(0.5 * np.ones(24))
+    spec = opt2._spec_with(opt2.base.sources, [], rho=opt2.rho)
+    s = opt2._run(spec)
+    with pytest.raises(KhronosError):
+        s.engine.adjoint_gradient([], [], [None, None, None])
