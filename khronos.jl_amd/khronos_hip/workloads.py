@@ -17,6 +17,14 @@
   U[0.5, 2]) and chi3 = 1e-3; a plane-wave Gaussian pulse whose transverse k
   matches the Bloch phase; one z-normal flux plane at 100 frequencies;
   complex FP32 fields, FP64 DFT accumulators (all draws from ``seed``).
+* ``adjoint_bend`` -- the 2D 90-degree waveguide bend of the reference's
+  inverse-design example (examples/waveguide_bend_inverse_design.jl:16-109)
+  at resolution 40 with 2 um padding: 400 x 400 cells, 40-cell PML, 0.5 um
+  eps 11.56 guides in eps 2.0736, a 3 x 3 um design region of 120 x 120
+  parameters rho ~ U[0, 1] (``seed``), eps in [2.0736, 11.56], Ez line
+  source (Gaussian pulse, fcen 0.625, fwidth 0.5), objective sum |Ez|^2 on
+  the output guide's line at 3 frequencies; 2000 steps (until = 25),
+  checkpoints every 100 steps for the time-reversed adjoint.
 """
 from __future__ import annotations
 
@@ -122,3 +130,37 @@ def algorithmic_bytes(spec) -> dict:
         # the inner-box kernel alone: p = 0 cells, 9 accesses (+3 per-voxel)
         out[kind + "_inner"] = (9 + (3 if any(m is not None for m in minv) else 0)) * counts[0] * s
     return out
+
+
+def adjoint_bend(precision="f32", mode="fast", seed=0, setup_precision=None, method="time_reversed",
+                 checkpoint_every=100, n_design=120, resolution=40, pad=2.0, steps=2000):
+    """(OptimizationProblem, rho) of the 2D bend (see the module docstring)."""
+    import torch
+
+    from . import adjoint as A
+    n_si, n_sio2 = 3.4, 1.44
+    eps_si, eps_sio2 = n_si ** 2, n_sio2 ** 2
+    wg, pml = 0.5, 1.0
+    design = n_design / resolution
+    fcen, fwidth = 0.625, 0.5
+    dz = 1.0 / resolution
+    cell = design + 2 * pad + 2 * pml + 2 * wg
+    geo = [Object(Box([-(design / 2 + pad + wg / 2), 0.0, 0.0], [pad + wg + pml, wg, dz]), Material(ε=eps_si)),
+           Object(Box([0.0, design / 2 + pad + wg / 2, 0.0], [wg, pad + wg + pml, dz]), Material(ε=eps_si)),
+           Object(Box([0.0, 0.0, 0.0], [cell + 1, cell + 1, dz + 1]), Material(ε=eps_sio2))]
+    src = [UniformSource(time_profile=GaussianPulseSource(fcen=fcen, fwidth=fwidth), component=G.Ez(),
+                         center=[-(design / 2 + pad), 0.0, 0.0], size=[0.0, wg, 0.0])]
+    sim = Simulation(cell_size=[cell, cell, 0.0], resolution=resolution, sources=src, geometry=geo,
+                     boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode,
+                     setup_precision=setup_precision)
+    dr = A.DesignRegion(volume=G.Volume([0.0, 0.0, 0.0], [design, design, 0.0]),
+                        design_parameters=np.full(n_design * n_design, 0.5), grid_size=(n_design, n_design),
+                        ε_min=eps_sio2, ε_max=eps_si)
+    obj = A.FourierFieldsObjective(volume=G.Volume([0.0, design / 2 + pad, 0.0], [2 * wg, 0.0, 0.0]),
+                                   component=G.Ez())
+    until = steps * float(sim.dt_n)
+    opt = A.OptimizationProblem(sim, [lambda ez: torch.sum(torch.abs(ez) ** 2)], [obj], [dr],
+                                [0.55, 0.625, 0.7], until=until, method=method,
+                                checkpoint_every=checkpoint_every)
+    rho = np.random.default_rng(seed).random(n_design * n_design)
+    return opt, rho

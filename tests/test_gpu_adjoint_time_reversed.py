@@ -194,3 +194,40 @@ def test_rejects_what_it_does_not_cover():
     s = opt2._run(spec)
     with pytest.raises(KhronosError):
         s.engine.adjoint_gradient([], [], [None, None, None])
+
+
+def test_bend_400x400_full_size():
+    """The 2D bend at full size (400 x 400 cells, 120 x 120 design
+    parameters rho ~ U[0, 1], 2000 steps, 3 frequencies, checkpoints every
+    100 steps): FP32 fields (FP64 setup) and FP64 gradients against FP64
+    central differences at 20 seeded pixels."""
+    import time
+    from khronos_hip import workloads as W
+    opt64, rho = W.adjoint_bend(precision="f64")
+    opt64.update_design(rho)
+    t0 = time.perf_counter()
+    f64 = opt64.forward_run()
+    t1 = time.perf_counter()
+    opt64.adjoint_run()
+    g64 = opt64.calculate_gradient()[:, 0]
+    t2 = time.perf_counter()
+    _note(f"[adjoint-tr] bend f64: J {f64:.6e}, forward {t1 - t0:.3f} s, gradient {t2 - t1:.3f} s, "
+          f"{opt64._fwd_sim.engine.timestep()} steps")
+    opt32, _ = W.adjoint_bend(precision="f32", setup_precision="f64")
+    opt32.update_design(rho)
+    t0 = time.perf_counter()
+    f32 = opt32.forward_run()
+    t1 = time.perf_counter()
+    opt32.adjoint_run()
+    g32 = opt32.calculate_gradient()[:, 0]
+    t2 = time.perf_counter()
+    _note(f"[adjoint-tr] bend f32: J {f32:.6e}, forward {t1 - t0:.3f} s, gradient {t2 - t1:.3f} s")
+    fd, idx = opt64.calculate_fd_gradient(rho, db=1e-3, n_samples=20, seed=0)
+    scale = np.max(np.abs(g64))
+    e64 = np.abs(fd[:, 0] - g64[idx]) / scale
+    e32 = np.abs(fd[:, 0] - g32[idx]) / scale
+    _note(f"[adjoint-tr] bend FD at 20 pixels: max rel err f64 {e64.max():.3e}, f32 {e32.max():.3e}; "
+          f"f32 vs f64 gradient rel {np.linalg.norm(g32 - g64) / np.linalg.norm(g64):.3e}")
+    assert abs(f32 - f64) <= 1e-4 * abs(f64)
+    assert e64.max() < 1e-4, (fd[:, 0], g64[idx])
+    assert e32.max() < 1e-3, (fd[:, 0], g32[idx])
