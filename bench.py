@@ -9,6 +9,7 @@ DFT updates).  Fields, materials and monitors are resident in HBM before the
 timed region starts.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --near2far | --serving | --adjoint   (the other forms)
 
 Prints one JSON line.  Mcell-updates/s = Nx*Ny*Nz*K / wall_s / 1e6
 (src/Simulation.jl:517-518).  The roofline entry uses the E-update kernel
@@ -197,6 +198,66 @@ def near2far_bench(args):
     return 0
 
 
+def adjoint_bend_bench(args):
+    """Differentiation form, config 4: the gradient of the 2D bend's
+    objective (400 x 400 cells, 120 x 120 design parameters, 2001 steps, 3
+    frequencies) with respect to all 14400 parameters by the time-reversed
+    adjoint (checkpoints every 100 steps), FP32 fields on the FP64 setup.  A
+    step is one gradient evaluation: the forward run with checkpoints, then
+    the sweep (re-run of each interval and the transposed steps)."""
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    from khronos_hip import workloads as W
+    opt, rho = W.adjoint_bend(precision="f32", setup_precision="f64")
+    for _ in range(max(1, args.warmup)):
+        opt(rho)
+    k = max(1, args.steps)
+    tf = tg = 0.0
+    for _ in range(k):
+        opt.update_design(rho)
+        t0 = time.perf_counter()
+        opt.forward_run()
+        t1 = time.perf_counter()
+        opt.adjoint_run()
+        opt.calculate_gradient()
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tg += t2 - t1
+    n_steps = opt._fwd_sim.engine.timestep()
+    wall = (tf + tg) / k
+    line = {
+        "metric": "adjoint gradient latency (forward run + time-reversed sweep, all design parameters)",
+        "value": round(wall * 1e3, 2),
+        "unit": "ms per gradient",
+        "n_gpus": 1, "steps": k, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2D waveguide bend 400x400 cells, 120x120 design parameters rho ~ U[0,1] (seed 0), "
+                               f"{n_steps} steps, 3 frequencies, checkpoints every 100 steps; FP32 fields, "
+                               "FP64 setup and gradient accumulation",
+                   "forward_ms": round(tf / k * 1e3, 2), "sweep_ms": round(tg / k * 1e3, 2)},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_engine import OracleEngine  # the checker, used here only as the CPU baseline
+        ref, _ = W.adjoint_bend(precision="f32", setup_precision="f64", method="frequency", engine=OracleEngine)
+        ref.update_design(rho)
+        t0 = time.perf_counter()
+        ref.forward_run()
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        line["cpu_baseline"] = {"value": round(4 * dt * 1e3, 1), "unit": "ms per gradient", "cores": cores,
+                                "kind": "port",
+                                "sample": "the reference's frequency-domain method on the C oracle (-O3 -fopenmp, "
+                                          f"{cores} threads): its forward run of the same problem (with the "
+                                          "design-region D monitors) timed once, times 4 for the forward run "
+                                          "plus one adjoint run per frequency, all of the same length"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +270,8 @@ def main(argv=None):
                     help="only run a plain device copy of 1 GiB five times (counter calibration)")
     ap.add_argument("--serving", action="store_true", help="serving form instead of the headline benchmark")
     ap.add_argument("--near2far", action="store_true", help="near-to-far projection instead of the headline benchmark")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time-reversed adjoint gradient of the 2D bend instead of the headline benchmark")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--max-active", type=int, default=6)
     ap.add_argument("--chunk", type=int, default=256)
@@ -218,6 +281,8 @@ def main(argv=None):
         return serving_bench(args)
     if args.near2far:
         return near2far_bench(args)
+    if args.adjoint:
+        return adjoint_bend_bench(args)
     if args.calibrate_copy:
         import torch
         src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")

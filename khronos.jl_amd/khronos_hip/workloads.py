@@ -133,7 +133,7 @@ def algorithmic_bytes(spec) -> dict:
 
 
 def adjoint_bend(precision="f32", mode="fast", seed=0, setup_precision=None, method="time_reversed",
-                 checkpoint_every=100, n_design=120, resolution=40, pad=2.0, steps=2000):
+                 checkpoint_every=100, n_design=120, resolution=40, pad=2.0, steps=2000, engine=None):
     """(OptimizationProblem, rho) of the 2D bend (see the module docstring)."""
     import torch
 
@@ -152,7 +152,7 @@ def adjoint_bend(precision="f32", mode="fast", seed=0, setup_precision=None, met
                          center=[-(design / 2 + pad), 0.0, 0.0], size=[0.0, wg, 0.0])]
     sim = Simulation(cell_size=[cell, cell, 0.0], resolution=resolution, sources=src, geometry=geo,
                      boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode,
-                     setup_precision=setup_precision)
+                     setup_precision=setup_precision, engine=engine)
     dr = A.DesignRegion(volume=G.Volume([0.0, 0.0, 0.0], [design, design, 0.0]),
                         design_parameters=np.full(n_design * n_design, 0.5), grid_size=(n_design, n_design),
                         ε_min=eps_sio2, ε_max=eps_si)
