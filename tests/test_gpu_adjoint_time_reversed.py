@@ -231,3 +231,23 @@ def test_bend_400x400_full_size():
     assert abs(f32 - f64) <= 1e-4 * abs(f64)
     assert e64.max() < 1e-4, (fd[:, 0], g64[idx])
     assert e32.max() < 1e-3, (fd[:, 0], g32[idx])
+
+
+def test_h_field_objective_and_decimated_monitor(monkeypatch):
+    """An objective on Hy (seeded at the H time stamps) plus Ez, with the
+    objective monitors sampling every 3rd step: still equal to central
+    differences (the frequency-domain path has no H-field adjoint sources)."""
+    import functools
+    monkeypatch.setattr(A, "DFTMonitor", functools.partial(A.DFTMonitor, decimation=3))
+    opt = slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.55))
+    opt.objective_arguments = [
+        A.FourierFieldsObjective(volume=Volume([0.75, 0, 0], [0, 0.6, 0.0]), component=kh.Hy()),
+        A.FourierFieldsObjective(volume=Volume([0.75, 0, 0], [0, 0.6, 0.0]), component=kh.Ez())]
+    opt.objective_functions = [lambda h, e: torch.sum(torch.abs(h) ** 2) * 100.0 + 0.1 * torch.sum(torch.abs(e) ** 2)]
+    _time_reversed(opt, 45)
+    rho = 0.3 + 0.4 * np.random.default_rng(12).random(24)
+    res, _ = _directional(opt, rho)
+    decs = [md.decimation for md in opt._fwd_spec.dfts]
+    _note(f"[adjoint-tr] Hy + Ez objective, decimations {decs}: {res}")
+    assert max(decs) > 1
+    assert max(r[2] for r in res) < 1e-6, res
