@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 3
+#define KH_ABI_VERSION 4
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -309,16 +309,22 @@ int kh_server_poll(kh_server *server, int64_t *done_ids, int32_t cap, int32_t *n
 
 /* Geometry objects for kh_raster_component.  shape: KH_SHAPE_BOX (size =
  * full extents), KH_SHAPE_SPHERE (size[0] = radius), KH_SHAPE_CYLINDER
- * (size[0] = radius, size[1] = height, axis 0..2).  value[c]: relative
- * permittivity of Ex, Ey, Ez (c = 0..2) or permeability of Hx, Hy, Hz
- * (3..5) inside the object; NaN where the object does not set it. */
-enum kh_shape { KH_SHAPE_BOX = 0, KH_SHAPE_SPHERE = 1, KH_SHAPE_CYLINDER = 2 };
+ * (size[0] = radius, size[1] = height, axis 0..2), KH_SHAPE_PRISM (a convex
+ * polygon of n_vertices (x, y) pairs about center, counter-clockwise, in
+ * the x-y plane, extruded over size[0] along z: the prisms of the GDS
+ * import, src/GDS.jl:954-1002).  value[c]: relative permittivity of Ex, Ey,
+ * Ez (c = 0..2) or permeability of Hx, Hy, Hz (3..5) inside the object; NaN
+ * where the object does not set it.  vertices is read only for prisms. */
+enum kh_shape { KH_SHAPE_BOX = 0, KH_SHAPE_SPHERE = 1, KH_SHAPE_CYLINDER = 2, KH_SHAPE_PRISM = 3 };
 typedef struct kh_raster_object {
     int32_t shape;
     int32_t axis;
     double center[3];
     double size[3];
     double value[6];
+    int32_t n_vertices;
+    int32_t reserved;
+    const double *vertices;
 } kh_raster_object;
 
 /* Rasterise one material component on the current device: out (precision

@@ -1761,7 +1761,21 @@ struct RasterObj {
     int shape, axis;
     double c[3], p[3];
     double value[6];
+    const double *v;  // prism: nv (x, y) vertices about c, counter-clockwise (device copy)
+    int nv;
 };
+
+// Prism membership: |z - cz| <= h/2 and left of (or on) every edge.
+__device__ __forceinline__ bool prism_contains(const RasterObj &o, const double *P) {
+    const double x = P[0] - o.c[0], y = P[1] - o.c[1];
+    if (!(fabs(P[2] - o.c[2]) <= o.p[0] / 2)) return false;
+    for (int i = 0; i < o.nv; ++i) {
+        const int j = i + 1 == o.nv ? 0 : i + 1;
+        const double ax = o.v[2 * i], ay = o.v[2 * i + 1], bx = o.v[2 * j], by = o.v[2 * j + 1];
+        if (!((bx - ax) * (y - ay) - (by - ay) * (x - ax) >= 0)) return false;
+    }
+    return true;
+}
 
 // The object's bounding box [lo, hi] (the host painting's index range) and
 // the shape's own test, with the host's arithmetic.
@@ -1773,6 +1787,19 @@ __device__ __forceinline__ bool raster_holds(const RasterObj &o, double x, doubl
             lo[a] = o.c[a] - o.p[a] / 2;
             hi[a] = o.c[a] + o.p[a] / 2;
         }
+    } else if (o.shape == 3) {
+        double mn[2] = {o.v[0], o.v[1]}, mx[2] = {o.v[0], o.v[1]};
+        for (int i = 1; i < o.nv; ++i)
+            for (int a = 0; a < 2; ++a) {
+                mn[a] = fmin(mn[a], o.v[2 * i + a]);
+                mx[a] = fmax(mx[a], o.v[2 * i + a]);
+            }
+        for (int a = 0; a < 2; ++a) {
+            lo[a] = o.c[a] + mn[a];
+            hi[a] = o.c[a] + mx[a];
+        }
+        lo[2] = o.c[2] + -o.p[0] / 2;
+        hi[2] = o.c[2] + o.p[0] / 2;
     } else if (o.shape == 1) {
         for (int a = 0; a < 3; ++a) {
             lo[a] = o.c[a] - o.p[0];
@@ -1792,6 +1819,7 @@ __device__ __forceinline__ bool raster_holds(const RasterObj &o, double x, doubl
             if (!(fabs(P[a] - o.c[a]) <= o.p[a] / 2)) return false;
         return true;
     }
+    if (o.shape == 3) return prism_contains(o, P);
     if (o.shape == 1) {
         const double dx = x - o.c[0], dy = y - o.c[1], dz = z - o.c[2];
         return (dx * dx + dy * dy) + dz * dz <= o.p[0] * o.p[0];
@@ -1808,6 +1836,7 @@ __device__ __forceinline__ bool raster_holds(const RasterObj &o, double x, doubl
 // The shape's own test without the bounding box (the smoothing's
 // `contains`).
 __device__ __forceinline__ bool raster_contains(const RasterObj &o, const double *P) {
+    if (o.shape == 3) return prism_contains(o, P);
     if (o.shape == 0) {
         for (int a = 0; a < 3; ++a)
             if (!(fabs(P[a] - o.c[a]) <= o.p[a] / 2)) return false;
@@ -1868,7 +1897,62 @@ __device__ __forceinline__ void raster_normalize(double *v) {
     }
 }
 
+// Prism: the cylinder's rule with the polygon in place of the circle (in
+// plane: largest edge distance inside, nearest boundary point outside).
+__device__ __forceinline__ void prism_surfpt(const RasterObj &o, const double *P, double *sp, double *nout) {
+    const double x = P[0] - o.c[0], y = P[1] - o.c[1], ell = P[2] - o.c[2];
+    double best = -INFINITY, bnx = 0.0, bny = 0.0, dmin = INFINITY, dnx = 0.0, dny = 0.0;
+    for (int i = 0; i < o.nv; ++i) {
+        const int j = i + 1 == o.nv ? 0 : i + 1;
+        const double ax = o.v[2 * i], ay = o.v[2 * i + 1];
+        const double ex = o.v[2 * j] - ax, ey = o.v[2 * j + 1] - ay;
+        const double ln = sqrt(ex * ex + ey * ey);
+        const double nx = ey / ln, ny = -ex / ln;
+        const double sd = (x - ax) * nx + (y - ay) * ny;
+        if (sd > best) {
+            best = sd;
+            bnx = nx;
+            bny = ny;
+        }
+        const double t = fmin(fmax(((x - ax) * ex + (y - ay) * ey) / (ln * ln), 0.0), 1.0);
+        const double px = x - (ax + t * ex), py = y - (ay + t * ey);
+        const double d = sqrt(px * px + py * py);
+        if (d < dmin) {
+            dmin = d;
+            dnx = px;
+            dny = py;
+        }
+    }
+    const bool inpoly = best <= 0;
+    const double d_side = inpoly ? best : dmin;
+    const double ns[3] = {inpoly ? bnx : dnx / dmin, inpoly ? bny : dny / dmin, 0.0};
+    const double sgn = ell >= 0 ? 1.0 : -1.0;
+    const double d_cap = fabs(ell) - o.p[0] / 2;
+    const double nc[3] = {0.0, 0.0, sgn};
+    if (d_side <= 0 && d_cap <= 0) {
+        const bool side = d_side >= d_cap;
+        const double d = side ? d_side : d_cap;
+        for (int a = 0; a < 3; ++a) {
+            const double n = side ? ns[a] : nc[a];
+            sp[a] = P[a] - d * n;
+            nout[a] = n;
+        }
+        return;
+    }
+    const double ds = fmax(d_side, 0.0), dc = fmax(d_cap, 0.0);
+    for (int a = 0; a < 3; ++a) {
+        const double delta = ds * ns[a] + dc * nc[a];
+        sp[a] = P[a] - delta;
+        nout[a] = delta;
+    }
+    raster_normalize(nout);
+}
+
 __device__ __forceinline__ void raster_surfpt(const RasterObj &o, const double *P, double *sp, double *nout) {
+    if (o.shape == 3) {
+        prism_surfpt(o, P, sp, nout);
+        return;
+    }
     if (o.shape == 0) {
         double L[3], s[3], d[3], h[3];
         bool inside = true;

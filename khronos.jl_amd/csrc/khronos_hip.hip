@@ -2067,10 +2067,20 @@ void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_
                 const double *ys, const double *zs, int smoothing, const double *half, void *out) {
     const int64_t n = (int64_t)cnt[0] * cnt[1] * cnt[2];
     std::vector<kh::RasterObj> ho(std::max(1, n_objs));
+    std::vector<double> verts;  // all prism vertices, one device copy
+    std::vector<size_t> vfirst(std::max(1, n_objs), 0);
     for (int o = 0; o < n_objs; ++o) {
         const kh_raster_object &r = objs[o];
-        require(r.shape >= 0 && r.shape <= 2, "shape must be KH_SHAPE_BOX, _SPHERE or _CYLINDER");
+        require(r.shape >= 0 && r.shape <= 3, "shape must be KH_SHAPE_BOX, _SPHERE, _CYLINDER or _PRISM");
         require(r.axis >= 0 && r.axis <= 2, "cylinder axis must be 0..2");
+        ho[o] = kh::RasterObj{};
+        if (r.shape == KH_SHAPE_PRISM) {
+            require(r.n_vertices >= 3 && r.n_vertices <= 1000000 && r.vertices != nullptr,
+                    "a prism needs at least 3 vertices");
+            vfirst[o] = verts.size();
+            verts.insert(verts.end(), r.vertices, r.vertices + 2 * (size_t)r.n_vertices);
+            ho[o].nv = r.n_vertices;
+        }
         ho[o].shape = r.shape;
         ho[o].axis = r.axis;
         for (int a = 0; a < 3; ++a) {
@@ -2085,7 +2095,8 @@ void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_
     const size_t cb = (size_t)(cnt[0] + cnt[1] + cnt[2]) * sizeof(double);
     const size_t ab = (size_t)n * sizeof(T);
     void *buf = nullptr;
-    const size_t total = ob + cb + ab * (smooth ? 2 : 1) + 512;
+    const size_t vb = verts.size() * sizeof(double);
+    const size_t total = ob + cb + ab * (smooth ? 2 : 1) + vb + 768;
     HIPCHK(hipMalloc(&buf, total));
     struct Free {
         void *p;
@@ -2097,6 +2108,10 @@ void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_
     double *d_y = d_x + cnt[0], *d_z = d_y + cnt[1];
     T *d_a = (T *)((char *)d_x + ((cb + 255) / 256) * 256);
     T *d_s = d_a + n;
+    double *d_v = (double *)((char *)d_a + ((ab * (smooth ? 2 : 1) + 255) / 256) * 256);
+    for (int o = 0; o < n_objs; ++o)
+        if (ho[o].shape == KH_SHAPE_PRISM) ho[o].v = d_v + vfirst[o];
+    if (vb) HIPCHK(hipMemcpy(d_v, verts.data(), vb, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_obj, ho.data(), ob, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_x, xs, cnt[0] * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_y, ys, cnt[1] * sizeof(double), hipMemcpyHostToDevice));

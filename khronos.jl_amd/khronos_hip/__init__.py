@@ -15,7 +15,9 @@ from .mode import ModeSpec
 from .near2far import (Near2FarMonitor, LayerSpec, LayerStack, compute_far_field, compute_far_field_power,
                        compute_LEE)
 from .boundaries import PML, Periodic, Bloch, PECBoundary, PMCBoundary, Absorber
-from .geometry import Material, Object, Box, Sphere, Cylinder
+from .geometry import Material, Object, Box, Sphere, Cylinder, Prism
+from .gds import (read_gds, flatten_cell, gds_to_objects, gds_polygons_to_objects, get_layers, get_cell_names,
+                  decode_gdsii_real, GDSLibrary, GDSCell, GDSPolygon, GDSReference)
 from .smoothing import SubpixelSmoothing, NoSmoothing, VolumeAveraging, AnisotropicSmoothing
 from .susceptibility import (LorentzianSusceptibility, DrudeSusceptibility, compute_ade_coefficients,
                              eval_susceptibility)
@@ -34,4 +36,6 @@ __all__ = [
     "eval_susceptibility", "Simulation", "SimSpec", "run", "run_until", "stop_when_dft_decayed",
     "run_benchmark", "reset_fields", "get_flux", "dft_fields_norm", "prepare_simulation",
     "Near2FarMonitor", "LayerSpec", "LayerStack", "compute_far_field", "compute_far_field_power", "compute_LEE",
+    "Prism", "read_gds", "flatten_cell", "gds_to_objects", "gds_polygons_to_objects", "get_layers",
+    "get_cell_names", "decode_gdsii_real", "GDSLibrary", "GDSCell", "GDSPolygon", "GDSReference",
 ]

@@ -18,7 +18,8 @@ KH_MODE_FAST, KH_MODE_EXACT = 0, 1
 
 class KhRasterObject(C.Structure):
     _fields_ = [("shape", C.c_int32), ("axis", C.c_int32), ("center", C.c_double * 3), ("size", C.c_double * 3),
-                ("value", C.c_double * 6)]
+                ("value", C.c_double * 6), ("n_vertices", C.c_int32), ("reserved", C.c_int32),
+                ("vertices", C.c_void_p)]
 
 
 class KhSimDesc(C.Structure):

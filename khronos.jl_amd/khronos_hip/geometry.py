@@ -21,7 +21,7 @@ chi1 correction of the inverse permittivity.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import Tuple, List, Optional, Sequence
 
 import numpy as np
 
@@ -122,6 +122,36 @@ class Cylinder(Shape):
         P = (X - self.center[0], Y - self.center[1], Z - self.center[2])
         r2 = sum(P[a] ** 2 for a in range(3) if a != self.axis)
         return (r2 <= self.radius ** 2) & (np.abs(P[self.axis]) <= self.height / 2)
+
+
+@dataclass
+class Prism(Shape):
+    """Right prism: a convex polygon base (vertices relative to ``center``,
+    counter-clockwise) in the x-y plane, extruded over ``height`` along z --
+    GeometryPrimitives' Prism with identity axes, as src/GDS.jl:984-997
+    builds them.  Boundaries inclusive."""
+
+    center: Sequence[float]
+    vertices: Sequence[Tuple[float, float]]
+    height: float
+
+    def bounds(self):
+        v = np.asarray(self.vertices, float)
+        c = np.asarray(self.center, float)
+        return (c + np.array([v[:, 0].min(), v[:, 1].min(), -self.height / 2]),
+                c + np.array([v[:, 0].max(), v[:, 1].max(), self.height / 2]))
+
+    def contains(self, X, Y, Z):
+        c = self.center
+        x, y = X - c[0], Y - c[1]
+        inside = np.abs(Z - c[2]) <= self.height / 2
+        v = [(float(a), float(b)) for a, b in self.vertices]
+        for i in range(len(v)):
+            ax, ay = v[i]
+            bx, by = v[(i + 1) % len(v)]
+            # left of (or on) every edge of the counter-clockwise boundary
+            inside = inside & ((bx - ax) * (y - ay) - (by - ay) * (x - ax) >= 0)
+        return inside
 
 
 @dataclass

@@ -16,13 +16,14 @@ import math
 import numpy as np
 
 from . import _lib
-from .geometry import Box, Cylinder, Sphere
+from .geometry import Box, Cylinder, Prism, Sphere
 from .grid import Ex, Ey, Ez, Hx, Hy, Hz, component_voxel_count, full_grid_volume, gv_coords
 from .smoothing import AnisotropicSmoothing, NoSmoothing, VolumeAveraging
 
 
 def _objects(geometry):
     arr = (_lib.KhRasterObject * max(1, len(geometry)))()
+    keep = []  # prism vertex arrays, alive as long as the object array
     for o, obj in enumerate(geometry):
         sh = obj.shape
         r = arr[o]
@@ -40,6 +41,14 @@ def _objects(geometry):
             r.axis = int(sh.axis)
             r.center[:] = [float(v) for v in sh.center]
             r.size[:] = [float(sh.radius), float(sh.height), 0.0]
+        elif isinstance(sh, Prism):
+            r.shape = 3
+            r.center[:] = [float(v) for v in sh.center]
+            r.size[:] = [float(sh.height), 0.0, 0.0]
+            v = np.ascontiguousarray(np.asarray(sh.vertices, dtype=np.float64).reshape(-1, 2))
+            keep.append(v)
+            r.n_vertices = len(v)
+            r.vertices = v.ctypes.data_as(C.c_void_p)
         else:
             raise NotImplementedError(f"device rasterisation of {type(sh).__name__}")
         vals = []
@@ -48,6 +57,7 @@ def _objects(geometry):
                 p = obj.material.perm(electric, a)
                 vals.append(math.nan if p is None else float(p))
         r.value[:] = vals
+    arr._vertex_arrays = keep
     return arr
 
 

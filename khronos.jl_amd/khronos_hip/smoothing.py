@@ -135,8 +135,72 @@ def surfpt_cylinder(P, center, radius, height, axis):
     return sp, nout
 
 
+def surfpt_prism(P, center, vertices, height):
+    """Nearest surface point and outward normal of a z-extruded convex
+    polygon (vertices about ``center``, counter-clockwise), by the cylinder's
+    rule with the polygon in place of the circle: inside, the nearer of the
+    closest side (largest edge distance) and the cap; outside, the clamped
+    offset from the nearest boundary point of the polygon and the cap."""
+    c = np.asarray(center, float)
+    v = np.asarray(vertices, float)
+    q = P - c
+    x, y, ell = q[:, 0], q[:, 1], q[:, 2]
+    k = len(v)
+    best = np.full(len(P), -np.inf)
+    bn = np.zeros((len(P), 2))
+    dmin = np.full(len(P), np.inf)
+    dn = np.zeros((len(P), 2))
+    for i in range(k):
+        ax, ay = v[i]
+        bx, by = v[(i + 1) % k]
+        ex, ey = bx - ax, by - ay
+        ln = np.sqrt(ex * ex + ey * ey)
+        nx, ny = ey / ln, -ex / ln
+        sd = (x - ax) * nx + (y - ay) * ny
+        up = sd > best
+        best = np.where(up, sd, best)
+        bn[up] = (nx, ny)
+        t = np.clip(((x - ax) * ex + (y - ay) * ey) / (ln * ln), 0.0, 1.0)
+        px, py = x - (ax + t * ex), y - (ay + t * ey)
+        d = np.sqrt(px * px + py * py)
+        closer = d < dmin
+        dmin = np.where(closer, d, dmin)
+        dn[closer, 0] = px[closer]
+        dn[closer, 1] = py[closer]
+    inpoly = best <= 0
+    d_side = np.where(inpoly, best, dmin)
+    n_side = np.zeros_like(P)
+    n_side[inpoly, :2] = bn[inpoly]
+    out_p = ~inpoly
+    n_side[out_p, 0] = dn[out_p, 0] / dmin[out_p]
+    n_side[out_p, 1] = dn[out_p, 1] / dmin[out_p]
+    sgn = np.where(ell >= 0, 1.0, -1.0)
+    d_cap = np.abs(ell) - height / 2
+    n_cap = np.zeros_like(P)
+    n_cap[:, 2] = sgn
+    inside = (d_side <= 0) & (d_cap <= 0)
+    use_side = inside & (d_side >= d_cap)
+    use_cap = inside & ~use_side
+    sp = np.empty_like(P)
+    nout = np.empty_like(P)
+    sp[use_side] = P[use_side] - d_side[use_side, None] * n_side[use_side]
+    nout[use_side] = n_side[use_side]
+    sp[use_cap] = P[use_cap] - d_cap[use_cap, None] * n_cap[use_cap]
+    nout[use_cap] = n_cap[use_cap]
+    out = ~inside
+    if np.any(out):
+        ds = np.maximum(d_side[out], 0.0)
+        dc = np.maximum(d_cap[out], 0.0)
+        delta = ds[:, None] * n_side[out] + dc[:, None] * n_cap[out]
+        sp[out] = P[out] - delta
+        nout[out] = _normalize(delta)
+    return sp, nout
+
+
 def surfpt_nearby(P, shape):
-    from .geometry import Box, Cylinder, Sphere
+    from .geometry import Box, Cylinder, Prism, Sphere
+    if isinstance(shape, Prism):
+        return surfpt_prism(P, shape.center, shape.vertices, shape.height)
     if isinstance(shape, Box):
         return surfpt_box(P, shape.center, shape.size)
     if isinstance(shape, Sphere):

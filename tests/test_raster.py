@@ -9,9 +9,11 @@ from khronos_hip import _lib
 
 
 def test_raster_object_layout():
-    # int32 shape, int32 axis, double center[3], size[3], value[6]
-    assert C.sizeof(_lib.KhRasterObject) == 4 + 4 + 24 + 24 + 48
+    # int32 shape, int32 axis, double center[3], size[3], value[6], int32
+    # n_vertices, int32 reserved, const double *vertices
+    assert C.sizeof(_lib.KhRasterObject) == 4 + 4 + 24 + 24 + 48 + 4 + 4 + 8
     assert _lib.KhRasterObject.center.offset == 8 and _lib.KhRasterObject.value.offset == 56
+    assert _lib.KhRasterObject.n_vertices.offset == 104 and _lib.KhRasterObject.vertices.offset == 112
 
 
 def test_raster_rejects_bad_arguments_without_a_device():
@@ -28,6 +30,11 @@ def test_raster_rejects_bad_arguments_without_a_device():
     obj[0].shape = 5
     assert lib.kh_raster_component(1, 0, cnt, p(x), p(x), p(x), C.cast(obj, C.c_void_p), 1, 0, half, p(out)) != 0
     assert "shape" in lib.kh_last_error().decode()
+    # a prism without vertices
+    obj[0].shape = 3
+    obj[0].n_vertices = 2
+    assert lib.kh_raster_component(1, 0, cnt, p(x), p(x), p(x), C.cast(obj, C.c_void_p), 1, 0, half, p(out)) != 0
+    assert "prism" in lib.kh_last_error().decode()
     # smoothing mode out of range
     obj[0].shape = 0
     assert lib.kh_raster_component(1, 0, cnt, p(x), p(x), p(x), C.cast(obj, C.c_void_p), 1, 3, half, p(out)) != 0
