@@ -100,6 +100,9 @@ class KhronosError(RuntimeError):
         self.code = code
 
 
+ABI_VERSION = 4  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
+
+
 def load(path: str = LIB_PATH):
     """Load (once) and return the shared library with typed entry points."""
     global _lib
@@ -121,6 +124,9 @@ def load(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.kh_version() != ABI_VERSION:
+        raise ImportError(f"{path} has ABI version {lib.kh_version()}, these bindings need {ABI_VERSION} "
+                          "(rebuild with __graft_entry__.build())")
     _lib = lib
     return lib
 
