@@ -54,7 +54,8 @@ def cpu_baseline(spec, cores: int, budget_s: float = 15.0):
         "unit": "Mcell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"strip-waveguide 384x192x160 FP32, {n} steps after 2 warm-up steps "
+        "sample": f"strip-waveguide 384x192x160 {'FP32' if np.dtype(spec.real).itemsize == 4 else 'FP64'}, "
+                  f"{n} steps after 2 warm-up steps "
                   f"(C oracle, -O3 -fopenmp, {cores} threads)",
     }
 
@@ -265,6 +266,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"],
+                    help="field precision of the strip waveguide (the headline is f32; f64 for the comparison)")
     ap.add_argument("--graph-steps", type=int, default=16, help="steps per captured hipGraph (0: eager launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (counter calibration)")
@@ -307,7 +310,9 @@ def main(argv=None):
 
     from khronos_hip.workloads import strip_waveguide, algorithmic_bytes
 
-    sim = strip_waveguide(precision="f32", mode=args.mode)
+    sim = strip_waveguide(precision=args.precision, mode=args.mode)
+    ctype = "float" if args.precision == "f32" else "double"
+    pname = "FP32" if args.precision == "f32" else "FP64"
     sim.prepare()
     eng = sim.engine
     cells = sim.Nx * sim.Ny * sim.Nz
@@ -340,25 +345,25 @@ def main(argv=None):
     # dominant kernel: the E update over the whole grid (FAST: the PML kernel,
     # whose sigma = 0 branch is the inner update; EXACT: the cascade kernel)
     if args.mode == "fast" and avg["e_inner"] > 0:
-        dom, dom_bytes, dom_name = avg["e_inner"], ab["E_inner"], "inner_e_kernel<float,true,true>"
-        trace_name = "kh::inner_e_kernel<float, true, true>"
+        dom, dom_bytes, dom_name = avg["e_inner"], ab["E_inner"], f"inner_e_kernel<{ctype},true,true>"
+        trace_name = f"kh::inner_e_kernel<{ctype}, true, true>"
     elif args.mode == "fast":
         # the library reads a byte per cell into a value table when the
         # per-voxel eps^-1 (with the ghost value 1) holds at most 256 values
         vals = np.unique(np.concatenate([np.ravel(a) for a in sim.spec.eps_inv if a is not None] + [np.ones(1)]))
         lut = os.environ.get("KH_LUT", "1") != "0" and len(vals) <= 256
-        tmpl = "float, true, false, " + ("true" if lut else "false")
+        tmpl = f"{ctype}, true, false, " + ("true" if lut else "false")
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "pml_e_kernel<" + tmpl.replace(" ", "") + ">"
         trace_name = "kh::pml_e_kernel<" + tmpl + ">"
     else:
-        dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "cascade_e_kernel<float>"
-        trace_name = "kh::cascade_e_kernel<float>"
+        dom, dom_bytes, dom_name = avg["e_full"], ab["E"], f"cascade_e_kernel<{ctype}>"
+        trace_name = f"kh::cascade_e_kernel<{ctype}>"
     achieved = dom_bytes / (dom * 1e-3) / 1e9 if dom > 0 else 0.0
     b_alg = (ab["H"] + ab["E"]) / cells
     step_kernel_ms = sum(v[0] for v in tm.values()) / k_steps
 
     line = {
-        "metric": "Mcell-updates/s (FP32 3D Yee + CPML time step)",
+        "metric": f"Mcell-updates/s ({pname} 3D Yee + CPML time step)",
         "value": round(value, 2),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
@@ -368,12 +373,12 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.precision,
         "data": "synthetic",
         "config": {
             "workload": "strip waveguide 384x192x160, 12-cell CPML, anisotropically smoothed diagonal "
                         "per-voxel eps, Ez Gaussian-pulse planar source with Gaussian transverse profile, "
-                        "2 flux planes x 50 freqs (f64 DFT accumulators), FP32 fields",
+                        f"2 flux planes x 50 freqs (f64 DFT accumulators), {pname} fields",
             "cells": cells,
             "mode": args.mode,
             "graph_steps": args.graph_steps,
@@ -388,7 +393,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(trace_name),
+            "traffic": measured_traffic(trace_name) if args.precision == "f32" else None,
             "kernel": dom_name,
             "alg_bytes_per_launch": dom_bytes,
             "kernel_ms": round(dom, 5),
