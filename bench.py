@@ -200,7 +200,7 @@ def near2far_bench(args):
 
 
 def adjoint_bend_bench(args):
-    """Differentiation form, config 4: the gradient of the 2D bend's
+    """Differentiation form: the gradient of the 2D bend's
     objective (400 x 400 cells, 120 x 120 design parameters, 2001 steps, 3
     frequencies) with respect to all 14400 parameters by the time-reversed
     adjoint (checkpoints every 100 steps), FP32 fields on the FP64 setup.  A

@@ -1,4 +1,4 @@
-"""Config 3, the periodic dispersive metasurface (Bloch x/y, PML z, 8 x 8
+"""The periodic dispersive metasurface (Bloch x/y, PML z, 8 x 8
 Lorentz + chi3 cylinders, plane-wave pulse, complex FP32 fields, FP64 DFT at
 100 frequencies).
 
@@ -40,7 +40,7 @@ def test_small_lattice_against_oracle():
     o = metasurface(engine=OracleEngine, mode="exact", **kw)
     e = metasurface(mode="exact", **kw)
     f = metasurface(mode="fast", **kw)
-    _note(f"[config3] small lattice {e.Nx}x{e.Ny}x{e.Nz}, 120 steps")
+    _note(f"[metasurface] small lattice {e.Nx}x{e.Ny}x{e.Nz}, 120 steps")
     for s in (o, e, f):
         s.step(120)
     fo = [o.read_field(C()) for C in COMPS]
@@ -58,7 +58,7 @@ def test_full_size_fast_vs_exact():
     steps = 200
     f = metasurface(mode="fast")
     e = metasurface(mode="exact")
-    _note(f"[config3] full size {f.Nx}x{f.Ny}x{f.Nz}, {steps} steps FAST and EXACT")
+    _note(f"[metasurface] full size {f.Nx}x{f.Ny}x{f.Nz}, {steps} steps FAST and EXACT")
     f.prepare()
     f.step(8)
     f.engine.sync()
@@ -66,14 +66,14 @@ def test_full_size_fast_vs_exact():
     f.step(steps - 8)
     f.engine.sync()
     dt = time.perf_counter() - t0
-    _note(f"[config3] FAST {f.Nx * f.Ny * f.Nz * (steps - 8) / dt / 1e6:.0f} Mcell-updates/s "
+    _note(f"[metasurface] FAST {f.Nx * f.Ny * f.Nz * (steps - 8) / dt / 1e6:.0f} Mcell-updates/s "
           f"({dt / (steps - 8) * 1e3:.3f} ms/step, complex FP32, 2 poles + chi3, 100-frequency flux)")
     e.step(steps)
     fa = [f.read_field(C()) for C in COMPS]
     fb = [e.read_field(C()) for C in COMPS]
     assert all(np.all(np.isfinite(x)) for x in fa)
     err = _rel(fa, fb)
-    _note(f"[config3] FAST vs EXACT rel-L2 {err:.3e}")
+    _note(f"[metasurface] FAST vs EXACT rel-L2 {err:.3e}")
     assert err < 1e-5
     for i in range(len(f.spec.dfts)):
         b = e.engine.read_dft(i)

@@ -1,6 +1,6 @@
 """GPU parity: libkhronos_hip (through its C ABI) against the CPU oracle.
 
-Tolerances (the north-star bar):
+Tolerances:
   EXACT mode   -- fields equal to the oracle value for value (same operation
                   order, no FMA contraction), DFTs within 1e-13.
   FAST mode    -- fields within rel-L2 1e-12 (float64) / 1e-5 (float32) of
@@ -115,8 +115,8 @@ def test_device_flux_and_norm_match_host_reduction():
         assert abs(norms[i] - ref) <= 1e-12 * ref
 
 
-def test_config1_dipole_f64_500_steps():
-    """Survey config 1: 80^3 vacuum, 10-cell PML, Ez Gaussian dipole (fcen 1,
+def test_dipole_80_f64_500_steps():
+    """The 80^3 dipole: vacuum, 10-cell PML, Ez Gaussian dipole (fcen 1,
     fwidth 0.2), Courant 0.5, float64, 500 steps, one DFT plane at 5 freqs."""
     def mk(engine=None, mode="fast"):
         src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=1.0, fwidth=0.2), component=kh.Ez(),
@@ -126,7 +126,7 @@ def test_config1_dipole_f64_500_steps():
         return kh.Simulation(cell_size=[4.0, 4.0, 4.0], resolution=20, sources=src, monitors=mon,
                              boundaries=[[0.5, 0.5]] * 3, precision="f64", mode=mode, engine=engine)
     g, o = mk(), mk(OracleEngine, "exact")
-    _note("[parity] config1 80^3 f64 500 steps")
+    _note("[parity] dipole 80^3 f64 500 steps")
     g.step(500)
     o.step(500)
     tot, per = field_errors(g, o)
@@ -197,7 +197,7 @@ def test_abi_errors_are_reported_not_fatal():
 
 
 def _strip_sim(precision, mode, amp=1.0, steps_flux=True):
-    """Config 2 geometry at full size (384 x 192 x 160, 12-cell PML,
+    """The strip waveguide at full size (384 x 192 x 160, 12-cell PML,
     anisotropically smoothed strip)."""
     res = 20
     fcen = 1 / 1.55
@@ -216,9 +216,9 @@ def _strip_sim(precision, mode, amp=1.0, steps_flux=True):
                          subpixel_smoothing=kh.AnisotropicSmoothing())
 
 
-def test_config2_full_size_fast_vs_exact_and_linearity():
+def test_strip_waveguide_full_size_fast_vs_exact_and_linearity():
     steps = 300
-    _note("[parity] config2 384x192x160 f32 fast/exact/2x-amplitude, 300 steps each")
+    _note("[parity] strip waveguide 384x192x160 f32 fast/exact/2x-amplitude, 300 steps each")
     a = _strip_sim("f32", "fast")
     b = _strip_sim("f32", "exact")
     c = _strip_sim("f32", "fast", amp=2.0)

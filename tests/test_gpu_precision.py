@@ -36,8 +36,8 @@ def _fluxes(precision, dft_accum="f64", steps=4000, setup=None):
     return out
 
 
-def test_config2_fp32_flux_within_1e4_of_fp64():
-    _note("[precision] config2 full size, 4000 steps: f64, f32 (f64 accumulators), f32 (f32 accumulators)")
+def test_strip_waveguide_fp32_flux_within_1e4_of_fp64():
+    _note("[precision] strip waveguide full size, 4000 steps: f64, f32 (f64 accumulators), f32 (f32 accumulators)")
     ref = _fluxes("f64")
     f32 = _fluxes("f32", setup="f64")
     f32a = _fluxes("f32", dft_accum="f32", setup="f64")

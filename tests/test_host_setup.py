@@ -173,8 +173,8 @@ def test_cw_source_value():
         assert abs(v - complex(math.cos(math.pi * t), -math.sin(math.pi * t))) < 1e-15
 
 
-def test_auto_decimation_config1():
-    """Survey config 1: fcen 1, fwidth 0.2 at dt 0.025 gives D = 16."""
+def test_auto_decimation_dipole_80():
+    """The 80^3 dipole: fcen 1, fwidth 0.2 at dt 0.025 gives D = 16."""
     src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=1.0, fwidth=0.2), component=kh.Ez())]
     mon = [kh.DFTMonitor(component=kh.Ez(), center=[0, 0, 0], size=[4, 4, 0], frequencies=[1.0])]
     sim = kh.Simulation(cell_size=[4, 4, 4], resolution=20, sources=src, monitors=mon,
