@@ -50,3 +50,13 @@ def test_version_and_footprint_without_device():
     d.nz = 0
     assert lib.kh_mem_footprint(C.byref(d), C.byref(n)) == -1
     assert b"extents" in lib.kh_last_error()
+
+
+def test_adjoint_entry_points_reject_bad_arguments_without_device():
+    lib = _lib.load()
+    lo = (C.c_int32 * 9)(*([1] * 9))
+    n = (C.c_int32 * 9)(*([0] * 9))
+    g = (C.c_double * 1)()
+    assert lib.kh_set_checkpoints(None, 100) != 0
+    assert lib.kh_adjoint_gradient(None, 0, None, None, lo, n, g) != 0
+    assert lib.kh_last_error().decode() != ""
