@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 4
+#define KH_ABI_VERSION 5
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -347,6 +347,13 @@ int kh_raster_component(int32_t precision, int32_t component, const int32_t cnt[
  * dft_id = -1 gives the sum of that over all monitors.
  * Replaces: dft_fields_norm (src/Simulation.jl:534-539). */
 int kh_dft_norm(kh_ctx *ctx, int32_t dft_id, double *out);
+
+/* out[i] = sqrt(sum |F|^2) of DFT monitor i, for all n_out monitors of the
+ * context (n_out must equal their number), with one synchronisation per 64
+ * monitors; the same values as kh_dft_norm monitor by monitor.
+ * Replaces: the per-monitor dft_fields_norm calls of the
+ * stop_when_dft_decayed callback (src/Simulation.jl:437-481). */
+int kh_dft_norms(kh_ctx *ctx, int32_t n_out, double *out);
 
 /* Zero fields, auxiliaries, DFT accumulators and the timestep; re-arm sources.
  * Replaces: reset_fields! (src/Simulation.jl:571-637). */

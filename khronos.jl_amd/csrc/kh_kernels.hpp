@@ -1925,7 +1925,10 @@ __device__ __forceinline__ void prism_surfpt(const RasterObj &o, const double *P
     }
     const bool inpoly = best <= 0;
     const double d_side = inpoly ? best : dmin;
-    const double ns[3] = {inpoly ? bnx : dnx / dmin, inpoly ? bny : dny / dmin, 0.0};
+    // a point on the boundary that rounding puts outside (best > 0 with
+    // dmin == 0, e.g. on a vertex) takes the best edge's normal
+    const bool edge_n = inpoly || !(dmin > 0);
+    const double ns[3] = {edge_n ? bnx : dnx / dmin, edge_n ? bny : dny / dmin, 0.0};
     const double sgn = ell >= 0 ? 1.0 : -1.0;
     const double d_cap = fabs(ell) - o.p[0] / 2;
     const double nc[3] = {0.0, 0.0, sgn};

@@ -125,6 +125,7 @@ struct CtxBase {
     virtual void read_dft(int id, double *host, size_t bytes) = 0;
     virtual void flux(int e1, int e2, int h1, int h2, int ax, double dA, double *out) = 0;
     virtual void dft_norm(int id, double *out) = 0;
+    virtual void dft_norms(int n_out, double *out) = 0;
     virtual void mode_overlap(int e1, int e2, int h1, int h2, int ax, int n1, int n2, const double *mode,
                               double dA, double *out) = 0;
     virtual void near2far(const int32_t *ids, int ax, double ns, const double *base, double d1, double d2,
@@ -185,6 +186,7 @@ struct Ctx final : CtxBase {
         int s[3], d[3];
         T *amp;
         double *wave = nullptr;  // device copy of the caller's table, (re, im) per step
+        std::vector<double> host;  // host copy of the same table
         int64_t nwave = 0, cap = 0;
     };
     std::vector<Source> srcs;
@@ -210,7 +212,7 @@ struct Ctx final : CtxBase {
     int mlo[3] = {1, 1, 1}, mn[3] = {0, 0, 0};
     int64_t mcells = 0;
 
-    double *scratch = nullptr;  // reduction partials
+    double *partials = nullptr;  // reduction partials (flux, overlap, norms)
     int64_t *clk = nullptr;     // device timestep
     static constexpr int NPART = 256;
 
@@ -314,7 +316,7 @@ struct Ctx final : CtxBase {
         }
         if (epsi[0]) rebuild_lut(1);
         if (mui[0]) rebuild_lut(0);
-        scratch = (double *)dalloc(NPART * 64 * sizeof(double));
+        partials = (double *)dalloc(NPART * 64 * sizeof(double));
         clk = (int64_t *)dalloc(256);
         graph_steps = env_int("KH_GRAPH_STEPS", 16);
         HIPCHK(hipStreamSynchronize(st));
@@ -446,6 +448,12 @@ struct Ctx final : CtxBase {
         require(id >= 0 && id < (int)srcs.size(), "bad source id");
         require(n >= 0 && (n == 0 || v), "bad waveform");
         Source &o = srcs[id];
+        // entries of steps already run that change make a re-run from a
+        // checkpoint differ from the run: the checkpoints go
+        const int64_t ran = std::min<int64_t>(ts, std::min<int64_t>(n, o.nwave));
+        if (ran > 0 && std::memcmp(o.host.data(), v, (size_t)(2 * ran) * sizeof(double)) != 0) drop_checkpoints();
+        if (n < o.nwave && n < ts) drop_checkpoints();
+        o.host.assign(v, v + 2 * n);
         if (n > o.cap) {
             // a new table: captured step graphs hold the old pointer
             int64_t cap = std::max<int64_t>(n, 2 * o.cap);
@@ -463,6 +471,9 @@ struct Ctx final : CtxBase {
     }
 
     void add_dft(int comp, const int32_t *s, const int32_t *d, int nf, const double *f, int dec, int32_t *id) override {
+        // a monitor accumulates from timestep 0 like the others: the graph
+        // period, the adjoint's seeding and kh_flux all assume so
+        require(ts == 0, "DFT monitors must be added at timestep 0 (before stepping or after kh_reset)", KH_E_STATE);
         require(comp >= 0 && comp < 12, "component out of range");
         require(nf >= 1 && nf <= 4096, "n_freq must be in 1..4096");
         require(dec >= 1, "decimation must be >= 1");
@@ -487,6 +498,10 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamSynchronize(st));
         dfts.push_back(m);
         *id = (int32_t)dfts.size() - 1;
+        if (planned) {  // after kh_reset: the DFT period may change
+            drop_graph();
+            plan_graph_len();
+        }
     }
 
     // bounding box (grid indices) of the non-zero entries of an (nx, ny, nz) array
@@ -1399,12 +1414,12 @@ struct Ctx final : CtxBase {
         require(nf <= NPART * 64, "too many frequencies");
         if (acc64)
             hipLaunchKernelGGL(kh::flux_kernel<double>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
-                               ax, t0, t1, n1, n2, dA, scratch);
+                               ax, t0, t1, n1, n2, dA, partials);
         else
             hipLaunchKernelGGL(kh::flux_kernel<T>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
-                               ax, t0, t1, n1, n2, dA, scratch);
+                               ax, t0, t1, n1, n2, dA, partials);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, scratch, nf * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out, partials, nf * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1436,12 +1451,12 @@ struct Ctx final : CtxBase {
         HIPCHK(hipMemcpyAsync(mode_buf, mode, bytes, hipMemcpyHostToDevice, st));
         if (acc64)
             hipLaunchKernelGGL(kh::mode_overlap_kernel<double>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2],
-                               fm[3], ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, scratch);
+                               fm[3], ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, partials);
         else
             hipLaunchKernelGGL(kh::mode_overlap_kernel<T>, dim3(nf), dim3(256), 0, st, fm[0], fm[1], fm[2], fm[3],
-                               ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, scratch);
+                               ax, t0, t1, n1, n2, nf, (const double *)mode_buf, dA, partials);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, scratch, (size_t)4 * nf * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out, partials, (size_t)4 * nf * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1616,6 +1631,33 @@ struct Ctx final : CtxBase {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // every monitor's norm, in registration order: launches into separate
+    // partial slots, one copy and one synchronisation per 64 monitors
+    void dft_norms(int n_out, double *out) override {
+        require(n_out == (int)dfts.size(), "n_out must equal the number of DFT monitors");
+        std::vector<double> part((size_t)NPART * 64);
+        for (int q0 = 0; q0 < n_out; q0 += 64) {
+            const int nq = std::min(64, n_out - q0);
+            for (int k = 0; k < nq; ++k) {
+                Dft &m = dfts[q0 + k];
+                const int64_t n = m.ncell * m.nf;
+                double *dst = partials + (size_t)k * NPART;
+                if (acc64)
+                    hipLaunchKernelGGL(kh::sumsq_kernel<double>, dim3(NPART), dim3(256), 0, st, (const double *)m.acc, n, dst);
+                else
+                    hipLaunchKernelGGL(kh::sumsq_kernel<T>, dim3(NPART), dim3(256), 0, st, (const T *)m.acc, n, dst);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipMemcpyAsync(part.data(), partials, (size_t)nq * NPART * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < nq; ++k) {
+                double s = 0.0;
+                for (int b = 0; b < NPART; ++b) s += part[(size_t)k * NPART + b];
+                out[q0 + k] = std::sqrt(s);
+            }
+        }
+    }
+
     void dft_norm(int id, double *out) override {
         require(id >= -1 && id < (int)dfts.size(), "bad dft id");
         double total = 0.0;
@@ -1625,11 +1667,11 @@ struct Ctx final : CtxBase {
             Dft &m = dfts[q];
             int64_t n = m.ncell * m.nf;
             if (acc64)
-                hipLaunchKernelGGL(kh::sumsq_kernel<double>, dim3(NPART), dim3(256), 0, st, (const double *)m.acc, n, scratch);
+                hipLaunchKernelGGL(kh::sumsq_kernel<double>, dim3(NPART), dim3(256), 0, st, (const double *)m.acc, n, partials);
             else
-                hipLaunchKernelGGL(kh::sumsq_kernel<T>, dim3(NPART), dim3(256), 0, st, (const T *)m.acc, n, scratch);
+                hipLaunchKernelGGL(kh::sumsq_kernel<T>, dim3(NPART), dim3(256), 0, st, (const T *)m.acc, n, partials);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(part.data(), scratch, NPART * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(part.data(), partials, NPART * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             double s = 0.0;
             for (double v : part) s += v;
@@ -1731,10 +1773,23 @@ struct Ctx final : CtxBase {
         std::vector<size_t> list;
         for (size_t i = 0; i < mons.size(); ++i)
             if ((mons[i]->comp < 3) == electric && (mons[i]->dec <= 1 || j % mons[i]->dec == 0)) list.push_back(i);
-        for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
+        // Monitors of one launch update lambda without atomics, so two that
+        // write a common cell of one component (overlapping objectives, or
+        // one monitor listed twice) go to successive launches, in list order.
+        auto meet = [](const Dft &a, const Dft &b) {
+            if (a.comp % 3 != b.comp % 3) return false;
+            for (int k = 0; k < 3; ++k)
+                if (a.s[k] + a.d[k] <= b.s[k] || b.s[k] + b.d[k] <= a.s[k]) return false;
+            return true;
+        };
+        for (size_t i0 = 0; i0 < list.size();) {
             kh::AdjSeedArgs<T> q{};
             int blocks = 0, nfmax = 1;
-            for (size_t i = i0; i < list.size() && q.nmon < kh::MAX_DFT; ++i) {
+            size_t i = i0;
+            for (; i < list.size() && q.nmon < kh::MAX_DFT; ++i) {
+                bool clash = false;
+                for (size_t r = i0; r < i && !clash; ++r) clash = meet(*mons[list[r]], *mons[list[i]]);
+                if (clash) break;
                 const Dft &m = *mons[list[i]];
                 kh::AdjSeed<T> &x = q.mon[q.nmon++];
                 x.g = adj_cot + off[list[i]];
@@ -1753,6 +1808,7 @@ struct Ctx final : CtxBase {
             q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
             q.sy = g.sy();
             q.sz = g.sz();
+            i0 = i;
             if (blocks == 0) continue;
             hipLaunchKernelGGL((kh::adjoint_seed_kernel<T>), dim3(blocks), dim3(256), (size_t)nfmax * 2 * sizeof(double),
                                st, q);
@@ -1933,7 +1989,7 @@ struct Ctx final : CtxBase {
         for (int part = 0; part < nparts; ++part)
             for (int a = 0; a < 3; ++a) {
                 const Part &f = fp[part];
-                T *arrs[8] = {f.E[a], f.H[a], f.D[a], f.B[a], f.UD[a], f.UB[a], f.WD[a], f.WB[a]};
+                T *arrs[10] = {f.E[a], f.H[a], f.D[a], f.B[a], f.UD[a], f.UB[a], f.WD[a], f.WB[a], f.CD[a], f.CB[a]};
                 for (T *p : arrs)
                     if (p) HIPCHK(hipMemsetAsync(p - g.front, 0, fb, st));
             }
@@ -2452,6 +2508,14 @@ int kh_dft_norm(kh_ctx *ctx, int32_t dft_id, double *out) {
         CTX_OK;
         require(out != nullptr, "null argument");
         ctx->impl->dft_norm(dft_id, out);
+    });
+}
+
+int kh_dft_norms(kh_ctx *ctx, int32_t n_out, double *out) {
+    return guarded([&] {
+        CTX_OK;
+        require(out != nullptr || n_out == 0, "null argument");
+        ctx->impl->dft_norms(n_out, out);
     });
 }
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "kh_flux": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                           C.c_double, C.c_void_p]),
     "kh_dft_norm": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "kh_dft_norms": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "kh_reset": (C.c_int, [C.c_void_p]),
     "kh_set_checkpoints": (C.c_int, [C.c_void_p, C.c_int32]),
     "kh_adjoint_gradient": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -100,7 +101,7 @@ class KhronosError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 4  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
+ABI_VERSION = 5  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
 
 
 def load(path: str = LIB_PATH):

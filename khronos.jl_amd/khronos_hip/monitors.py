@@ -145,7 +145,7 @@ def expand_monitors(sim, monitors):
         elif isinstance(m, (FluxMonitor, ModeMonitor)):
             normal = [i for i, s in enumerate(m.size) if float(s) == 0.0]
             if not normal:
-                raise ValueError(f"{type(m).__name__} size must have exactly one zero dimension")
+                raise ValueError(f"{type(m).__name__} is not planar: exactly one of its size entries must be 0")
             na = normal[0] + 1
             ecomps, hcomps = {1: ((Ey, Ez), (Hy, Hz)), 2: ((Ex, Ez), (Hx, Hz)), 3: ((Ex, Ey), (Hx, Hy))}[na]
             mk = lambda c: DFTMonitor(component=c(), center=list(m.center), size=list(m.size),

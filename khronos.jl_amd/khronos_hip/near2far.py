@@ -97,7 +97,7 @@ def init_near2far_monitor(sim, m: Near2FarMonitor):
     from .monitors import DFTMonitor
     normal = [i for i, s in enumerate(m.size) if float(s) == 0.0]
     if not normal:
-        raise ValueError("Near2FarMonitor size must have exactly one zero dimension")
+        raise ValueError("a near-to-far surface is a plane: exactly one size entry must be 0")
     if any(math.isinf(float(d)) for d in sim.delta_n):
         raise ValueError("near-to-far projection needs a 3D simulation")
     na = normal[0] + 1
@@ -153,7 +153,7 @@ def observation_points(md: Near2FarMonitorData) -> np.ndarray:
         return np.stack([md.r * np.sin(T) * np.cos(P), md.r * np.sin(T) * np.sin(P), md.r * np.cos(T)], axis=1)
     if md.observation_points is not None:
         return np.asarray(md.observation_points, dtype=np.float64).reshape(-1, 3)
-    raise ValueError("Near2FarMonitorData must have either theta/phi or observation_points")
+    raise ValueError("give the far-field directions (theta and phi) or the observation points")
 
 
 def _owner(md: Near2FarMonitorData):
@@ -307,9 +307,9 @@ def compute_far_field_layered(md, layer_stack, spectrum=None) -> np.ndarray:
     given."""
     md = md.monitor_data if isinstance(md, Near2FarMonitor) else md
     if md.normal_axis != 3:
-        raise ValueError("Layered far-field projection currently only supports z-normal monitors (normal_axis=3)")
+        raise ValueError("the layered projection needs a surface normal to z (normal_axis 3)")
     if md.theta is None or md.phi is None:
-        raise ValueError("Layered far-field projection requires theta/phi angular specification")
+        raise ValueError("the layered projection takes directions (theta and phi), not observation points")
     base, n1, n2, d1, d2, dA = surface_layout(md)
     if spectrum is None:
         owner, ids = _owner(md)

@@ -224,7 +224,7 @@ def run_until(sim: Simulation, until: float) -> Callable[[Simulation], bool]:
 def stop_when_dft_decayed(*, tolerance=1e-6, minimum_runtime=0.0, maximum_runtime=math.inf):
     """src/Simulation.jl:411-485: per-monitor relative change of the DFT norm."""
     if minimum_runtime > maximum_runtime:
-        raise ValueError("Minimum runtime cannot be greater than maximum runtime.")
+        raise ValueError(f"minimum_runtime {minimum_runtime} exceeds maximum_runtime {maximum_runtime}")
     state = {}
 
     def _stop(sim: Simulation) -> bool:
@@ -267,9 +267,9 @@ def run(sim: Simulation, *, until=None, until_after_sources=None, chunk_steps: i
     is evaluated after every step, as in the reference.
     """
     if until is None and until_after_sources is None:
-        raise ValueError("Must specify a terminating conditon in the run functions.")
+        raise ValueError("run needs a stop condition: pass until or until_after_sources")
     if until is not None and until_after_sources is not None:
-        raise ValueError("Must specify a single terminating conditon in the run functions.")
+        raise ValueError("run takes one stop condition: pass until or until_after_sources, not both")
     sim.prepare()
     wait = until_after_sources is not None
     cond = until if until is not None else until_after_sources

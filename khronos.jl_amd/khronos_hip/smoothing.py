@@ -170,8 +170,11 @@ def surfpt_prism(P, center, vertices, height):
     inpoly = best <= 0
     d_side = np.where(inpoly, best, dmin)
     n_side = np.zeros_like(P)
-    n_side[inpoly, :2] = bn[inpoly]
-    out_p = ~inpoly
+    # a point on the boundary that rounding puts outside (best > 0 with
+    # dmin == 0, e.g. on a vertex) takes the best edge's normal
+    edge_n = inpoly | ~(dmin > 0)
+    n_side[edge_n, :2] = bn[edge_n]
+    out_p = ~edge_n
     n_side[out_p, 0] = dn[out_p, 0] / dmin[out_p]
     n_side[out_p, 1] = dn[out_p, 1] / dmin[out_p]
     sgn = np.where(ell >= 0, 1.0, -1.0)

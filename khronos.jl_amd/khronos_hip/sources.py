@@ -50,7 +50,7 @@ class ContinuousWaveData(TimeSource):
         return complex(R(math.cos(float(ph))), -R(math.sin(float(ph))))
 
     def cutoff(self) -> float:
-        raise ValueError("Cutoff not possible with CW source...")
+        raise ValueError("a continuous-wave source has no cutoff time")
 
     @property
     def f_max(self):
@@ -224,7 +224,7 @@ class EquivalentSource(Source):
         self.amplitude = amplitude
         zeros = [a for a in range(3) if self.size[a] == 0.0]
         if len(zeros) != 1:
-            raise ValueError(f"The specified volume is not a plane (size={self.size})")
+            raise ValueError(f"expected a planar volume (one size entry equal to 0), got size={self.size}")
         self.normal_axis = zeros[0]
 
     def components(self):
@@ -282,7 +282,7 @@ class PlaneWaveSource(Source):
     def __init__(self, *, time_profile, center=(0.0, 0.0, 0.0), size=(0.0, 0.0, 0.0), amplitude=1.0,
                  ε=1.0, μ=1.0, polarization_angle=0.0, angle_theta=None, angle_phi=None, k_vector=None):
         if k_vector is None and angle_theta is None and angle_phi is None:
-            raise ValueError("Must either specify a `k_vector` or `angle_phi` and `angle_theta`.")
+            raise ValueError("give the propagation direction: k_vector, or angle_theta together with angle_phi")
         if k_vector is None:
             k_vector = [math.sin(angle_phi) * math.cos(angle_theta), math.sin(angle_phi) * math.sin(angle_theta),
                         math.cos(angle_phi)]
@@ -296,7 +296,7 @@ class PlaneWaveSource(Source):
         self.k_vector = kv / np.linalg.norm(kv)
         zeros = [a for a in range(3) if self.size[a] == 0.0]
         if len(zeros) != 1:
-            raise ValueError(f"The specified volume is not a plane (size={self.size})")
+            raise ValueError(f"expected a planar volume (one size entry equal to 0), got size={self.size}")
         self.normal_axis = int(np.argmin(self.size))
         n = np.zeros(3)
         n[self.normal_axis] = 1.0

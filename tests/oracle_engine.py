@@ -177,6 +177,9 @@ class OracleEngine:
     def sync(self):
         pass
 
+    def timestep(self):
+        return int(self.lib.orc_timestep(self.sim))
+
     def field_shape(self, code):
         d = np.zeros(3, dtype=np.int32)
         self.lib.orc_dims(self.sim, code, _p(d))

@@ -164,6 +164,31 @@ def test_square_prism_is_the_box():
     assert np.allclose(sp1, sp2, atol=1e-14) and np.allclose(n1, n2, atol=1e-11)
 
 
+def test_prism_normal_on_a_vertex_is_an_edge_normal():
+    # A point exactly on a vertex is on the boundary; rounding can put it a
+    # hair outside (largest edge distance > 0) while its distance to the
+    # polygon is 0.  The normal must then be an adjacent edge's unit normal,
+    # not 0/0 (before this rule, 801 of these 2000 triangles gave NaN).
+    rng = np.random.default_rng(0)
+    n_out = 0
+    for _ in range(2000):
+        v = rng.uniform(-1.0, 1.0, (3, 2))
+        if (v[1, 0] - v[0, 0]) * (v[2, 1] - v[0, 1]) - (v[1, 1] - v[0, 1]) * (v[2, 0] - v[0, 0]) < 0:
+            v = v[::-1].copy()
+        P = np.array([[v[k, 0], v[k, 1], 0.0] for k in range(3)])
+        sp, nrm = surfpt_prism(P, [0.0, 0.0, 0.0], v, 1.0)
+        assert np.all(np.isfinite(nrm)) and np.all(np.isfinite(sp))
+        for k in range(3):
+            edges = []
+            for i in ((k - 1) % 3, k):
+                e = v[(i + 1) % 3] - v[i]
+                edges.append(np.array([e[1], -e[0], 0.0]) / np.hypot(*e))
+            assert min(np.abs(nrm[k] - e).max() for e in edges) < 1e-15
+            assert np.allclose(sp[k], P[k], atol=1e-15)
+        n_out += 1
+    assert n_out == 2000
+
+
 def test_prism_raster_volume_and_smoothing():
     tri = [(0.0, 0.0), (1.2, 0.1), (0.3, 0.9)]
     objs = GD.polygon_to_objects(tri, -0.2, 0.2, kh.Material(ε=4.0))

@@ -251,3 +251,55 @@ def test_h_field_objective_and_decimated_monitor(monkeypatch):
     _note(f"[adjoint-tr] Hy + Ez objective, decimations {decs}: {res}")
     assert max(decs) > 1
     assert max(r[2] for r in res) < 1e-6, res
+
+
+def test_overlapping_same_component_objectives():
+    """Two Ez objectives whose planes overlap (and one listed twice): their
+    seeds write common cells of one adjoint array, so they go to successive
+    launches.  Central differences pin the sum, and two sweeps of the same
+    forward run are bit-identical (no lost updates that vary run to run)."""
+    opt = box3d_problem()
+    plane = Volume([0.8, 0, 0], [0, 1.2, 0.5])
+    part = Volume([0.8, 0.1, 0], [0, 0.6, 0.3])
+    opt.objective_arguments = [A.FourierFieldsObjective(volume=plane, component=kh.Ez()),
+                               A.FourierFieldsObjective(volume=part, component=kh.Ez()),
+                               A.FourierFieldsObjective(volume=plane, component=kh.Ez())]
+    opt.objective_functions = [lambda a, b, c: torch.sum(torch.abs(a) ** 2) + 3.0 * torch.sum(torch.abs(b) ** 2)
+                               + 0.5 * torch.sum(torch.abs(c) ** 2)]
+    _time_reversed(opt, 50)
+    rho = 0.3 + 0.4 * np.random.default_rng(21).random(20)
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] overlapping Ez objectives (adjoint, fd, rel): {res}")
+    assert max(r[2] for r in res) < 1e-6, res
+    g2 = opt.calculate_gradient()[:, 0]
+    opt.adjoint_run()
+    g3 = opt.calculate_gradient()[:, 0]
+    assert np.array_equal(g2, g3)
+
+
+def pml_design_2d(mode):
+    """2D TE (Ey line source; Ex, Ey, Hz) with the design region reaching
+    into the +x and both y PMLs, so Ex has a non-zero own-axis sigma in part
+    of the design (the W stage carries the update there) and Ey in another."""
+    pml, cx, cy = 1.0, 4.0, 2.6
+    geo = [kh.Object(kh.Box([0, 0, 0], [cx + 1, cy + 1, 10.0]), kh.Material(ε=1.0))]
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ey(),
+                            center=[-0.6, 0.0, 0.0], size=[0.0, 0.6, 0.0])]
+    sim = kh.Simulation(cell_size=[cx, cy, 0.0], resolution=10, sources=src, geometry=geo,
+                        boundaries=[[pml, pml], [pml, pml]], precision="f64", mode=mode)
+    dr = A.DesignRegion(volume=Volume([1.0, 0, 0], [1.0, 0.8, 0.0]), design_parameters=np.full(24, 0.5),
+                        grid_size=(6, 4), ε_min=1.0, ε_max=4.0)
+    objs = [A.FourierFieldsObjective(volume=Volume([0.2, 0, 0], [0, 0.6, 0.0]), component=kh.Ey()),
+            A.FourierFieldsObjective(volume=Volume([0.2, 0, 0], [0, 0.6, 0.0]), component=kh.Ex())]
+    f = lambda ey, ex: torch.sum(torch.abs(ey) ** 2) + 10.0 * torch.sum(torch.abs(ex) ** 2)
+    return A.OptimizationProblem(sim, [f], objs, [dr], [0.45, 0.55], until=30.0)
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_design_region_inside_the_pml(mode):
+    opt = _time_reversed(pml_design_2d(mode), 40)
+    rho = 0.3 + 0.4 * np.random.default_rng(17).random(24)
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] design in the PML, {mode} (adjoint, fd, rel): {res}")
+    assert np.any(g != 0)
+    assert max(r[2] for r in res) < 1e-6, res
