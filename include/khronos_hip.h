@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 5
+#define KH_ABI_VERSION 6
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -342,6 +342,25 @@ typedef struct kh_raster_object {
 int kh_raster_component(int32_t precision, int32_t component, const int32_t cnt[3], const double *xs,
                         const double *ys, const double *zs, const kh_raster_object *objs, int32_t n_objs,
                         int32_t smoothing, const double half[3], void *out);
+
+/* Bytes of device staging available to kh_raster_component_noalloc. */
+#define KH_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* Allocation-free variant of kh_raster_component.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (a
+ * serving loop preparing requests beside running streams, where hipMalloc
+ * and hipFree would synchronise the device; code under a harness that forbids
+ * allocation).  Same arguments and values as kh_raster_component, except:
+ * out_device is device memory of cnt[0] * cnt[1] * cnt[2] values of the
+ * precision type, and the work runs on `stream` (a hipStream_t; NULL for the
+ * null stream); the call returns once that stream has finished it.  Staging
+ * (objects, coordinates, prism vertices and, with smoothing, the unsmoothed
+ * raster) lives in a device array of KH_NOALLOC_SCRATCH_BYTES (64 MiB) that
+ * each GPU holds until the process ends; a call that needs more fails with
+ * KH_E_INVALID.  Calls that overlap on one GPU share that array and race. */
+int kh_raster_component_noalloc(int32_t precision, int32_t component, const int32_t cnt[3], const double *xs,
+                                const double *ys, const double *zs, const kh_raster_object *objs, int32_t n_objs,
+                                int32_t smoothing, const double half[3], void *out_device, void *stream);
 
 /* sqrt(sum |F|^2) of DFT monitor `dft_id` in float64, reduced on the device;
  * dft_id = -1 gives the sum of that over all monitors.

@@ -2118,72 +2118,154 @@ struct kh_server {
 };
 
 namespace {
-template <typename T>
-void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_t *cnt, const double *xs,
-                const double *ys, const double *zs, int smoothing, const double *half, void *out) {
-    const int64_t n = (int64_t)cnt[0] * cnt[1] * cnt[2];
-    std::vector<kh::RasterObj> ho(std::max(1, n_objs));
-    std::vector<double> verts;  // all prism vertices, one device copy
-    std::vector<size_t> vfirst(std::max(1, n_objs), 0);
+// Host copy of an object list in the kernels' form; prism vertex lists are
+// gathered into one array (one device copy).
+struct RasterInput {
+    std::vector<kh::RasterObj> ho;
+    std::vector<double> verts;
+    std::vector<size_t> vfirst;
+};
+
+RasterInput pack_objects(const kh_raster_object *objs, int n_objs) {
+    RasterInput in;
+    in.ho.resize(std::max(1, n_objs));
+    in.vfirst.assign(std::max(1, n_objs), 0);
     for (int o = 0; o < n_objs; ++o) {
         const kh_raster_object &r = objs[o];
         require(r.shape >= 0 && r.shape <= 3, "shape must be KH_SHAPE_BOX, _SPHERE, _CYLINDER or _PRISM");
         require(r.axis >= 0 && r.axis <= 2, "cylinder axis must be 0..2");
-        ho[o] = kh::RasterObj{};
+        in.ho[o] = kh::RasterObj{};
         if (r.shape == KH_SHAPE_PRISM) {
             require(r.n_vertices >= 3 && r.n_vertices <= 1000000 && r.vertices != nullptr,
                     "a prism needs at least 3 vertices");
-            vfirst[o] = verts.size();
-            verts.insert(verts.end(), r.vertices, r.vertices + 2 * (size_t)r.n_vertices);
-            ho[o].nv = r.n_vertices;
+            in.vfirst[o] = in.verts.size();
+            in.verts.insert(in.verts.end(), r.vertices, r.vertices + 2 * (size_t)r.n_vertices);
+            in.ho[o].nv = r.n_vertices;
         }
-        ho[o].shape = r.shape;
-        ho[o].axis = r.axis;
+        in.ho[o].shape = r.shape;
+        in.ho[o].axis = r.axis;
         for (int a = 0; a < 3; ++a) {
-            ho[o].c[a] = r.center[a];
-            ho[o].p[a] = r.size[a];
+            in.ho[o].c[a] = r.center[a];
+            in.ho[o].p[a] = r.size[a];
         }
-        for (int c = 0; c < 6; ++c) ho[o].value[c] = r.value[c];
+        for (int c = 0; c < 6; ++c) in.ho[o].value[c] = r.value[c];
     }
+    return in;
+}
+
+constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Device staging of one call: objects | coordinates | prism vertices |
+// `rasters` component arrays of `ab` bytes, each part 256-byte aligned.
+struct RasterStage {
+    size_t ob, cb, vb, ab, total;
+    RasterStage(const RasterInput &in, const int32_t *cnt, size_t elem, int rasters) {
+        ob = in.ho.size() * sizeof(kh::RasterObj);
+        cb = (size_t)(cnt[0] + cnt[1] + cnt[2]) * sizeof(double);
+        vb = in.verts.size() * sizeof(double);
+        ab = (size_t)cnt[0] * cnt[1] * cnt[2] * elem;
+        total = round256(ob) + round256(cb) + round256(vb) + rasters * round256(ab);
+    }
+};
+
+// Copies the inputs into `stage` (device memory of RasterStage::total bytes)
+// on stream `st` and paints the component into `paint`; with smoothing the
+// smoothed values go to `smoothed` (then `paint` is the unsmoothed raster).
+template <typename T>
+void raster_launch(RasterInput &in, const RasterStage &L, char *stage, int n_objs, int comp, const int32_t *cnt,
+                   const double *xs, const double *ys, const double *zs, bool smooth, int smoothing,
+                   const double *half, T *paint, T *smoothed, hipStream_t st) {
+    kh::RasterObj *d_obj = (kh::RasterObj *)stage;
+    double *d_x = (double *)(stage + round256(L.ob));
+    double *d_y = d_x + cnt[0], *d_z = d_y + cnt[1];
+    double *d_v = (double *)((char *)d_x + round256(L.cb));
+    for (int o = 0; o < n_objs; ++o)
+        if (in.ho[o].shape == KH_SHAPE_PRISM) in.ho[o].v = d_v + in.vfirst[o];
+    if (L.vb) HIPCHK(hipMemcpyAsync(d_v, in.verts.data(), L.vb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_obj, in.ho.data(), L.ob, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_x, xs, cnt[0] * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_y, ys, cnt[1] * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_z, zs, cnt[2] * sizeof(double), hipMemcpyHostToDevice, st));
+    const int64_t n = (int64_t)cnt[0] * cnt[1] * cnt[2];
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(kh::raster_kernel<T>, dim3(blocks), dim3(256), 0, st, (const kh::RasterObj *)d_obj, n_objs,
+                       comp, cnt[0], cnt[1], cnt[2], (const double *)d_x, (const double *)d_y, (const double *)d_z,
+                       paint);
+    HIPCHK(hipGetLastError());
+    if (smooth) {
+        hipLaunchKernelGGL(kh::smooth_kernel<T>, dim3(blocks), dim3(256), 0, st, (const kh::RasterObj *)d_obj,
+                           n_objs, comp, smoothing == 2 ? 1 : 0, cnt[0], cnt[1], cnt[2], (const double *)d_x,
+                           (const double *)d_y, (const double *)d_z, half[0], half[1], half[2], (const T *)paint,
+                           smoothed);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+template <typename T>
+void raster_run(const kh_raster_object *objs, int n_objs, int comp, const int32_t *cnt, const double *xs,
+                const double *ys, const double *zs, int smoothing, const double *half, void *out) {
+    RasterInput in = pack_objects(objs, n_objs);
     const bool smooth = smoothing != 0 && comp < 3;
-    // device buffers: objects, coordinates, raster (+ smoothed copy)
-    const size_t ob = ho.size() * sizeof(kh::RasterObj);
-    const size_t cb = (size_t)(cnt[0] + cnt[1] + cnt[2]) * sizeof(double);
-    const size_t ab = (size_t)n * sizeof(T);
+    const RasterStage L(in, cnt, sizeof(T), smooth ? 2 : 1);
     void *buf = nullptr;
-    const size_t vb = verts.size() * sizeof(double);
-    const size_t total = ob + cb + ab * (smooth ? 2 : 1) + vb + 768;
-    HIPCHK(hipMalloc(&buf, total));
+    HIPCHK(hipMalloc(&buf, L.total));
     struct Free {
         void *p;
         ~Free() { (void)hipFree(p); }
     } guard{buf};
-    char *b = (char *)buf;
-    kh::RasterObj *d_obj = (kh::RasterObj *)b;
-    double *d_x = (double *)(b + ((ob + 255) / 256) * 256);
-    double *d_y = d_x + cnt[0], *d_z = d_y + cnt[1];
-    T *d_a = (T *)((char *)d_x + ((cb + 255) / 256) * 256);
-    T *d_s = d_a + n;
-    double *d_v = (double *)((char *)d_a + ((ab * (smooth ? 2 : 1) + 255) / 256) * 256);
-    for (int o = 0; o < n_objs; ++o)
-        if (ho[o].shape == KH_SHAPE_PRISM) ho[o].v = d_v + vfirst[o];
-    if (vb) HIPCHK(hipMemcpy(d_v, verts.data(), vb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_obj, ho.data(), ob, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_x, xs, cnt[0] * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_y, ys, cnt[1] * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_z, zs, cnt[2] * sizeof(double), hipMemcpyHostToDevice));
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(kh::raster_kernel<T>, dim3(blocks), dim3(256), 0, 0, (const kh::RasterObj *)d_obj, n_objs,
-                       comp, cnt[0], cnt[1], cnt[2], (const double *)d_x, (const double *)d_y, (const double *)d_z,
-                       d_a);
-    HIPCHK(hipGetLastError());
-    if (smooth) {
-        hipLaunchKernelGGL(kh::smooth_kernel<T>, dim3(blocks), dim3(256), 0, 0, (const kh::RasterObj *)d_obj,
-                           n_objs, comp, smoothing == 2 ? 1 : 0, cnt[0], cnt[1], cnt[2], (const double *)d_x,
-                           (const double *)d_y, (const double *)d_z, half[0], half[1], half[2], (const T *)d_a, d_s);
-        HIPCHK(hipGetLastError());
+    char *stage = (char *)buf;
+    T *d_a = (T *)(stage + L.total - (smooth ? 2 : 1) * round256(L.ab));
+    T *d_s = smooth ? (T *)((char *)d_a + round256(L.ab)) : nullptr;
+    raster_launch<T>(in, L, stage, n_objs, comp, cnt, xs, ys, zs, smooth, smoothing, half, d_a, d_s, 0);
+    HIPCHK(hipMemcpy(out, smooth ? d_s : d_a, L.ab, hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+// The allocation-free variant's staging: a device array of fixed size; every
+// GPU holds its own copy (at its own address) until the process ends.  Host
+// code never passes `scratch` itself (a host-side stand-in), only the device
+// address looked up once per device below.
+static __device__ __align__(256) unsigned char scratch[KH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_dev[256] = {};
+
+void *scratch_on_current_device() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    require(dev >= 0 && dev < 256, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            throw KhError(KH_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
     }
-    HIPCHK(hipMemcpy(out, smooth ? d_s : d_a, ab, hipMemcpyDeviceToHost));
+    return g_scratch_dev[dev];
+}
+
+template <typename T>
+void raster_run_noalloc(const kh_raster_object *objs, int n_objs, int comp, const int32_t *cnt, const double *xs,
+                        const double *ys, const double *zs, int smoothing, const double *half, void *out_dev,
+                        hipStream_t st) {
+    RasterInput in = pack_objects(objs, n_objs);
+    const bool smooth = smoothing != 0 && comp < 3;
+    // staging: the inputs and, with smoothing, the unsmoothed raster; the
+    // painted (or smoothed) values go straight to the caller's array
+    const RasterStage L(in, cnt, sizeof(T), smooth ? 1 : 0);
+    require(L.total <= (size_t)KH_NOALLOC_SCRATCH_BYTES,
+            "kh_raster_component_noalloc: the call needs " + std::to_string(L.total) +
+                " bytes of staging, more than the " + std::to_string((size_t)KH_NOALLOC_SCRATCH_BYTES) +
+                " bytes of the library's scratch array; use kh_raster_component");
+    char *stage = (char *)scratch_on_current_device();
+    T *paint = smooth ? (T *)(stage + L.total - round256(L.ab)) : (T *)out_dev;
+    raster_launch<T>(in, L, stage, n_objs, comp, cnt, xs, ys, zs, smooth, smoothing, half, paint,
+                     smooth ? (T *)out_dev : nullptr, st);
+    // the host inputs (object list, coordinates) are read by the copies above
+    HIPCHK(hipStreamSynchronize(st));
 }
 }  // namespace
 
@@ -2500,6 +2582,44 @@ int kh_raster_component(int32_t precision, int32_t component, const int32_t cnt[
             raster_run<float>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out);
         else
             raster_run<double>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out);
+    });
+}
+
+int kh_raster_component_noalloc(int32_t precision, int32_t component, const int32_t cnt[3], const double *xs,
+                                const double *ys, const double *zs, const kh_raster_object *objs, int32_t n_objs,
+                                int32_t smoothing, const double half[3], void *out_device, void *stream) {
+    return guarded([&] {
+        require(precision == KH_F32 || precision == KH_F64, "precision must be KH_F32 or KH_F64");
+        require(component >= 0 && component < 6, "component must be 0..5");
+        require(cnt != nullptr && xs != nullptr && ys != nullptr && zs != nullptr && out_device != nullptr,
+                "null argument");
+        require(cnt[0] >= 1 && cnt[1] >= 1 && cnt[2] >= 1, "empty component array");
+        require((int64_t)cnt[0] * cnt[1] * cnt[2] < (1LL << 40), "component array too large");
+        require(n_objs >= 0 && (n_objs == 0 || objs != nullptr), "bad object list");
+        require(smoothing >= 0 && smoothing <= 2, "smoothing must be 0 (none), 1 (volume average), 2 (anisotropic)");
+        require(smoothing == 0 || component >= 3 || half != nullptr, "smoothing needs the half cell size");
+        // out_device must be device memory of the current device holding the
+        // whole component array (the kernels write every value)
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipPointerAttribute_t attr{};
+        const bool is_dev = hipPointerGetAttributes(&attr, out_device) == hipSuccess &&
+                            attr.type == hipMemoryTypeDevice && attr.device == dev;
+        (void)hipGetLastError();
+        require(is_dev, "out_device must be device memory of the current device");
+        hipDeviceptr_t base = nullptr;
+        size_t range = 0;
+        const size_t need = (size_t)cnt[0] * cnt[1] * cnt[2] * (precision == KH_F32 ? 4 : 8);
+        const bool fits = hipMemGetAddressRange(&base, &range, (hipDeviceptr_t)out_device) == hipSuccess &&
+                          (const char *)out_device + need <= (const char *)base + range;
+        (void)hipGetLastError();
+        require(fits, "out_device holds fewer than cnt[0] * cnt[1] * cnt[2] values");
+        if (precision == KH_F32)
+            raster_run_noalloc<float>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out_device,
+                                      (hipStream_t)stream);
+        else
+            raster_run_noalloc<double>(objs, n_objs, component, cnt, xs, ys, zs, smoothing, half, out_device,
+                                       (hipStream_t)stream);
     });
 }
 

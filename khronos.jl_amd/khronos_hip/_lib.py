@@ -60,6 +60,9 @@ SIGNATURES = {
                                        C.c_int32, C.c_void_p]),
     "kh_raster_component": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "kh_raster_component_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]),
     "kh_server_create": (C.c_int, [C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
     "kh_server_destroy": (C.c_int, [C.c_void_p]),
     "kh_server_admit": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
@@ -101,7 +104,7 @@ class KhronosError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 5  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
+ABI_VERSION = 6  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
 
 
 def load(path: str = LIB_PATH):
