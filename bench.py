@@ -12,9 +12,16 @@ timed region starts.
     python bench.py --near2far | --serving | --adjoint   (the other forms)
 
 Prints one JSON line.  Mcell-updates/s = Nx*Ny*Nz*K / wall_s / 1e6
-(src/Simulation.jl:517-518).  The roofline entry uses the E-update kernel
-(the dominant one): algorithmic bytes per launch / mean launch time from HIP
-events recorded around each launch on the library's stream.
+(src/Simulation.jl:517-518).  The warm-up runs on to a step where the flux
+planes sample, the graphs the window replays are captured before it, and
+``config.window`` says what the window ran (graph replays, eager steps, DFT
+launches).  The roofline entry uses the E-update kernel (the dominant one):
+algorithmic bytes per launch / mean launch time from HIP events recorded
+around each launch on the library's stream, beside the measured HBM traffic
+per launch (committed counter summary) over the same time.  Also in the
+line: the 80^3 FP64 dipole's rate, a short Poisson serving stream (served vs
+one at a time, p50/p99, ideal-overlap bound) and the CPU oracle on all the
+process's cores (at most 16) and on one core.
 """
 from __future__ import annotations
 
@@ -33,17 +40,25 @@ sys.path.insert(0, os.path.join(ROOT, "khronos.jl_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(spec, cores: int, budget_s: float = 15.0):
-    """Time the CPU oracle (C restatement of the reference CPU path, OpenMP)
-    on the same workload for a bounded number of steps."""
+def _omp_threads(n: int):
+    """Thread count of the oracle's OpenMP regions from here on (the oracle
+    runs its parallel loops on the calling thread's team)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(spec, cores: int, budget_s: float = 12.0, warmup: int = 10, label: str = "strip-waveguide"):
+    """Time the CPU oracle (C restatement of the reference CPU path, OpenMP on
+    ``cores`` threads) on the same workload: ``warmup`` untimed steps, as
+    run_benchmark does (src/Simulation.jl:492-527), then a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_engine import OracleEngine  # the checker, used here only as the CPU baseline
+    _omp_threads(cores)
     eng = OracleEngine(spec)
-    eng.step(1)  # warm-up: first touch of every array
     t0 = time.perf_counter()
-    eng.step(1)
-    t1 = time.perf_counter() - t0
-    n = max(2, min(40, int(budget_s / max(t1, 1e-6))))
+    eng.step(warmup)
+    t1 = (time.perf_counter() - t0) / max(1, warmup)
+    n = max(3, min(200, int(budget_s / max(t1, 1e-6))))
     t0 = time.perf_counter()
     eng.step(n)
     dt = time.perf_counter() - t0
@@ -54,9 +69,78 @@ def cpu_baseline(spec, cores: int, budget_s: float = 15.0):
         "unit": "Mcell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"strip-waveguide 384x192x160 {'FP32' if np.dtype(spec.real).itemsize == 4 else 'FP64'}, "
-                  f"{n} steps after 2 warm-up steps "
-                  f"(C oracle, -O3 -fopenmp, {cores} threads)",
+        "sample": f"{label} {spec.N[0]}x{spec.N[1]}x{spec.N[2]} "
+                  f"{'FP32' if np.dtype(spec.real).itemsize == 4 else 'FP64'}, {n} steps after {warmup} warm-up "
+                  f"steps (C oracle, -O3 -fopenmp, {cores} thread{'s' if cores > 1 else ''})",
+    }
+
+
+def dft_period(spec) -> int:
+    """lcm of the DFT monitors' decimations (the library's graph period)."""
+    p = 1
+    for md in spec.dfts:
+        p = p * max(1, md.decimation) // math.gcd(p, max(1, md.decimation))
+    return p
+
+
+def config1_leg(steps: int = 500):
+    """The 80^3 FP64 dipole (vacuum, 10-cell PML, Ez Gaussian dipole, 5-frequency
+    DFT plane): Mcell-updates/s over steps 10..steps, as run_benchmark times it
+    (src/Simulation.jl:492-527)."""
+    from khronos_hip.workloads import dipole_80
+    sim = dipole_80(precision="f64", mode="fast")
+    sim.prepare()
+    sim.step(10)
+    sim.engine.prepare_graphs(steps - 10)
+    sim.engine.sync()
+    t0 = time.perf_counter()
+    sim.step(steps - 10)
+    sim.engine.sync()
+    dt = time.perf_counter() - t0
+    rate = sim.Nx * sim.Ny * sim.Nz * (steps - 10) / dt / 1e6
+    sim.engine.close()
+    return {"workload": "dipole 80^3 FP64, 10-cell PML, Ez Gaussian dipole, 5-frequency DFT plane, "
+                        f"steps 10..{steps}", "value": round(rate, 1), "unit": "Mcell-updates/s",
+            "reference_gh200": 640.1, "reference_source": "benchmark/dipole.yml:380 (Khronos.jl, FP64, 80^3)"}
+
+
+def serving_leg(n_requests: int, rate: float, seed: int = 0):
+    """A short Poisson stream of the serving workload, continuous batching vs
+    one at a time, with the ideal-overlap bound: every request stepped at the
+    rate the largest grid of the stream reaches alone, no earlier than it
+    arrives."""
+    import torch
+    from khronos_hip import serving
+    reqs = serving.serving_trace(n_requests, seed=seed, rate=rate)
+    free = torch.cuda.mem_get_info()[0]
+    served = serving.Server(budget_bytes=0.85 * free, max_active=6, chunk=256).run(reqs)
+    solo = serving.run_one_at_a_time(reqs)
+    big = max(reqs, key=lambda r: r.cells)
+    sim = big.make()
+    sim.prepare()
+    sim.step(32)
+    sim.engine.prepare_graphs(320)
+    sim.engine.sync()
+    t0 = time.perf_counter()
+    sim.step(320)
+    sim.engine.sync()
+    r_max = big.cells * 320 / (time.perf_counter() - t0)
+    sim.engine.close()
+    t_first = min(r.arrival for r in reqs)
+    work = sum(r.cells * r.steps for r in reqs)
+    finish = max(work / r_max + t_first, max(r.arrival + r.cells * r.steps / r_max for r in reqs))
+    bound = work / (finish - t_first) / 1e6
+    same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
+    return {
+        "workload": f"{len(reqs)} requests (grids 64^3..256^3, 1000-3000 steps, FP32 on the FP64 setup, "
+                    f"20-frequency flux), Poisson arrivals at {rate} requests/s (seed {seed})",
+        "served": round(served.throughput, 1), "one_at_a_time": round(solo.throughput, 1),
+        "unit": "Mcell-updates/s", "speedup": round(served.throughput / solo.throughput, 3),
+        "ideal_overlap_bound": round(bound, 1), "served_over_bound": round(served.throughput / bound, 3),
+        "largest_grid_alone": round(r_max / 1e6, 1),
+        "latency_s": {"served_p50": round(served.latency_pct(50), 3), "served_p99": round(served.latency_pct(99), 3),
+                      "solo_p50": round(solo.latency_pct(50), 3), "solo_p99": round(solo.latency_pct(99), 3)},
+        "fluxes_identical": bool(same),
     }
 
 
@@ -279,6 +363,10 @@ def main(argv=None):
     ap.add_argument("--max-active", type=int, default=6)
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="headline only: skip the config-1 dipole and the short serving stream")
+    ap.add_argument("--leg-requests", type=int, default=24, help="requests of the serving leg")
+    ap.add_argument("--leg-rate", type=float, default=8.0, help="Poisson arrival rate of the serving leg")
     args = ap.parse_args(argv)
     if args.serving:
         return serving_bench(args)
@@ -300,7 +388,8 @@ def main(argv=None):
                          "(multi-GPU runs are not part of this build); use --gpus 1\n")
         return 2
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    # the host cores this process may use (at most 16, the GPU box's share)
+    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
 
     import torch
@@ -319,9 +408,18 @@ def main(argv=None):
     ab = algorithmic_bytes(sim.spec)
 
     eng.set_graph(args.graph_steps)
+    # warm-up: W steps, then on to the next multiple of the DFT period, so
+    # that the timed window starts on a step where the flux planes sample
+    # (the window then holds its share of DFT launches)
+    period = dft_period(sim.spec)
     sim.step(args.warmup)
+    extra = (-sim.timestep) % period
+    sim.step(extra)
+    eng.prepare_graphs(args.steps)  # capture outside the timed window
     eng.sync()
     torch.cuda.synchronize()
+    eng.launch_counts()  # clear
+    t_start = sim.timestep
 
     # timed region: K steps as the library runs them (captured step graphs)
     t0 = time.perf_counter()
@@ -329,6 +427,7 @@ def main(argv=None):
     eng.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    counts = eng.launch_counts()
 
     # per-kernel durations: HIP events around every launch on the library's
     # stream (events need eager launches, so this is a separate run)
@@ -359,6 +458,7 @@ def main(argv=None):
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], f"cascade_e_kernel<{ctype}>"
         trace_name = f"kh::cascade_e_kernel<{ctype}>"
     achieved = dom_bytes / (dom * 1e-3) / 1e9 if dom > 0 else 0.0
+    traffic = measured_traffic(trace_name) if args.precision == "f32" else None
     b_alg = (ab["H"] + ab["E"]) / cells
     step_kernel_ms = sum(v[0] for v in tm.values()) / k_steps
 
@@ -382,6 +482,8 @@ def main(argv=None):
             "cells": cells,
             "mode": args.mode,
             "graph_steps": args.graph_steps,
+            "warmup_steps_run": args.warmup + extra,
+            "window": {"timesteps": [t_start, t_start + args.steps - 1], "dft_period": period, **counts},
             "B_alg_bytes_per_cell_update": round(b_alg, 2),
             "kernel_ms": {k: round(v, 5) for k, v in avg.items()},
             "kernel_ms_per_step": round(step_kernel_ms, 5),
@@ -393,16 +495,26 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(trace_name) if args.precision == "f32" else None,
+            "traffic": traffic,
+            "traffic_achieved": round(traffic / (dom * 1e-3) / 1e9, 1) if traffic and dom > 0 else None,
+            "traffic_frac": round(traffic / (dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic and dom > 0 else None,
             "kernel": dom_name,
             "alg_bytes_per_launch": dom_bytes,
             "kernel_ms": round(dom, 5),
         },
     }
+    if not args.no_legs:
+        sys.stderr.write("config-1 leg (80^3 FP64 dipole)\n")
+        line["config1_dipole_80_f64"] = config1_leg()
+        sys.stderr.write(f"serving leg ({args.leg_requests} Poisson requests)\n")
+        line["serving"] = serving_leg(args.leg_requests, args.leg_rate)
     if not args.no_cpu_baseline:
         spec = sim.spec
+        eng.close()
         del sim, eng
+        sys.stderr.write(f"CPU baseline ({cores} threads, then 1)\n")
         line["cpu_baseline"] = cpu_baseline(spec, cores)
+        line["cpu_baseline_1core"] = cpu_baseline(spec, 1, budget_s=10.0)
     print(json.dumps(line), flush=True)
     return 0
 

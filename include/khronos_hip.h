@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 6
+#define KH_ABI_VERSION 7
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -198,14 +198,30 @@ int kh_write_pole(kh_ctx *ctx, int32_t pole_id, int32_t which, int32_t component
 int kh_step(kh_ctx *ctx, int64_t n_steps);
 
 /* Steps per captured graph (0 = launch every kernel of every step).  kh_step
- * captures one block of that many steps as a hipGraph and replays it while at
- * least a block remains; the launches of a step carry no per-step arguments
- * (the timestep, waveform values and DFT gates are read on the device), so
- * sources and monitors stay inside the graph.  Graphs are not used while
- * kernel timing is on.  Default 16, or the environment's KH_GRAPH_STEPS.
+ * captures blocks of steps as hipGraphs and replays them; the launches of a
+ * step carry no per-step arguments (the timestep, waveform values and DFT
+ * gates are read on the device), so sources and monitors stay inside the
+ * graph.  The main block is steps_per_graph rounded up to the DFT period
+ * (the lcm of the monitors' decimations) and starts on a multiple of it;
+ * shorter or unaligned stretches replay blocks of steps_per_graph steps
+ * captured for their start phase within the period (the last few phases are
+ * kept); the rest runs eagerly.  Graphs are not used while kernel timing is
+ * on.  Default 16, or the environment's KH_GRAPH_STEPS.
  * Replaces: _try_capture_graphs! and the graph replay branch of step!
  * (src/Kernels/Kernels.jl:40-61, :99-145; KHRONOS_CUDA_GRAPHS=0 there). */
 int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph);
+
+/* Capture now the graphs that kh_step(ctx, n_steps) would replay from the
+ * current state, without running any step, so that a timed kh_step replays
+ * graphs captured beforehand.  No counterpart in the reference (its graphs
+ * are captured on first use, src/Kernels/Kernels.jl:40-61). */
+int kh_prepare_graphs(kh_ctx *ctx, int64_t n_steps);
+
+/* What kh_step ran since the last call (then cleared): counts[0] graph
+ * replays, [1] steps inside those replays, [2] steps launched eagerly,
+ * [3] DFT kernel launches (inside replays and eager).  For measurement;
+ * no counterpart in the reference. */
+int kh_get_launch_counts(kh_ctx *ctx, int64_t counts[4]);
 
 /* Whether all queued work of the context has finished (*idle = 1), without
  * waiting; a scheduler serving many contexts polls this to keep each one's

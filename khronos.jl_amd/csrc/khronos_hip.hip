@@ -116,6 +116,8 @@ struct CtxBase {
     virtual void write_pole(int id, int which, int comp, const void *host, size_t bytes) = 0;
     virtual void step(int64_t n) = 0;
     virtual void set_graph(int steps) = 0;
+    virtual void prepare_graphs(int64_t n) = 0;
+    virtual void launch_counts(int64_t *out) = 0;
     virtual void sync() = 0;
     virtual bool idle() = 0;
     virtual int64_t timestep() const = 0;
@@ -1164,6 +1166,7 @@ struct Ctx final : CtxBase {
             if (acc64) hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3(blocks), dim3(256), shm, st, q);
             else hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3(blocks), dim3(256), shm, st, q);
             HIPCHK(hipGetLastError());
+            ++(capturing ? cap_dfts : n_dft);
             if (timing) {
                 HIPCHK(hipEventRecord(ev.b, st));
                 evs.push_back(ev);
@@ -1202,35 +1205,65 @@ struct Ctx final : CtxBase {
     }
 
     // ---------------------------------------------------------------- graphs
-    // A block of graph_steps steps is captured once and replayed (the
-    // reference captures its H and E updates after the sources switch off,
+    // Steps are captured as hipGraphs and replayed (the reference captures its
+    // H and E updates after the sources switch off,
     // src/Kernels/Kernels.jl:99-145; here sources and monitors are inside the
-    // graph too, gated by the device timestep).  Re-captured when a launch
-    // argument changes (a new waveform table, the FAST source correction
-    // dropping out after the switch-off).
+    // graph too, gated by the device timestep).  The host decides which DFT
+    // monitors are due at each captured position from the timestep the
+    // block starts at, so a graph is valid for replays that start at the same
+    // phase modulo the DFT period:
+    //   * the main graph: graph_len steps (graph_steps rounded up to the
+    //     period), started on multiples of the period (phase 0 every replay);
+    //   * short graphs: graph_steps steps from any phase, for runs shorter
+    //     than graph_len or not aligned to the period; the last few phases
+    //     used are kept.
+    // Re-captured when a launch argument changes (a new waveform table, the
+    // FAST source correction dropping out after the switch-off).
+    struct StepGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        int len = 0;
+        int64_t phase = -1;
+        bool src = false;      // the captured steps include the FAST source correction
+        int64_t dfts = 0;      // DFT launches captured
+        uint64_t used = 0;     // last use (short-graph replacement)
+        void drop() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            x = nullptr;
+            g = nullptr;
+            phase = -1;
+        }
+    };
+    static constexpr int N_SHORT = 4;
     int graph_steps = 16;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    bool graph_src = false;   // the captured steps include the FAST source correction
+    StepGraph main_graph;
+    StepGraph short_graph[N_SHORT];
+    uint64_t graph_clock = 0;
     bool graph_failed = false;
+    // launch counts since the last kh_get_launch_counts
+    int64_t n_replays = 0, n_graph_steps = 0, n_eager = 0, n_dft = 0;
+    bool capturing = false;
+    int64_t cap_dfts = 0;
 
     void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+        main_graph.drop();
+        for (auto &s : short_graph) s.drop();
     }
 
-    bool capture_graph() {
-        drop_graph();
+    bool capture(StepGraph &sg, int len) {
+        sg.drop();
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         bool ok = true;
+        capturing = true;
+        cap_dfts = 0;
         try {
-            for (int i = 0; i < graph_len; ++i) launch_step(i);
-            launch_tick(graph_len);
+            for (int i = 0; i < len; ++i) launch_step(i);
+            launch_tick(len);
         } catch (const KhError &) {
             ok = false;
         }
+        capturing = false;
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(st, &gr);
         if (!ok || e != hipSuccess || !gr) {
@@ -1238,17 +1271,20 @@ struct Ctx final : CtxBase {
             (void)hipGetLastError();
             return false;
         }
-        graph = gr;
-        if (hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        sg.g = gr;
+        if (hipGraphInstantiate(&sg.x, sg.g, nullptr, nullptr, 0) != hipSuccess) {
             (void)hipGetLastError();
-            drop_graph();
+            sg.drop();
             return false;
         }
-        graph_src = active || last_active;
+        sg.len = len;
+        sg.phase = ts % dft_period;
+        sg.src = active || last_active;
+        sg.dfts = cap_dfts;
         return true;
     }
 
-    int graph_len = 16;  // steps per captured graph: graph_steps rounded up to the DFT period
+    int graph_len = 16;  // steps per main graph: graph_steps rounded up to the DFT period
 
     void plan_graph_len() {
         dft_period = 1;
@@ -1264,6 +1300,39 @@ struct Ctx final : CtxBase {
         graph_len = graph_steps <= 0 ? 0 : (int)(((graph_steps + dft_period - 1) / dft_period) * dft_period);
     }
 
+    // The graph the next `left` steps from the current host state replay,
+    // or nullptr for an eager step.  begin_step() has run for the first step.
+    StepGraph *pick_graph(int64_t left) {
+        if (graph_len <= 0 || timing || graph_failed) return nullptr;
+        const auto fits = [&](int len) { return ckpt_every <= 0 || ts % ckpt_every + len <= ckpt_every; };
+        if (left >= graph_len && ts % dft_period == 0 && fits(graph_len)) return &main_graph;
+        if (graph_len > graph_steps && left >= graph_steps && fits(graph_steps)) {
+            const int64_t ph = ts % dft_period;
+            StepGraph *hit = nullptr, *old = &short_graph[0];
+            for (auto &s : short_graph) {
+                if (s.phase == ph) hit = &s;
+                if (s.used < old->used) old = &s;
+            }
+            return hit ? hit : old;
+        }
+        return nullptr;
+    }
+
+    // Capture (if needed) the graph for the current host state; false: eager.
+    bool ready(StepGraph *sg, int64_t left) {
+        const int len = sg == &main_graph ? graph_len : graph_steps;
+        const bool src_now = active || last_active;
+        if (!sg->x || sg->src != src_now || sg->len != len || sg->phase != ts % dft_period) {
+            if (!capture(*sg, len)) {
+                graph_failed = true;
+                return false;
+            }
+        }
+        (void)left;
+        sg->used = ++graph_clock;
+        return true;
+    }
+
     void step(int64_t n) override {
         require(n >= 0, "n_steps must be >= 0");
         if (!planned) {
@@ -1275,31 +1344,62 @@ struct Ctx final : CtxBase {
         while (done < n) {
             if (ckpt_every > 0 && ts % ckpt_every == 0) save_checkpoint();
             begin_step();
-            const bool src_now = active || last_active;
-            // a graph starts on a multiple of the DFT period, so the due
-            // monitors of each captured position are the same on every replay;
-            // with checkpoints on, a graph block stays inside one interval
-            if (graph_len > 0 && !timing && !graph_failed && n - done >= graph_len && ts % dft_period == 0 &&
-                (ckpt_every <= 0 || ts % ckpt_every + graph_len <= ckpt_every)) {
-                if (!graph_exec || graph_src != src_now) {
-                    if (!capture_graph()) graph_failed = true;
-                }
-                if (graph_exec) {
-                    HIPCHK(hipGraphLaunch(graph_exec, st));
+            StepGraph *sg = pick_graph(n - done);
+            if (sg && ready(sg, n - done)) {
+                HIPCHK(hipGraphLaunch(sg->x, st));
+                ++n_replays;
+                n_graph_steps += sg->len;
+                n_dft += sg->dfts;
+                end_step();
+                for (int i = 1; i < sg->len; ++i) {
+                    begin_step();
                     end_step();
-                    for (int i = 1; i < graph_len; ++i) {
-                        begin_step();
-                        end_step();
-                    }
-                    done += graph_len;
-                    continue;
                 }
+                done += sg->len;
+                continue;
             }
             launch_step(0);
             launch_tick(1);
             end_step();
+            ++n_eager;
             ++done;
         }
+    }
+
+    // Capture the graphs that step(n) would replay from the current state
+    // without running anything, so that the first replays of a timed run do
+    // not pay for capture.
+    void prepare_graphs(int64_t n) override {
+        require(n >= 0, "n_steps must be >= 0");
+        if (!planned) {
+            make_plans();
+            plan_graph_len();
+        }
+        const int64_t ts0 = ts;
+        const bool a0 = active, l0 = last_active;
+        int64_t done = 0;
+        while (done < n) {
+            begin_step();
+            StepGraph *sg = pick_graph(n - done);
+            const int len = sg && ready(sg, n - done) ? sg->len : 1;
+            end_step();
+            for (int i = 1; i < len; ++i) {
+                begin_step();
+                end_step();
+            }
+            done += len;
+        }
+        ts = ts0;
+        active = a0;
+        last_active = l0;
+    }
+
+    void launch_counts(int64_t *out) override {
+        out[0] = n_replays;
+        out[1] = n_graph_steps;
+        out[2] = n_eager;
+        out[3] = n_dft;
+        n_replays = n_graph_steps = n_eager = n_dft = 0;
     }
 
     void set_graph(int steps) override {
@@ -2376,6 +2476,21 @@ int kh_set_graph(kh_ctx *ctx, int32_t steps_per_graph) {
     return guarded([&] {
         CTX_OK;
         ctx->impl->set_graph(steps_per_graph);
+    });
+}
+
+int kh_prepare_graphs(kh_ctx *ctx, int64_t n_steps) {
+    return guarded([&] {
+        CTX_OK;
+        ctx->impl->prepare_graphs(n_steps);
+    });
+}
+
+int kh_get_launch_counts(kh_ctx *ctx, int64_t counts[4]) {
+    return guarded([&] {
+        CTX_OK;
+        require(counts != nullptr, "null argument");
+        ctx->impl->launch_counts(counts);
     });
 }
 

@@ -75,6 +75,8 @@ SIGNATURES = {
     "kh_write_pole": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]),
     "kh_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "kh_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "kh_prepare_graphs": (C.c_int, [C.c_void_p, C.c_int64]),
+    "kh_get_launch_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kh_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "kh_sync": (C.c_int, [C.c_void_p]),
     "kh_get_timestep": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
@@ -104,7 +106,7 @@ class KhronosError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 6  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
+ABI_VERSION = 7  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
 
 
 def load(path: str = LIB_PATH):

@@ -165,6 +165,19 @@ class HipEngine:
         """Steps per captured hipGraph (0: no graphs)."""
         _lib.check(self.lib.kh_set_graph(self.ctx, int(steps)))
 
+    def prepare_graphs(self, n: int):
+        """Capture the graphs the next ``step(n)`` replays, without stepping."""
+        self._ensure_waveforms(self.ts + n)
+        _lib.check(self.lib.kh_prepare_graphs(self.ctx, int(n)))
+
+    def launch_counts(self) -> dict:
+        """What the steps since the last call ran: graph replays, steps inside
+        them, eager steps and DFT launches (then cleared)."""
+        out = np.zeros(4, dtype=np.int64)
+        _lib.check(self.lib.kh_get_launch_counts(self.ctx, _ptr(out)))
+        return {"graph_replays": int(out[0]), "graph_steps": int(out[1]), "eager_steps": int(out[2]),
+                "dft_launches": int(out[3])}
+
     def idle(self) -> bool:
         """All queued work finished (non-blocking)."""
         v = C.c_int32()

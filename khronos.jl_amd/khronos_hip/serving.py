@@ -243,10 +243,14 @@ SERVING_GRIDS = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (192, 192, 128), (
 
 
 def serving_request(rid: int, rng: np.random.Generator, grids=SERVING_GRIDS, steps=(1000, 3000),
-                    resolution: int = 20, precision: str = "f32", n_freq: int = 20) -> Request:
+                    resolution: int = 20, precision: str = "f32", n_freq: int = 20,
+                    setup_precision: Optional[str] = "f64") -> Request:
     """A request of the serving workload: a grid drawn from ``grids``,
     10-cell PML, 1-3 boxes/spheres with eps ~ U[1.5, 12], an Ez Gaussian
-    dipole and one flux plane at ``n_freq`` frequencies."""
+    dipole and one flux plane at ``n_freq`` frequencies.  FP32 requests take
+    the FP64 setup (``setup_precision``: grid snapping, weights, waveforms)
+    with FP64 DFT accumulators, so that their fluxes are those of the FP64
+    problem up to the FP32 stencil's rounding."""
     from . import geometry as geo_mod
     from .grid import Ez
     from .monitors import FluxMonitor
@@ -268,13 +272,14 @@ def serving_request(rid: int, rng: np.random.Generator, grids=SERVING_GRIDS, ste
     freqs = list(np.linspace(0.8, 1.2, n_freq))
     plane_x = 0.5 * (cell[0] / 2 - pml)
 
-    def make():
+    def make(precision=precision, engine=None):
         src = [UniformSource(time_profile=GaussianPulseSource(fcen=1.0, fwidth=0.2), component=Ez(),
                              center=[0.0, 0.0, 0.0], size=[0.0, 0.0, 0.0])]
         mon = [FluxMonitor(center=[plane_x, 0.0, 0.0], size=[0.0, cell[1] - 2 * pml, cell[2] - 2 * pml],
                            frequencies=freqs)]
         return Simulation(cell_size=cell, resolution=resolution, sources=src, geometry=objs, monitors=mon,
-                          boundaries=[[pml, pml]] * 3, precision=precision, mode="fast")
+                          boundaries=[[pml, pml]] * 3, precision=precision, mode="fast",
+                          setup_precision=setup_precision, engine=engine)
 
     return Request(rid, make, nsteps, cells=n[0] * n[1] * n[2])
 

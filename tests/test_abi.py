@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_footprint_without_device():
     lib = _lib.load()
-    assert lib.kh_version() == 6
+    assert lib.kh_version() == 7
     assert C.sizeof(_lib.KhSimDesc) == 336  # layout of kh_sim_desc in include/khronos_hip.h
     d = _lib.KhSimDesc()
     d.nx, d.ny, d.nz = 384, 192, 160

@@ -97,3 +97,49 @@ def test_graph_speeds_up_small_grid():
         res[steps_per_graph] = (time.perf_counter() - t0) / 512 * 1e3
     _note(f"[graph] 80^3 f32 ms/step eager {res[0]:.4f}  graph {res[32]:.4f}")
     assert res[32] > 0
+
+
+def test_launch_counts_and_prepared_graphs():
+    """kh_get_launch_counts reports what kh_step ran: every step either inside
+    a replay or eager, and one DFT launch per half step with a due monitor
+    (the monitors of a half step share a launch), counted from the monitors'
+    decimations; kh_prepare_graphs captures without stepping (timestep and
+    fields unchanged), and runs of every length and phase keep replay equal
+    to eager (short graphs at several phases of the DFT period)."""
+    eager = _scene("f32", "fast", False)
+    graph = _scene("f32", "fast", False)
+    eager.prepare()
+    graph.prepare()
+    eager.engine.set_graph(0)
+    graph.engine.set_graph(8)
+    decs = [(md.component.electric, md.decimation) for md in graph.spec.dfts]
+    assert any(d > 1 for _, d in decs)
+
+    def due_launches(t0, n):
+        k = 0
+        for t in range(t0, t0 + n):
+            for electric in (False, True):
+                k += any(e == electric and t % d == 0 for e, d in decs)
+        return k
+
+    graph.engine.launch_counts()
+    before = [graph.read_field(C()) for C in COMPS]
+    graph.engine.prepare_graphs(50)
+    assert graph.engine.timestep() == 0
+    for a, b in zip(before, [graph.read_field(C()) for C in COMPS]):
+        assert np.array_equal(a, b)
+    assert graph.engine.launch_counts() == {"graph_replays": 0, "graph_steps": 0, "eager_steps": 0,
+                                            "dft_launches": 0}
+    t = 0
+    for chunk in (50, 3, 8, 20, 9, 31, 17, 64, 11):
+        eager.step(chunk)
+        graph.step(chunk)
+        c = graph.engine.launch_counts()
+        _note(f"[graph] steps {t}..{t + chunk - 1}: {c}")
+        assert c["graph_steps"] + c["eager_steps"] == chunk
+        assert c["dft_launches"] == due_launches(t, chunk)
+        if chunk >= 8:
+            assert c["graph_replays"] >= 1
+        t += chunk
+        for a, b in zip(_state(eager), _state(graph)):
+            assert np.array_equal(a, b), t

@@ -2,9 +2,9 @@
 PML, anisotropically smoothed strip, 4000 steps) with the FP64 run's setup
 (grid snapping, weights, profiles, waveforms: ``setup_precision="f64"``),
 FP32 fields and FP64 DFT accumulators gives both flux planes' spectra within
-1e-4 (relative to the spectrum's peak) of the FP64 run.  The FP64 device run
-stands in for the FP64 oracle at this size: on the small cases of
-test_gpu_parity it equals the oracle within 1e-12.
+1e-4 (relative to the spectrum's peak) of the FP64 run.  Both runs are
+checked against the FP64 oracle's spectra in test_gpu_full_size_oracle.py;
+here the FP32 variants are compared with the FP64 device run.
 
 Reported beside it: the same with FP32 accumulators, and the reference-style
 FP32 run, whose setup in FP32 snaps the source plane and monitors to other
