@@ -833,7 +833,7 @@ struct Ctx final : CtxBase {
                 ia.tiles[1] = (P.hi[1] - P.lo[1] + 1 + kh::TY - 1) / kh::TY;
                 ia.tiles[2] = (P.hi[2] - P.lo[2] + 1 + ia.zb - 1) / ia.zb;
                 int nt = ia.tiles[0] * ia.tiles[1] * ia.tiles[2];
-                ia.per_xcd = (nt + kh::NXCD - 1) / kh::NXCD;
+                ia.per_xcd = kh::xcd_per(nt);
                 P.inner_blocks = ia.per_xcd * kh::NXCD;
                 bool pervox = (e ? epsi[0] : mui[0]) != nullptr;
                 for (int a = 1; a < 3; ++a)
@@ -871,7 +871,7 @@ struct Ctx final : CtxBase {
         p.nslab = P.nslab;
         for (int s = 0; s < P.nslab; ++s) p.slab[s] = P.slab[s];
         p.nblock = P.cblocks;
-        p.per_xcd = (P.cblocks + kh::NXCD - 1) / kh::NXCD;
+        p.per_xcd = kh::xcd_per(P.cblocks);
         if (e && fpd[0][0][0]) {
             for (int a = 0; a < 3; ++a) {
                 p.fpd[a] = fpd[0][part][a];
@@ -1071,7 +1071,7 @@ struct Ctx final : CtxBase {
             q.nslab = P.npslab;
             for (int s = 0; s < P.npslab; ++s) q.slab[s] = P.pslab[s];
             q.nblock = P.pblocks;
-            q.per_xcd = (P.pblocks + kh::NXCD - 1) / kh::NXCD;
+            q.per_xcd = kh::xcd_per(P.pblocks);
             const bool pervox = q.minv[0] != nullptr;
             const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
             const bool lut_on = pervox && use_lut[e];
