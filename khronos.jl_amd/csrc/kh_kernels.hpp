@@ -637,7 +637,24 @@ struct PmlArgs {
     // values (LUT kernels): the same numbers as minv, a quarter of the bytes
     const uint8_t *mid[3];
     const T *lut;
+    // FAST with a periodic x axis (perx != 0): the x-neighbour across the
+    // boundary is read from the opposite interior plane instead of the ghost
+    // plane, as the wrap would have filled it: wa F + wb Fo in float64 (F
+    // this part, Fo the other part of a complex field or null; the Bloch
+    // phase for this part)
+    int perx;
+    int nxg;
+    const T *Fo[3];
+    double wa, wb;
 };
+
+// the ghost value wrap_kernel would write from interior element i
+template <typename T>
+__device__ __forceinline__ T xwrap_val(const PmlArgs<T> &p, int a, size_t i) {
+    double v = p.wa * (double)p.F[a][i];
+    if (p.Fo[a]) v += p.wb * (double)p.Fo[a][i];
+    return (T)v;
+}
 
 template <typename T>
 __device__ __forceinline__ bool pml_lane(const PmlArgs<T> &p, int &x, int &y, int &z, bool &v0, bool &v1) {
@@ -784,10 +801,23 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const size_t idx = (size_t)x + (size_t)y * sy + (size_t)z * sz;
     const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
     const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
-    const auto ex = ld2(Ex, idx), ey = ld2(Ey, idx), ez = ld2(Ez, idx);
+    const auto ex = ld2(Ex, idx);
+    auto ey = ld2(Ey, idx), ez = ld2(Ez, idx);
     const auto ex1 = ld2(Ex, idx + sz), ey1 = ld2(Ey, idx + sz);
     const auto ez_y1 = ld2(Ez, idx + sy), ex_y1 = ld2(Ex, idx + sy);
-    const T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
+    T ez_x2 = Ez[idx + 2], ey_x2 = Ey[idx + 2];
+    if (p.perx) {
+        // E at x = N + 1 is E at x = 1 times the Bloch phase
+        if (x + 1 == p.nxg) {
+            const size_t i1 = idx + 2 - (size_t)p.nxg;
+            ez_x2 = xwrap_val(p, 2, i1);
+            ey_x2 = xwrap_val(p, 1, i1);
+        } else if (x == p.nxg) {
+            const size_t i1 = idx + 1 - (size_t)p.nxg;
+            ez.y = xwrap_val(p, 2, i1);
+            ey.y = xwrap_val(p, 1, i1);
+        }
+    }
     PmlComp<T> c[3];
     pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
@@ -814,7 +844,13 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const auto hx = ld2(Hx, idx), hy = ld2(Hy, idx), hz = ld2(Hz, idx);
     const auto hxm = ld2(Hx, idx - sz), hym = ld2(Hy, idx - sz);
     const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
-    const T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
+    T hz_xm = Hz[idx - 1], hy_xm = Hy[idx - 1];
+    if (p.perx && x == 1) {
+        // H at x = 0 is H at x = N times the Bloch phase
+        const size_t i0 = idx - 1 + (size_t)p.nxg;
+        hz_xm = xwrap_val(p, 2, i0);
+        hy_xm = xwrap_val(p, 1, i0);
+    }
     PmlComp<T> c[3];
     pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
@@ -1048,15 +1084,18 @@ struct WrapArgs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void wrap_kernel(WrapArgs<T> w) {
-    const int ax = w.ax, t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
-    const int64_t n1 = w.N[t1], n2 = w.N[t2];
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the face's two axes, lower one fastest (x whenever the face holds it,
+    // so that consecutive lanes touch consecutive elements of a row)
+    const int ax = w.ax;
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    const int n1 = w.N[t1], n2 = w.N[t2];
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);  // faces x 6 < 2^31 (host-checked)
     if (c >= n1 * n2 * 6) return;
-    const int i = (int)(c % n1);
-    int64_t r = c / n1;
-    const int j = (int)(r % n2);
+    const int i = c % n1;
+    int r = c / n1;
+    const int j = r % n2;
     r /= n2;
-    const int comp = (int)(r % 3), side = (int)(r / 3);
+    const int comp = r % 3, side = r / 3;
     int gs[3], gd[3];
     gs[ax] = side == 0 ? w.N[ax] : 1;
     gd[ax] = side == 0 ? 0 : w.N[ax] + 1;
@@ -1107,6 +1146,11 @@ struct DftArgs {
     int electric;           // E monitors sample at t + dt/2
     int n0, n1, n2;
     int64_t sy, sz;
+    // x ghost planes not filled (periodic x folded into the updates): a
+    // sample on x = 0 or x = N + 1 is formed from x = N or x = 1 as the wrap
+    // would have formed it
+    int perx, xunit;
+    double xfwd[2], xrev[2];
 };
 
 // 2 pi t of a DFT sample at step ts: E monitors sample at t + dt/2, H at t;
@@ -1160,11 +1204,25 @@ __global__ __launch_bounds__(256) void dft_kernel(DftArgs<T> p) {
         }
         mi = m.minv ? m.minv[fi] : m.minv_s;
     }
-    const double v = in ? (double)(m.rebuild ? src[fi] / mi : src[fi]) : 0.0;
+    double v = in ? (double)(m.rebuild ? src[fi] / mi : src[fi]) : 0.0;
+    double u = in && m.fim ? (double)(m.rebuild ? srci[fi] / mi : srci[fi]) : 0.0;
+    if (p.perx && !m.rebuild && (gx == 0 || gx == p.n0 + 1) && gy >= 1 && gy <= p.n1 && gz >= 1 && gz <= p.n2) {
+        const bool hi = gx == p.n0 + 1;
+        const size_t si = hi ? fi - (size_t)p.n0 : fi + (size_t)p.n0;
+        const T fr = src[si], fm = m.fim ? srci[si] : T(0);
+        if (m.fim && !p.xunit) {
+            const double *ph = hi ? p.xfwd : p.xrev;
+            const double re = (double)fr, im = (double)fm;
+            v = (double)(T)(re * ph[0] - im * ph[1]);
+            u = (double)(T)(re * ph[1] + im * ph[0]);
+        } else {
+            v = (double)fr;
+            u = (double)fm;
+        }
+    }
     ACC *a = (ACC *)m.acc + 2 * cell;
     if (m.fim) {
         // complex field: (w_r + i w_i)(v + i u)
-        const double u = in ? (double)(m.rebuild ? srci[fi] / mi : srci[fi]) : 0.0;
         for (int k = 0; k < m.nf; ++k, a += 2 * ncell) {
             const double wr = wsh[2 * k], wi = wsh[2 * k + 1];
             double re = (double)a[0] + (wr * v - wi * u);

@@ -164,6 +164,12 @@ struct Ctx final : CtxBase {
     Part fp[2];
     int nparts = 1;
     bool periodic[3] = {false, false, false};
+    // FAST, periodic x: the PML kernels read the x-neighbour across the
+    // boundary from the opposite interior plane (xfold), so the x-axis ghost
+    // fill is skipped and the x ghosts are refreshed only before a field is
+    // read (xghost_stale); off when anything else reads or writes those
+    // ghosts (a material box or a DFT monitor on them, a field write)
+    bool xfold = false, xghost_stale = false, fields_written = false;
     double ph_fwd[3][2] = {}, ph_rev[3][2] = {};
     T *epsi[3]{}, *mui[3]{};
     T *sigB[3]{}, *sigD[3]{};
@@ -842,7 +848,29 @@ struct Ctx final : CtxBase {
             }
         }
         plan_materials();
+        plan_xfold();
         planned = true;
+    }
+
+    void plan_xfold() {
+        const int N = g.n[0];
+        bool ok = mode == KH_MODE_FAST && periodic[0] && !plan[0].inner && !plan[1].inner && !fields_written;
+        if (ok && mat) ok = mlo[0] > 1 && mlo[0] + mn[0] - 1 < N;
+        // E/H monitors on an x ghost plane form its value from the opposite
+        // plane (dft_kernel); FAST D/B monitors rebuild from stage arrays
+        // that have no ghost copies, so they keep the ghost fill
+        for (auto &m : dfts) ok = ok && (m.comp < 6 || (m.s[0] > 0 && m.s[0] + m.d[0] - 1 < N + 1));
+        if (ok != xfold) drop_graph();
+        if (xfold && !ok) refresh_xghosts();
+        xfold = ok;
+    }
+
+    // fill the x ghost planes of E and H as the per-half-step wrap would
+    void refresh_xghosts() {
+        if (!xghost_stale) return;
+        launch_wrap_axis(0, 0);
+        launch_wrap_axis(1, 0);
+        xghost_stale = false;
     }
 
     kh::CascadeArgs<T> cascade_args(int e, int part) const {
@@ -960,6 +988,16 @@ struct Ctx final : CtxBase {
     void launch_wrap(int e) {
         for (int ax = 0; ax < 3; ++ax) {
             if (!periodic[ax]) continue;
+            if (ax == 0 && xfold) {
+                xghost_stale = true;
+                continue;
+            }
+            launch_wrap_axis(e, ax);
+        }
+    }
+
+    void launch_wrap_axis(int e, int ax) {
+        {
             kh::WrapArgs<T> w{};
             for (int pt = 0; pt < nparts; ++pt)
                 for (int a = 0; a < 3; ++a) w.F[pt][a] = e ? fp[pt].E[a] : fp[pt].H[a];
@@ -974,6 +1012,7 @@ struct Ctx final : CtxBase {
             }
             w.unit = ph_fwd[ax][0] == 1.0 && ph_fwd[ax][1] == 0.0 && ph_rev[ax][0] == 1.0 && ph_rev[ax][1] == 0.0;
             const int64_t n = (int64_t)g.n[(ax + 1) % 3] * g.n[(ax + 2) % 3] * 6;
+            require(n < (1LL << 31) - 256, "periodic face too large");
             hipLaunchKernelGGL((kh::wrap_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
             HIPCHK(hipGetLastError());
         }
@@ -1064,6 +1103,19 @@ struct Ctx final : CtxBase {
             const bool cond = q.C[0] || q.C[1] || q.C[2];
             for (int a = 0; a < 3; ++a) q.mid[a] = mid[(electric ? 0 : 3) + a];
             q.lut = lut[e];
+            q.perx = xfold ? 1 : 0;
+            q.nxg = g.n[0];
+            if (xfold) {
+                // the H update reads E's ghost N+1 (forward phase), the E update
+                // H's ghost 0 (reverse phase), as the wrap fills them
+                const double *ph = electric ? ph_rev[0] : ph_fwd[0];
+                const bool cplx = nparts == 2 && !(ph_fwd[0][0] == 1.0 && ph_fwd[0][1] == 0.0 &&
+                                                   ph_rev[0][0] == 1.0 && ph_rev[0][1] == 0.0);
+                q.wa = cplx ? ph[0] : 1.0;
+                q.wb = cplx ? (part == 0 ? -ph[1] : ph[1]) : 0.0;
+                for (int a = 0; a < 3; ++a)
+                    q.Fo[a] = cplx ? (electric ? fp[1 - part].H[a] : fp[1 - part].E[a]) : nullptr;
+            }
             q.minv_s = electric ? eps_s : mu_s;
             q.dt = dt;
             q.sy = g.sy();
@@ -1157,6 +1209,12 @@ struct Ctx final : CtxBase {
             q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
             q.sy = g.sy();
             q.sz = g.sz();
+            q.perx = xfold ? 1 : 0;
+            q.xunit = ph_fwd[0][0] == 1.0 && ph_fwd[0][1] == 0.0 && ph_rev[0][0] == 1.0 && ph_rev[0][1] == 0.0;
+            for (int k = 0; k < 2; ++k) {
+                q.xfwd[k] = ph_fwd[0][k];
+                q.xrev[k] = ph_rev[0][k];
+            }
             const size_t shm = (size_t)nfmax * 2 * sizeof(double);
             EvPair ev{};
             if (timing) {
@@ -1340,6 +1398,7 @@ struct Ctx final : CtxBase {
             plan_graph_len();
         }
         stepped = true;
+        if (xfold && n > 0) xghost_stale = true;
         int64_t done = 0;
         while (done < n) {
             if (ckpt_every > 0 && ts % ckpt_every == 0) save_checkpoint();
@@ -1428,6 +1487,7 @@ struct Ctx final : CtxBase {
     // fields interleave (re, im).
     void read_field(int comp, void *host, size_t bytes) override {
         require(comp >= 0 && comp < 6, "component out of range");
+        refresh_xghosts();
         int cnt[3];
         comp_count(g.n, comp, cnt);
         int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
@@ -1452,6 +1512,10 @@ struct Ctx final : CtxBase {
     void write_field(int comp, const void *host, size_t bytes) override {
         require(comp >= 0 && comp < 6, "component out of range");
         drop_checkpoints();
+        // written ghosts are read as written: the x ghost fill goes back to
+        // every half step (after bringing the other ghosts up to date)
+        fields_written = true;
+        if (planned) plan_xfold();
         int cnt[3];
         comp_count(g.n, comp, cnt);
         int r[3] = {cnt[0] + 2, cnt[1] + 2, cnt[2] + 2};
@@ -1839,6 +1903,7 @@ struct Ctx final : CtxBase {
 
     void restore_state(const void *src, const CkHost &h) {
         copy_state(nullptr, src, false);
+        if (xfold) xghost_stale = true;
         clk_stage = h.ts;
         HIPCHK(hipMemcpyAsync(clk, &clk_stage, sizeof(int64_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));

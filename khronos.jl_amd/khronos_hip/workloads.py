@@ -110,9 +110,11 @@ def algorithmic_bytes(spec) -> dict:
     read the 3 differentiated components, read+write D/B (6), write E/H (3),
     read E/H for the p own-axis components, read+write W and U for p
     components each (12 + 5p).  A per-voxel inverse permittivity (or
-    permeability) adds 3 reads.  s = bytes of the backend real.
+    permeability) adds 3 reads.  Field accesses are of the field precision,
+    twice that for complex (Bloch) fields; material reads are real.
     """
-    s = np.dtype(spec.real).itemsize
+    s_mat = np.dtype(spec.field_real or spec.real).itemsize
+    s = s_mat * (2 if spec.complex_fields else 1)
     N = spec.N
     out = {}
     for kind, key, minv in (("H", "B", spec.mu_inv), ("E", "D", spec.eps_inv)):
@@ -123,12 +125,11 @@ def algorithmic_bytes(spec) -> dict:
         p = (nz[0][:, None, None].astype(int) + nz[1][None, :, None] + nz[2][None, None, :])
         counts = [int(np.count_nonzero(p == k)) for k in range(4)]
         acc = 9 * counts[0] + sum((12 + 5 * k) * counts[k] for k in (1, 2, 3))
-        if any(m is not None for m in minv):
-            acc += 3 * sum(counts)
-        out[kind] = acc * s
+        mat = 3 * sum(counts) if any(m is not None for m in minv) else 0
+        out[kind] = acc * s + mat * s_mat
         out[kind + "_counts"] = counts
         # the inner-box kernel alone: p = 0 cells, 9 accesses (+3 per-voxel)
-        out[kind + "_inner"] = (9 + (3 if any(m is not None for m in minv) else 0)) * counts[0] * s
+        out[kind + "_inner"] = (9 * s + (3 * s_mat if mat else 0)) * counts[0]
     return out
 
 
