@@ -126,7 +126,8 @@ int kh_destroy(kh_ctx *ctx);
 
 /* Register a current source: S[start + i] += Re(f(t) * amplitude[i]) on the
  * component's half step while sources are active.  `amplitude` is complex,
- * backend precision, dims[0]*dims[1]*dims[2] values, column-major.
+ * backend precision, dims[0]*dims[1]*dims[2] values, column-major.  Any
+ * number of sources; those of a half step are added in registration order.
  * Replaces: assemble_sources upload + update_source! (src/Sources/Sources.jl:44-131,
  * :346-357). */
 int kh_add_source(kh_ctx *ctx, int32_t component, const int32_t start_idx[3],

@@ -131,7 +131,7 @@ struct CascadeArgs {
     int nblock;           // logical blocks over all slabs
     int per_xcd;
     int nsrc;
-    SrcL<T> src[MAX_SRC];
+    const SrcL<T> *src;   // device array of nsrc sources of this half step, registration order
     // material conductivity (per voxel, padded grid) and the C stage
     // arrays; null for a component without conductivity
     const T *msig[3];

@@ -438,9 +438,6 @@ struct Ctx final : CtxBase {
             require(s[a] >= 1 && s[a] + d[a] - 1 <= cnt[a] + 1,
                     "source grid volume outside the component grid");
         }
-        int same = 0;
-        for (auto &o : srcs) same += (o.comp / 3) == (comp / 3);
-        require(same < kh::MAX_SRC, "too many sources on one half step (max 8)", KH_E_UNSUPPORTED);
         Source o;
         o.comp = comp;
         for (int a = 0; a < 3; ++a) { o.s[a] = s[a]; o.d[a] = d[a]; }
@@ -449,6 +446,7 @@ struct Ctx final : CtxBase {
         HIPCHK(hipMemcpyAsync(o.amp, amp, 2 * n * sizeof(T), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         srcs.push_back(std::move(o));
+        ++src_version;
         *id = (int32_t)srcs.size() - 1;
     }
 
@@ -475,6 +473,7 @@ struct Ctx final : CtxBase {
             HIPCHK(hipStreamSynchronize(st));
         }
         o.nwave = n;
+        ++src_version;
         drop_graph();  // nwave is a launch argument
     }
 
@@ -953,6 +952,42 @@ struct Ctx final : CtxBase {
         launch_wrap(e);
     }
 
+    // EXACT: the sources of each half step as a device list read by the
+    // cascade kernels (any number of them), brought up to date before the
+    // step's launches are issued or captured
+    kh::SrcL<T> *srcl_dev[2] = {nullptr, nullptr};
+    int srcl_n[2] = {0, 0}, srcl_cap[2] = {0, 0};
+    uint64_t src_version = 1, srcl_version = 0;
+
+    void sync_source_lists() {
+        if (mode != KH_MODE_EXACT || srcl_version == src_version) return;
+        for (int e = 0; e < 2; ++e) {
+            std::vector<kh::SrcL<T>> l;
+            for (auto &o : srcs) {
+                if ((o.comp < 3) != (e == 1)) continue;
+                kh::SrcL<T> x{};
+                x.comp = o.comp % 3;
+                for (int a = 0; a < 3; ++a) { x.s[a] = o.s[a]; x.d[a] = o.d[a]; }
+                x.amp = o.amp;
+                x.wave = o.wave;
+                x.nwave = o.nwave;
+                l.push_back(x);
+            }
+            if ((int)l.size() > srcl_cap[e]) {
+                // a new list: captured graphs hold the old pointer
+                drop_graph();
+                srcl_cap[e] = std::max<int>((int)l.size(), 2 * srcl_cap[e]);
+                srcl_dev[e] = (kh::SrcL<T> *)dalloc((size_t)srcl_cap[e] * sizeof(kh::SrcL<T>));
+            }
+            if (!l.empty())
+                HIPCHK(hipMemcpyAsync(srcl_dev[e], l.data(), l.size() * sizeof(kh::SrcL<T>), hipMemcpyHostToDevice, st));
+            if ((int)l.size() != srcl_n[e]) drop_graph();  // the count is a launch argument
+            srcl_n[e] = (int)l.size();
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        srcl_version = src_version;
+    }
+
     std::vector<kh::FixSrc<T>> fix_sources(bool electric) const {
         std::vector<kh::FixSrc<T>> out;
         const Part &f = fp[0];
@@ -1057,18 +1092,9 @@ struct Ctx final : CtxBase {
                 HIPCHK(hipEventRecord(ev.a, st));
             }
             kh::CascadeArgs<T> p = cascade_args(e, part);
-            p.nsrc = 0;
             // sources add Re(f A) to the real part only
-            for (auto &o : srcs) {
-                if (part != 0) break;
-                if ((o.comp < 3) != electric) continue;
-                kh::SrcL<T> &l = p.src[p.nsrc++];
-                l.comp = o.comp % 3;
-                for (int a = 0; a < 3; ++a) { l.s[a] = o.s[a]; l.d[a] = o.d[a]; }
-                l.amp = o.amp;
-                l.wave = o.wave;
-                l.nwave = o.nwave;
-            }
+            p.nsrc = part == 0 ? srcl_n[e] : 0;
+            p.src = srcl_dev[e];
             p.clk = clk;
             p.koff = koff;
             p.cutoff = cutoff;
@@ -1398,6 +1424,7 @@ struct Ctx final : CtxBase {
             plan_graph_len();
         }
         stepped = true;
+        sync_source_lists();
         if (xfold && n > 0) xghost_stale = true;
         int64_t done = 0;
         while (done < n) {
@@ -1434,6 +1461,7 @@ struct Ctx final : CtxBase {
             make_plans();
             plan_graph_len();
         }
+        sync_source_lists();
         const int64_t ts0 = ts;
         const bool a0 = active, l0 = last_active;
         int64_t done = 0;

@@ -20,8 +20,9 @@
 * ``adjoint_bend`` -- the 2D 90-degree waveguide bend of the reference's
   inverse-design example (examples/waveguide_bend_inverse_design.jl:16-109)
   at resolution 40 with 2 um padding: 400 x 400 cells, 40-cell PML, 0.5 um
-  eps 11.56 guides in eps 2.0736, a 3 x 3 um design region of 120 x 120
-  parameters rho ~ U[0, 1] (``seed``), eps in [2.0736, 11.56], Ez line
+  eps 12.1 guides in eps 2.085 (the strip waveguide's silicon and
+  cladding), a 3 x 3 um design region of 120 x 120 parameters rho ~ U[0, 1]
+  (``seed``), eps in [2.085, 12.1], Ez line
   source (Gaussian pulse, fcen 0.625, fwidth 0.5), objective sum |Ez|^2 on
   the output guide's line at 3 frequencies; 2000 steps (until = 25),
   checkpoints every 100 steps for the time-reversed adjoint.
@@ -139,8 +140,7 @@ def adjoint_bend(precision="f32", mode="fast", seed=0, setup_precision=None, met
     import torch
 
     from . import adjoint as A
-    n_si, n_sio2 = 3.4, 1.44
-    eps_si, eps_sio2 = n_si ** 2, n_sio2 ** 2
+    eps_si, eps_sio2 = 12.1, 2.085
     wg, pml = 0.5, 1.0
     design = n_design / resolution
     fcen, fwidth = 0.625, 0.5

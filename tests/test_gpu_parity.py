@@ -238,3 +238,33 @@ def test_strip_waveguide_full_size_fast_vs_exact_and_linearity():
         d = np.linalg.norm(fc.astype(np.float64) - 2.0 * fa.astype(np.float64))
         assert d / norm < 1e-6, C.__name__
     del a, b, c
+
+
+@pytest.mark.parametrize("mode,tol", [("exact", 0.0), ("fast", 1e-12)])
+def test_twelve_point_sources_and_line_sources(mode, tol):
+    """Sixteen sources (src/Sources/Sources.jl:288-357 injects every source of
+    its list): twelve Ez dipoles with different pulses, two of them on the
+    same cell, more than one launch holds, and four Hx/Hy line sources, FP64,
+    against the oracle (EXACT value for value, FAST within rel-L2 1e-12)."""
+    def mk(engine=None, m="exact"):
+        rng = np.random.default_rng(12)
+        src = []
+        for k in range(12):
+            c = [0.0, 0.0, 0.0] if k in (3, 7) else [float(v) for v in rng.uniform(-0.6, 0.6, 3)]
+            src.append(kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.8 + 0.05 * k, fwidth=0.4),
+                                        component=kh.Ez(), center=c, size=[0.0, 0.0, 0.0],
+                                        amplitude=1.0 + 0.1 * k))
+        for k, C in enumerate((kh.Hx, kh.Hy, kh.Hx, kh.Hy)):
+            src.append(kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=1.0, fwidth=0.5), component=C(),
+                                        center=[-0.3 + 0.2 * k, 0.1, -0.2], size=[0.0, 0.4, 0.0]))
+        mon = [kh.DFTMonitor(component=kh.Ez(), center=[0.0, 0.0, 0.1], size=[1.4, 1.4, 0.0],
+                             frequencies=[0.9, 1.1])]
+        return kh.Simulation(cell_size=[2.0, 2.0, 2.0], resolution=10, sources=src, monitors=mon,
+                             boundaries=[[0.3, 0.3]] * 3, precision="f64", mode=m, engine=engine)
+    g, o = mk(m=mode), mk(OracleEngine)
+    _note(f"[parity] 16 sources, {mode}")
+    g.step(70)
+    o.step(70)
+    tot, per = field_errors(g, o)
+    assert tot <= tol, (tot, per)
+    assert rel_l2(g.engine.read_dft(0), o.engine.read_dft(0)) <= max(tol, 1e-13)
