@@ -309,9 +309,11 @@ int kh_near2far_spectrum(kh_ctx *ctx, int32_t e1, int32_t e2, int32_t h1, int32_
  *     steps; the caller keeps ownership and must not touch the context until
  *     the request is reported done.  kh_server_cancel returns an admission
  *     that will not be submitted.
- *   kh_server_poll: without blocking, gives every idle submitted context its
- *     next chunk_steps steps (kh_step on its own stream) and retires those
- *     with no steps left; done_ids receives up to cap finished request ids in
+ *   kh_server_poll: without blocking, keeps two chunks of chunk_steps steps
+ *     queued on every submitted context (kh_step on its own stream; the next
+ *     chunk waits while the current one runs, so a context does not drain
+ *     between polls) and retires those with no steps left and an idle
+ *     stream; done_ids receives up to cap finished request ids in
  *     completion order, *n_done their count, *n_inflight (may be NULL) the
  *     requests admitted and not yet reported.
  * admit/submit/cancel may be called from other threads than poll; each

@@ -21,6 +21,7 @@
 #include <vector>
 #include <mutex>
 #include <deque>
+#include <map>
 
 #include "../../include/khronos_hip.h"
 #include "kh_kernels.hpp"
@@ -99,6 +100,113 @@ void comp_count(const int n[3], int c, int out[3]) {
     }
 }
 
+// Device memory and streams released by destroyed contexts, kept for later
+// contexts of the process.  hipFree waits for the whole device, so freeing a
+// finished request's arrays would stall every other context of a serving
+// loop; with the pool, kh_destroy synchronises only its own stream, and a
+// context of a size seen before allocates nothing.  Blocks are matched by size
+// rounded up to 64 KiB; the pool holds at most KH_POOL_BYTES (default 32 GiB,
+// beyond that a released block is freed) and is emptied when an allocation
+// fails for want of memory.
+struct DevicePool {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void *> blocks;  // (device, rounded bytes) -> block
+    std::multimap<int, hipStream_t> streams;               // device -> idle stream
+    size_t held = 0;
+    size_t limit = 0;
+    DevicePool() {
+        const char *v = getenv("KH_POOL_BYTES");
+        limit = v && *v ? (size_t)strtoull(v, nullptr, 10) : ((size_t)32 << 30);
+    }
+};
+
+DevicePool &dpool() {
+    static DevicePool *p = new DevicePool();  // outlives every context
+    return *p;
+}
+
+size_t pool_round(size_t b) { return (b + 65535) / 65536 * 65536; }
+
+void pool_trim(int dev) {
+    std::vector<void *> out;
+    {
+        std::lock_guard<std::mutex> lock(dpool().mu);
+        auto &bl = dpool().blocks;
+        for (auto it = bl.begin(); it != bl.end();) {
+            if (it->first.first == dev) {
+                out.push_back(it->second);
+                dpool().held -= it->first.second;
+                it = bl.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (void *q : out) (void)hipFree(q);
+}
+
+void *pool_alloc(int dev, size_t bytes) {
+    const size_t rb = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> lock(dpool().mu);
+        auto it = dpool().blocks.find({dev, rb});
+        if (it != dpool().blocks.end()) {
+            void *q = it->second;
+            dpool().blocks.erase(it);
+            dpool().held -= rb;
+            return q;
+        }
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, rb);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        pool_trim(dev);
+        e = hipMalloc(&q, rb);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw KhError(e == hipErrorOutOfMemory ? KH_E_NOMEM : KH_E_DEVICE,
+                      std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return q;
+}
+
+// the block's last use has completed
+void pool_release(int dev, void *q, size_t bytes) {
+    const size_t rb = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> lock(dpool().mu);
+        if (dpool().held + rb <= dpool().limit) {
+            dpool().blocks.insert({{dev, rb}, q});
+            dpool().held += rb;
+            return;
+        }
+    }
+    (void)hipFree(q);
+}
+
+hipStream_t pool_stream(int dev) {
+    {
+        std::lock_guard<std::mutex> lock(dpool().mu);
+        auto it = dpool().streams.find(dev);
+        if (it != dpool().streams.end()) {
+            hipStream_t s = it->second;
+            dpool().streams.erase(it);
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+// the stream is idle
+void pool_stream_release(int dev, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(dpool().mu);
+    dpool().streams.insert({dev, s});
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -118,6 +226,7 @@ struct CtxBase {
     virtual void set_graph(int steps) = 0;
     virtual void prepare_graphs(int64_t n) = 0;
     virtual void launch_counts(int64_t *out) = 0;
+    virtual int graph_block() = 0;
     virtual void sync() = 0;
     virtual bool idle() = 0;
     virtual int64_t timestep() const = 0;
@@ -147,6 +256,7 @@ struct CtxBase {
 template <typename T>
 struct Ctx final : CtxBase {
     Grid g;
+    int dev = 0;  // the device the context lives on
     int mode;
     bool acc64;
     T dt, dx[3];
@@ -245,8 +355,7 @@ struct Ctx final : CtxBase {
     double tcnt[kh_timing_slots] = {};
 
     void *dalloc(size_t bytes) {
-        void *p = nullptr;
-        HIPCHK(hipMalloc(&p, bytes));
+        void *p = pool_alloc(dev, bytes);
         owned.push_back({p, bytes});
         HIPCHK(hipMemsetAsync(p, 0, bytes, st));
         return p;
@@ -276,7 +385,8 @@ struct Ctx final : CtxBase {
                                     ph_rev[a][1] == 0.0),
                     "a Bloch phase other than 1 needs complex_fields = 1");
         }
-        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipGetDevice(&dev));
+        st = pool_stream(dev);
         for (int p = 0; p < nparts; ++p) {
             Part &f = fp[p];
             for (int a = 0; a < 3; ++a) {
@@ -421,11 +531,12 @@ struct Ctx final : CtxBase {
     ~Ctx() override {
         if (st) (void)hipStreamSynchronize(st);
         drop_graph();
+        reap_graphs(true);
         for (auto &e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto &e : pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
-        for (auto &b : owned) (void)hipFree(b.p);
+        for (auto &b : owned) pool_release(dev, b.p, b.bytes);
         for (void *p : ckpt) (void)hipFree(p);
-        if (st) (void)hipStreamDestroy(st);
+        if (st) pool_stream_release(dev, st);
     }
 
     void add_source(int comp, const int32_t *s, const int32_t *d, const void *amp, int32_t *id) override {
@@ -1311,14 +1422,52 @@ struct Ctx final : CtxBase {
         bool src = false;      // the captured steps include the FAST source correction
         int64_t dfts = 0;      // DFT launches captured
         uint64_t used = 0;     // last use (short-graph replacement)
-        void drop() {
-            if (x) (void)hipGraphExecDestroy(x);
-            if (g) (void)hipGraphDestroy(g);
-            x = nullptr;
-            g = nullptr;
-            phase = -1;
-        }
     };
+    // A replaced graph may still have launches queued on the stream: it is
+    // destroyed once an event recorded behind them has completed.
+    struct DeadGraph {
+        hipGraph_t g;
+        hipGraphExec_t x;
+        hipEvent_t ev;
+    };
+    std::vector<DeadGraph> dead_graphs;
+
+    void retire_graph(StepGraph &sg) {
+        if (sg.x || sg.g) {
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+                hipEventRecord(ev, st) == hipSuccess) {
+                dead_graphs.push_back({sg.g, sg.x, ev});
+            } else {
+                // no event: wait for the stream instead
+                (void)hipGetLastError();
+                if (ev) (void)hipEventDestroy(ev);
+                (void)hipStreamSynchronize(st);
+                if (sg.x) (void)hipGraphExecDestroy(sg.x);
+                if (sg.g) (void)hipGraphDestroy(sg.g);
+            }
+        }
+        sg.x = nullptr;
+        sg.g = nullptr;
+        sg.phase = -1;
+        reap_graphs(false);
+    }
+
+    // destroy the retired graphs whose launches have completed (all: wait)
+    void reap_graphs(bool wait) {
+        for (auto it = dead_graphs.begin(); it != dead_graphs.end();) {
+            const hipError_t e = wait ? hipEventSynchronize(it->ev) : hipEventQuery(it->ev);
+            if (e == hipErrorNotReady) {
+                ++it;
+                continue;
+            }
+            (void)hipGetLastError();
+            if (it->x) (void)hipGraphExecDestroy(it->x);
+            if (it->g) (void)hipGraphDestroy(it->g);
+            (void)hipEventDestroy(it->ev);
+            it = dead_graphs.erase(it);
+        }
+    }
     static constexpr int N_SHORT = 4;
     int graph_steps = 16;
     StepGraph main_graph;
@@ -1331,12 +1480,12 @@ struct Ctx final : CtxBase {
     int64_t cap_dfts = 0;
 
     void drop_graph() {
-        main_graph.drop();
-        for (auto &s : short_graph) s.drop();
+        retire_graph(main_graph);
+        for (auto &s : short_graph) retire_graph(s);
     }
 
     bool capture(StepGraph &sg, int len) {
-        sg.drop();
+        retire_graph(sg);
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         bool ok = true;
         capturing = true;
@@ -1358,7 +1507,9 @@ struct Ctx final : CtxBase {
         sg.g = gr;
         if (hipGraphInstantiate(&sg.x, sg.g, nullptr, nullptr, 0) != hipSuccess) {
             (void)hipGetLastError();
-            sg.drop();
+            (void)hipGraphDestroy(sg.g);
+            sg.g = nullptr;
+            sg.x = nullptr;
             return false;
         }
         sg.len = len;
@@ -1479,6 +1630,15 @@ struct Ctx final : CtxBase {
         ts = ts0;
         active = a0;
         last_active = l0;
+    }
+
+    // steps of the main graph (the block a run of steps is best cut into)
+    int graph_block() override {
+        if (!planned) {
+            make_plans();
+            plan_graph_len();
+        }
+        return graph_len > 0 ? graph_len : 1;
     }
 
     void launch_counts(int64_t *out) override {
@@ -1659,10 +1819,10 @@ struct Ctx final : CtxBase {
             HIPCHK(hipStreamSynchronize(st));
             for (auto it = owned.begin(); it != owned.end(); ++it)
                 if (it->p == old) {
+                    pool_release(dev, it->p, it->bytes);
                     owned.erase(it);
                     break;
                 }
-            HIPCHK(hipFree(old));
         }
         cap = bytes;
         return dalloc(bytes);
@@ -2296,18 +2456,39 @@ struct kh_ctx {
 // admit/submit may come from preparation threads while one thread polls;
 // every context is stepped from the polling thread only.
 struct kh_server {
+    // chunks queued ahead on each context's stream: the next chunk waits on
+    // the stream while the current one runs, so a context does not drain
+    // between two polls
+    static constexpr int DEPTH = 2;
     struct Req {
         kh_ctx *ctx;
         int64_t remaining;
         double footprint;
         int64_t id;
+        std::deque<hipEvent_t> pending;  // one event after each queued chunk
     };
     double budget = 0.0, used = 0.0;
     int max_active = 1, chunk = 256;
     int reserved = 0;           // admitted, context not yet submitted
     std::deque<Req> active;     // submitted, stepping or waiting for a chunk
     std::deque<int64_t> done;   // finished, not yet reported
+    std::vector<hipEvent_t> events;  // idle events for reuse
     std::mutex mu;
+    hipEvent_t take_event() {
+        if (!events.empty()) {
+            hipEvent_t e = events.back();
+            events.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+    ~kh_server() {
+        for (auto &r : active)
+            for (hipEvent_t e : r.pending) (void)hipEventDestroy(e);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -2651,14 +2832,27 @@ int kh_server_poll(kh_server *s, int64_t *done_ids, int32_t cap, int32_t *n_done
         require(cap >= 0 && (cap == 0 || done_ids != nullptr), "bad done buffer");
         std::lock_guard<std::mutex> lock(s->mu);
         for (auto it = s->active.begin(); it != s->active.end();) {
-            if (!it->ctx->impl->idle()) {
-                ++it;
-                continue;
+            // completed chunks
+            while (!it->pending.empty()) {
+                const hipError_t e = hipEventQuery(it->pending.front());
+                if (e == hipErrorNotReady) break;
+                HIPCHK(e);
+                s->events.push_back(it->pending.front());
+                it->pending.pop_front();
             }
-            if (it->remaining > 0) {
-                const int64_t n = std::min<int64_t>(s->chunk, it->remaining);
-                it->ctx->impl->step(n);  // asynchronous on the context's stream
+            // keep DEPTH chunks queued (kh_step is asynchronous on the context's stream)
+            while (it->remaining > 0 && (int)it->pending.size() < kh_server::DEPTH) {
+                // whole main-graph blocks, so that every chunk starts on the
+                // same phase of the DFT period and replays the same graph
+                const int64_t blk = it->ctx->impl->graph_block();
+                const int64_t n = std::min<int64_t>((s->chunk + blk - 1) / blk * blk, it->remaining);
+                it->ctx->impl->step(n);
                 it->remaining -= n;
+                hipEvent_t ev = s->take_event();
+                HIPCHK(hipEventRecord(ev, (hipStream_t)it->ctx->impl->stream()));
+                it->pending.push_back(ev);
+            }
+            if (it->remaining > 0 || !it->pending.empty() || !it->ctx->impl->idle()) {
                 ++it;
                 continue;
             }
