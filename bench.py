@@ -106,15 +106,40 @@ def config1_leg(steps: int = 500):
 
 def serving_leg(n_requests: int, rate: float, seed: int = 0):
     """A short Poisson stream of the serving workload, continuous batching vs
-    one at a time, with the ideal-overlap bound: every request stepped at the
-    rate the largest grid of the stream reaches alone, no earlier than it
-    arrives."""
+    one at a time, with the ideal-overlap bound: every request's algorithmic
+    bytes (its time steps' compulsory traffic, which grows with the PML share
+    of a small grid) moved at the bandwidth the stream's largest grid reaches
+    alone, no earlier than the request arrives.  Also printed: the same bound
+    in cell updates, every request at the largest grid's Mcell-updates/s."""
     import torch
     from khronos_hip import serving
     reqs = serving.serving_trace(n_requests, seed=seed, rate=rate)
     free = torch.cuda.mem_get_info()[0]
     served = serving.Server(budget_bytes=0.85 * free, max_active=6, chunk=256).run(reqs)
     solo = serving.run_one_at_a_time(reqs)
+    bound, bound_cells, r_max, bw = overlap_bound(reqs)
+    same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
+    return {
+        "workload": f"{len(reqs)} requests (grids 64^3..256^3, 1000-3000 steps, FP32 on the FP64 setup, "
+                    f"20-frequency flux), Poisson arrivals at {rate} requests/s (seed {seed})",
+        "served": round(served.throughput, 1), "one_at_a_time": round(solo.throughput, 1),
+        "unit": "Mcell-updates/s", "speedup": round(served.throughput / solo.throughput, 3),
+        "ideal_overlap_bound": round(bound, 1), "served_over_bound": round(served.throughput / bound, 3),
+        "largest_grid_alone": round(r_max / 1e6, 1), "largest_grid_GBps": round(bw / 1e9, 1),
+        "bound_at_largest_grid_rate": round(bound_cells, 1),
+        "latency_s": {"served_p50": round(served.latency_pct(50), 3), "served_p99": round(served.latency_pct(99), 3),
+                      "solo_p50": round(solo.latency_pct(50), 3), "solo_p99": round(solo.latency_pct(99), 3)},
+        "fluxes_identical": bool(same),
+    }
+
+
+def overlap_bound(reqs):
+    """(bound, bound in cell updates, largest grid's cell rate, its bytes/s):
+    the ideal-overlap throughput of a request stream in Mcell-updates/s.
+    The largest grid of the stream is timed alone (320 steps from graphs);
+    every request's algorithmic bytes per step (algorithmic_bytes of its
+    setup) then move at that grid's bandwidth, no earlier than it arrives."""
+    from khronos_hip.workloads import algorithmic_bytes
     big = max(reqs, key=lambda r: r.cells)
     sim = big.make()
     sim.prepare()
@@ -124,24 +149,23 @@ def serving_leg(n_requests: int, rate: float, seed: int = 0):
     t0 = time.perf_counter()
     sim.step(320)
     sim.engine.sync()
-    r_max = big.cells * 320 / (time.perf_counter() - t0)
+    dt_big = time.perf_counter() - t0
+    r_max = big.cells * 320 / dt_big
     sim.engine.close()
+    step_bytes = {}
+    for r in reqs:
+        ab = algorithmic_bytes(r.make().build_spec())
+        step_bytes[r.rid] = ab["E"] + ab["H"]
+    bw = step_bytes[big.rid] * 320 / dt_big  # bytes/s the largest grid moves alone
     t_first = min(r.arrival for r in reqs)
     work = sum(r.cells * r.steps for r in reqs)
-    finish = max(work / r_max + t_first, max(r.arrival + r.cells * r.steps / r_max for r in reqs))
-    bound = work / (finish - t_first) / 1e6
-    same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
-    return {
-        "workload": f"{len(reqs)} requests (grids 64^3..256^3, 1000-3000 steps, FP32 on the FP64 setup, "
-                    f"20-frequency flux), Poisson arrivals at {rate} requests/s (seed {seed})",
-        "served": round(served.throughput, 1), "one_at_a_time": round(solo.throughput, 1),
-        "unit": "Mcell-updates/s", "speedup": round(served.throughput / solo.throughput, 3),
-        "ideal_overlap_bound": round(bound, 1), "served_over_bound": round(served.throughput / bound, 3),
-        "largest_grid_alone": round(r_max / 1e6, 1),
-        "latency_s": {"served_p50": round(served.latency_pct(50), 3), "served_p99": round(served.latency_pct(99), 3),
-                      "solo_p50": round(solo.latency_pct(50), 3), "solo_p99": round(solo.latency_pct(99), 3)},
-        "fluxes_identical": bool(same),
-    }
+
+    def bound_of(cost):
+        finish = max(t_first + sum(cost(r) for r in reqs), max(r.arrival + cost(r) for r in reqs))
+        return work / (finish - t_first) / 1e6
+    bound = bound_of(lambda r: step_bytes[r.rid] * r.steps / bw)
+    bound_cells = bound_of(lambda r: r.cells * r.steps / r_max)
+    return bound, bound_cells, r_max, bw
 
 
 def measured_traffic(kernel_prefix: str):
@@ -191,6 +215,7 @@ def serving_bench(args):
     solo = serving.run_one_at_a_time(reqs, progress=progress("one at a time"))
     sys.stderr.write(f"one at a time: makespan {solo.makespan:.2f} s\n")
     same = all(np.array_equal(a.flux, b.flux) for a, b in zip(served.results, solo.results))
+    bound, bound_cells, r_max, bw = overlap_bound(reqs)
     line = {
         "metric": "serving throughput (Mcell-updates/s), continuous batching vs one at a time",
         "value": round(served.throughput, 2),
@@ -210,6 +235,9 @@ def serving_bench(args):
                       "solo_p50": round(solo.latency_pct(50), 3), "solo_p99": round(solo.latency_pct(99), 3)},
         "makespan_s": {"served": round(served.makespan, 3), "solo": round(solo.makespan, 3)},
         "fluxes_identical": bool(same),
+        "ideal_overlap_bound": round(bound, 1), "served_over_bound": round(served.throughput / bound, 3),
+        "largest_grid_alone": round(r_max / 1e6, 1), "largest_grid_GBps": round(bw / 1e9, 1),
+        "bound_at_largest_grid_rate": round(bound_cells, 1),
     }
     print(json.dumps(line), flush=True)
     return 0
