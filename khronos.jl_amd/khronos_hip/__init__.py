@@ -8,7 +8,7 @@ from .grid import (Ex, Ey, Ez, Hx, Hy, Hz, Center, Field, Volume, GridVolume, gr
                    component_voxel_count, interpolation_weight)
 from .sources import (GaussianPulseSource, ContinuousWaveSource, CustomSource, UniformSource,
                       ArraySource, GaussianProfileSource, PlaneWaveSource,
-                      EquivalentSource, ModeSource)
+                      EquivalentSource, ModeSource, TFSFSource, GaussianBeamSource)
 from .monitors import (DFTMonitor, FluxMonitor, ModeMonitor, compute_mode_amplitudes,
                        get_mode_transmission)
 from .mode import ModeSpec
@@ -28,7 +28,8 @@ __all__ = [
     "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Center", "Field", "Volume", "GridVolume", "grid_volume",
     "component_voxel_count", "interpolation_weight",
     "GaussianPulseSource", "ContinuousWaveSource", "CustomSource", "UniformSource", "ArraySource",
-    "GaussianProfileSource", "PlaneWaveSource", "EquivalentSource", "ModeSource", "ModeMonitor", "ModeSpec",
+    "GaussianProfileSource", "PlaneWaveSource", "EquivalentSource", "ModeSource", "TFSFSource",
+    "GaussianBeamSource", "ModeMonitor", "ModeSpec",
     "compute_mode_amplitudes", "get_mode_transmission", "DFTMonitor", "FluxMonitor", "PML", "Periodic", "Bloch", "PECBoundary",
     "PMCBoundary", "Absorber", "Material", "Object", "Box", "Sphere", "SubpixelSmoothing", "NoSmoothing",
     "VolumeAveraging", "AnisotropicSmoothing",

@@ -111,6 +111,13 @@ typedef struct {
     real *Fi[NCOMP], *Ti[NCOMP], *Ci[NCOMP], *Ui[NCOMP], *Wi[NCOMP], *Si[NCOMP];
     real *fPDi[3];
     int periodic[3];     /* axis wraps (Periodic or Bloch on both sides) */
+    /* the default 3D + PML chunk plan (27 chunks; src/Chunking.jl:622-738,
+     * :907-923): a cell's chunk has PML along axis a when the cell is in
+     * [1, pml_end[a]] or [pml_start[a], N]; that decides which auxiliary
+     * arrays the chunk holds (:1090-1122) and zero-fills the profiles of its
+     * other axes (:1355-1378) */
+    int chunk27;
+    int pml_end[3], pml_start[3];
     double ph_fwd[3][2]; /* lower interior -> upper ghost: exp(+i k L) */
     double ph_rev[3][2]; /* upper interior -> lower ghost: exp(-i k L) */
 } orc_sim;
@@ -283,6 +290,24 @@ int orc_set_periodic(orc_sim *s, int cplx, const int *periodic, const double *fw
 
 /* Replace a per-voxel inverse material array (geometry layout), as the
  * design update writes eps_inv (src/Adjoint/DesignRegion.jl:248-288). */
+/* Switch to the default 27-chunk plan's arithmetic (quirk: components whose
+ * chunk has no PML on their next axis get no U array, and the curl stage
+ * then drops the previous axis's damping, src/Kernels/Helpers.jl:200-223).
+ * pml_end[a]: last cell of the lower PML interval (0: none); pml_start[a]:
+ * first cell of the upper one (N + 1: none).  3D with PML, no material
+ * conductivity (its C arrays follow other rules).  Returns 0 or 1. */
+int orc_set_chunk27(orc_sim *s, const int *pml_end, const int *pml_start) {
+    if (!s->has_bound || s->nz < 1) return 1;
+    for (int c = 0; c < NCOMP; c++)
+        if (s->msig[c]) return 1;
+    s->chunk27 = 1;
+    for (int a = 0; a < 3; a++) {
+        s->pml_end[a] = pml_end[a];
+        s->pml_start[a] = pml_start[a];
+    }
+    return 0;
+}
+
 int orc_set_minv(orc_sim *s, int comp, const real *m) {
     if (!s->minv[comp]) return 1;
     memcpy(s->minv[comp], m, vol(s->gdim[comp]) * sizeof(real));
@@ -485,7 +510,21 @@ static void half_step(orc_sim *s, int electric) {
                     int hn = sg[n1] != NULL, hp = sg[n2] != NULL;
                     real sn = hn ? sig_at(sg[n1], gi[n1]) : 0;
                     real sp = hp ? sig_at(sg[n2], gi[n2]) : 0;
-                    err |= generic_curl(K, s->C[c], s->U[c], s->T[c], f, hsD, sD, hn, sn, hp, sp);
+                    real *Uc = s->U[c];
+                    if (s->chunk27) {
+                        /* a chunk with PML on some axis has profiles on every
+                         * axis (zero-filled off its PML axes) and U only with
+                         * PML on the next axis; a PML-free chunk has none */
+                        int hpa[3];
+                        for (int b = 0; b < 3; b++)
+                            hpa[b] = gi[b] <= s->pml_end[b] || gi[b] >= s->pml_start[b];
+                        const int pml_chunk = hpa[0] || hpa[1] || hpa[2];
+                        hn = hp = pml_chunk;
+                        sn = hpa[n1] ? sn : 0;
+                        sp = hpa[n2] ? sp : 0;
+                        if (!hpa[n1]) Uc = NULL;
+                    }
+                    err |= generic_curl(K, s->C[c], Uc, s->T[c], f, hsD, sD, hn, sn, hp, sp);
                 }
             }
     if (err) g_error = 1;
@@ -506,6 +545,11 @@ static void half_step(orc_sim *s, int electric) {
                     if (S) S[f] = 0;
                     int hs = sg[a] != NULL;
                     real so = hs ? sig_at(sg[a], gi[a]) : 0;
+                    if (s->chunk27) {
+                        /* W only with PML on the own axis; elsewhere the profile is zero-filled */
+                        const int own = gi[a] <= s->pml_end[a] || gi[a] >= s->pml_start[a];
+                        if (!own) so = 0;
+                    }
                     if (s->W[c] && hs && so != (real)0) {
                         real Wo = s->W[c][f];
                         s->W[c][f] = m * net;

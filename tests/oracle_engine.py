@@ -55,6 +55,8 @@ def load_path(real, path):
     lib.orc_set_periodic.restype = C.c_int
     lib.orc_set_periodic.argtypes = [P, C.c_int, P, P, P]
     lib.orc_set_chi3.argtypes = [P, P]
+    lib.orc_set_chunk27.restype = C.c_int
+    lib.orc_set_chunk27.argtypes = [P, P, P]
     lib.orc_set_minv.restype = C.c_int
     lib.orc_set_minv.argtypes = [P, C.c_int, P]
     lib.orc_add_pole.restype = C.c_int
@@ -221,6 +223,14 @@ class OracleEngine:
             return self.read(which, code) + 1j * self.read(which + 6, code)
         return self.read(which, code)
 
+    def set_chunk27(self, pml_end, pml_start):
+        """Step with the reference's default 27-chunk plan arithmetic
+        (src/Chunking.jl:907-923, :1090-1122; src/Kernels/Helpers.jl:200-223);
+        the PML cell intervals come from chunk27_intervals."""
+        e = np.asarray(pml_end, dtype=np.int32)
+        s = np.asarray(pml_start, dtype=np.int32)
+        assert self.lib.orc_set_chunk27(self.sim, _p(e), _p(s)) == 0, "chunk plan needs 3D PML and no conductivity"
+
     def set_eps_inv(self, a, arr):
         h = np.asfortranarray(arr, dtype=self.R)
         assert self.lib.orc_set_minv(self.sim, a, _p(h)) == 0
@@ -264,3 +274,19 @@ class OracleEngine:
 
     def dft_norms(self):
         return [float(np.sqrt(np.sum(np.abs(self.read_dft(i)) ** 2))) for i in range(len(self.dft_ids))]
+
+
+def chunk27_intervals(sim):
+    """PML cell intervals of the reference's chunk plan (src/Chunking.jl:
+    633-646): per axis the last cell of the lower PML (0: none) and the first
+    cell of the upper one (N + 1: none), from the user's thicknesses and the
+    backend-precision spacing."""
+    import math
+    N = [sim.Nx, sim.Ny, max(sim.Nz, 1)]
+    end, start = [], []
+    for a in range(3):
+        lo, hi = sim.boundaries[a] if sim.boundaries and len(sim.boundaries) > a else (0.0, 0.0)
+        d = float(sim.delta_n[a])
+        end.append(math.ceil(lo / d) if lo > 0.0 else 0)
+        start.append(N[a] - math.ceil(hi / d) + 1 if hi > 0.0 else N[a] + 1)
+    return end, start

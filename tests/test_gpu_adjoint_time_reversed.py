@@ -303,3 +303,34 @@ def test_design_region_inside_the_pml(mode):
     _note(f"[adjoint-tr] design in the PML, {mode} (adjoint, fd, rel): {res}")
     assert np.any(g != 0)
     assert max(r[2] for r in res) < 1e-6, res
+
+
+def slab2d_pml_design_problem(precision="f64", mode="fast", until=50.0, freqs=(0.45, 0.55)):
+    """2D, Ex/Ey/Hz fields (Ey line source): a 6 x 4 design region from x = 0
+    to x = 1.6 that reaches 0.6 into the x PML (PML from x = 1), so the Ex
+    samples there sit where their own-axis sigma is not zero (the forward
+    run carries them on W; the transposed half step takes the own-axis
+    branch)."""
+    pml, cx, cy = 1.0, 4.0, 2.6
+    geo = [kh.Object(kh.Box([0, 0, 0], [cx + 1, cy + 1, 10.0]), kh.Material(ε=1.0))]
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ey(),
+                            center=[-0.6, 0.0, 0.0], size=[0.0, cy - 2 * pml, 0.0])]
+    sim = kh.Simulation(cell_size=[cx, cy, 0.0], resolution=10, sources=src, geometry=geo,
+                        boundaries=[[pml, pml], [pml, pml]], precision=precision, mode=mode)
+    dr = A.DesignRegion(volume=Volume([0.8, 0, 0], [1.6, cy - 2 * pml, 0.0]), design_parameters=np.full(24, 0.5),
+                        grid_size=(6, 4), ε_min=1.0, ε_max=4.0)
+    objs = [A.FourierFieldsObjective(volume=Volume([-0.8, 0, 0], [0, cy - 2 * pml, 0.0]), component=kh.Ey()),
+            A.FourierFieldsObjective(volume=Volume([-0.8, 0, 0], [0, cy - 2 * pml, 0.0]), component=kh.Ex())]
+    f = lambda E1, E2: torch.sum(torch.abs(E1) ** 2) + torch.sum(torch.abs(E2) ** 2)
+    return A.OptimizationProblem(sim, [f], objs, [dr], list(freqs), until=until)
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_design_region_inside_the_x_pml(mode):
+    opt = _time_reversed(slab2d_pml_design_problem(mode=mode), 41)
+    rho = 0.3 + 0.4 * np.random.default_rng(19).random(24)
+    _note(f"[adjoint-tr] 2D design region in the x PML, {mode}")
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] 2D PML design {mode} (adjoint, fd, rel): {res}")
+    assert np.any(g != 0)
+    assert max(r[2] for r in res) < 1e-6, res

@@ -268,3 +268,30 @@ def test_twelve_point_sources_and_line_sources(mode, tol):
     tot, per = field_errors(g, o)
     assert tot <= tol, (tot, per)
     assert rel_l2(g.engine.read_dft(0), o.engine.read_dft(0)) <= max(tol, 1e-13)
+
+
+@pytest.mark.parametrize("mode,tol", [("exact", 0.0), ("fast", 1e-12)])
+def test_tfsf_and_gaussian_beam_sources(mode, tol):
+    """A TFSF box (six equivalent-source faces, src/Sources/SpatialSources.jl:
+    297-425) and a Gaussian beam launched from a plane (:650-857) through the
+    HIP library, against the oracle, FP64 (EXACT value for value, FAST within
+    rel-L2 1e-12)."""
+    def mk(engine=None, m="exact"):
+        tp = kh.GaussianPulseSource(fcen=1.0, fwidth=0.4)
+        src = kh.TFSFSource(time_profile=tp, size=[1.0, 0.8, 1.2], center=[0.1, 0.0, 0.0], injection_axis=3,
+                            direction=-1, polarization_angle=0.4, n_grid=12)
+        src.append(kh.GaussianBeamSource(time_profile=kh.GaussianPulseSource(fcen=0.9, fwidth=0.3),
+                                         beam_center=[0.0, 0.1, -0.4], beam_waist=0.5, k_vector=[0.2, 0.0, 1.0],
+                                         polarization=[1.0, 0.3, 0.0], center=[0.0, 0.0, -0.7],
+                                         size=[1.6, 1.6, 0.0]))
+        mon = [kh.FluxMonitor(center=[0.0, 0.0, 0.8], size=[1.6, 1.6, 0.0], frequencies=[0.9, 1.1])]
+        return kh.Simulation(cell_size=[2.2, 2.2, 2.4], resolution=10, sources=src, monitors=mon,
+                             boundaries=[[0.3, 0.3]] * 3, precision="f64", mode=m, engine=engine)
+    g, o = mk(m=mode), mk(OracleEngine)
+    _note(f"[parity] TFSF + Gaussian beam, {mode}")
+    g.step(90)
+    o.step(90)
+    tot, per = field_errors(g, o)
+    assert tot <= tol, (tot, per)
+    for i in range(len(o.spec.dfts)):
+        assert rel_l2(g.engine.read_dft(i), o.engine.read_dft(i)) <= max(tol, 1e-13)
