@@ -419,7 +419,10 @@ int kh_set_checkpoints(kh_ctx *ctx, int32_t every);
  * boxes in order, each
  * dense with x fastest.  Real fields without dispersive, chi3 or conductive
  * media and without periodic axes.  The fields, timestep and monitors are
- * those of the forward run again on return.
+ * those of the forward run again on return.  Unless graphs are off
+ * (kh_set_graph 0) or kernel timing is on, each full interval of up to 1024
+ * steps (re-run, snapshots, walk back) replays one graph captured for the
+ * call; the result is the same bit for bit.
  * Replaces: the pullback of simulate (src/Adjoint/ChainRulesIntegration.jl:
  * 38-60) with calculate_gradient! (src/Adjoint/GradientKernel.jl:35-109),
  * as dJ/d(eps^-1) per voxel before the design-grid contraction. */

@@ -2338,6 +2338,9 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
 // over frequencies, with w_k = dt exp(i f_k theta) the weight the DFT kernel
 // applied and G = dJ/dRe(F) + i dJ/dIm(F) per accumulator entry (same
 // layout as the accumulator).  Cells 1..N only (ghost samples are constant).
+// ts = device timestep + koff (the reverse walk leaves the clock at the end
+// of the interval it walks, so one launch sequence serves every interval);
+// each monitor skips the steps its decimation does not sample.
 template <typename T>
 struct AdjSeed {
     const double *g;
@@ -2352,7 +2355,8 @@ template <typename T>
 struct AdjSeedArgs {
     int nmon;
     AdjSeed<T> mon[MAX_DFT];
-    int64_t ts;
+    const int64_t *clk;
+    int64_t koff;
     T dt;
     int electric;
     int n0, n1, n2;
@@ -2366,8 +2370,9 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(AdjSeedArgs<T> p) {
     const int b = blockIdx.x;
     while (q + 1 < p.nmon && b >= p.mon[q + 1].first) ++q;
     const AdjSeed<T> &m = p.mon[q];
-    if (m.dec > 1 && (p.ts % m.dec) != 0) return;
-    const double theta = dft_theta(p.ts, p.dt, p.electric);
+    const int64_t ts = *p.clk + p.koff;
+    if (m.dec > 1 && (ts % m.dec) != 0) return;
+    const double theta = dft_theta(ts, p.dt, p.electric);
     const double dts = (double)p.dt;
     for (int k = threadIdx.x; k < m.nf; k += blockDim.x) {
         double sn, cs;

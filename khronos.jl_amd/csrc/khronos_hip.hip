@@ -2121,11 +2121,14 @@ struct Ctx final : CtxBase {
     T *adj_seg = nullptr;
     size_t adj_final_cap = 0, adj_cot_cap = 0, adj_grad_cap = 0, adj_seg_cap = 0;
 
+    // The seeds of the step at device timestep + koff; every monitor of the
+    // group is launched and skips the steps its decimation does not sample,
+    // so the launches do not depend on the step.
     void launch_adj_seed(const std::vector<const Dft *> &mons, const std::vector<size_t> &off, bool electric,
-                         int64_t j) {
+                         int64_t koff) {
         std::vector<size_t> list;
         for (size_t i = 0; i < mons.size(); ++i)
-            if ((mons[i]->comp < 3) == electric && (mons[i]->dec <= 1 || j % mons[i]->dec == 0)) list.push_back(i);
+            if ((mons[i]->comp < 3) == electric) list.push_back(i);
         // Monitors of one launch update lambda without atomics, so two that
         // write a common cell of one component (overlapping objectives, or
         // one monitor listed twice) go to successive launches, in list order.
@@ -2155,7 +2158,8 @@ struct Ctx final : CtxBase {
                 blocks += (int)((m.ncell + 255) / 256);
                 nfmax = std::max(nfmax, m.nf);
             }
-            q.ts = j;
+            q.clk = clk;
+            q.koff = koff;
             q.dt = dt;
             q.electric = electric ? 1 : 0;
             q.n0 = g.n[0]; q.n1 = g.n[1]; q.n2 = g.n[2];
@@ -2310,27 +2314,92 @@ struct Ctx final : CtxBase {
         }
         bc.sy = g.sy();
         bc.sz = g.sz();
-        for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
-            const int64_t s0 = sgi * K, s1 = std::min(N, s0 + K);
-            restore_state(ckpt[(size_t)sgi], ck_host[(size_t)sgi]);
-            for (int64_t j = s0; j < s1; ++j) {
-                begin_step();
-                launch_step(0);
-                launch_tick(1);
-                end_step();
+        // One interval of len steps from the restored checkpoint: the re-run
+        // (snapshot i of the design boxes after step i), then the walk back
+        // from the last step; the seeds read the device clock, which the
+        // re-run leaves at the interval's end.  Eager: host switch and one
+        // tick per step.  Captured: the launches of a whole interval, replayed
+        // for every interval of that length (the device clock drives every
+        // time-dependent quantity, the snapshots go to the same slots).
+        auto interval = [&](int64_t len, bool captured) {
+            for (int64_t i = 0; i < len; ++i) {
+                if (!captured) begin_step();
+                launch_step(captured ? (int)i : 0);
+                if (!captured) {
+                    launch_tick(1);
+                    end_step();
+                }
                 if (gtot) {
                     for (int a = 0; a < 3; ++a)
-                        bc.out[a] = gn[a][0] > 0 ? adj_seg + (size_t)(j - s0) * gtot + goff[a] : nullptr;
+                        bc.out[a] = gn[a][0] > 0 ? adj_seg + (size_t)i * gtot + goff[a] : nullptr;
                     hipLaunchKernelGGL((kh::box_copy_kernel<T>), dim3(bmax, 3), dim3(256), 0, st, bc);
                     HIPCHK(hipGetLastError());
                 }
             }
-            for (int64_t j = s1 - 1; j >= s0; --j) {
-                launch_adj_seed(mons, off, true, j);
-                launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)(j - s0) * gtot);
-                launch_adj_seed(mons, off, false, j);
+            if (captured) launch_tick(len);
+            for (int64_t i = len - 1; i >= 0; --i) {
+                launch_adj_seed(mons, off, true, i - len);
+                launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
+                launch_adj_seed(mons, off, false, i - len);
                 launch_adj_half(0, glo, gn, goff, nullptr);
             }
+        };
+        // graphs of a full interval, with and without the FAST source
+        // correction (captured with it while the sources are or were on at
+        // the interval's start; the kernels gate themselves by the clock).
+        // Destroyed after the stream has drained, however the sweep ends.
+        struct SweepGraphs {
+            hipStream_t st;
+            hipGraph_t g[2]{};
+            hipGraphExec_t x[2]{};
+            ~SweepGraphs() {
+                (void)hipStreamSynchronize(st);
+                for (int i = 0; i < 2; ++i) {
+                    if (x[i]) (void)hipGraphExecDestroy(x[i]);
+                    if (g[i]) (void)hipGraphDestroy(g[i]);
+                }
+            }
+        } sw{st};
+        bool use_graph = graph_steps > 0 && !timing && !graph_failed && K <= 1024;
+        auto capture_interval = [&](int which) -> bool {
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            bool ok = true;
+            try {
+                interval(K, true);
+            } catch (const KhError &) {
+                ok = false;
+            }
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &gr);
+            if (!ok || e != hipSuccess || !gr) {
+                if (gr) (void)hipGraphDestroy(gr);
+                (void)hipGetLastError();
+                return false;
+            }
+            sw.g[which] = gr;
+            if (hipGraphInstantiate(&sw.x[which], gr, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                sw.x[which] = nullptr;
+                return false;
+            }
+            return true;
+        };
+        for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
+            const int64_t s0 = sgi * K, s1 = std::min(N, s0 + K);
+            restore_state(ckpt[(size_t)sgi], ck_host[(size_t)sgi]);
+            if (use_graph && s1 - s0 == K) {
+                const int which = (active || last_active) ? 1 : 0;
+                if (!sw.x[which] && !capture_interval(which)) use_graph = false;
+                if (use_graph) {
+                    HIPCHK(hipGraphLaunch(sw.x[which], st));
+                    for (int64_t j = s0; j < s1; ++j) {
+                        begin_step();
+                        end_step();
+                    }
+                    continue;
+                }
+            }
+            interval(s1 - s0, false);
         }
         if (gtot) HIPCHK(hipMemcpyAsync(grad, adj_grad, gtot * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
