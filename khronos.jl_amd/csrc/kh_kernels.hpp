@@ -603,7 +603,10 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
 //              A' = A + (1 + s_o) W' - (1 - s_o) W
 // Source terms are added afterwards by source_fix_kernel.  A lane owns the x
 // pair (x, x+1) like the inner kernels; slab x ranges are widened to whole
-// pairs and the extra element is written back unchanged.
+// pairs and the extra element is written back unchanged.  With complex
+// fields (TWO kernels) a lane updates the real part and then the imaginary
+// part of its pair: the material bytes, sigma values and indexing are read
+// and computed once for both.
 struct PSlab {
     int lo[3];      // first cell: lo[0] odd
     int n[3];       // extent: n[0] even
@@ -611,12 +614,24 @@ struct PSlab {
     int first;      // first logical block
 };
 
+// One part's arrays: A updated from the curl of F, its U and W stages; with
+// a periodic x axis folded into the kernel (perx), the x-neighbour across
+// the boundary is read from the opposite interior plane as the wrap would
+// have filled the ghost: wa F + wb Fo in float64 (Fo the other part of a
+// complex field or null; wa, wb the Bloch phase for this part).
 template <typename T>
-struct PmlArgs {
+struct PmlPart {
     T *A[3];
     const T *F[3];
     T *U[3];
     T *W[3];
+    const T *Fo[3];
+    double wa, wb;
+};
+
+template <typename T>
+struct PmlArgs {
+    PmlPart<T> f[2];  // the part updated; [1]: the imaginary part (TWO kernels)
     const T *sig[3];
     const T *minv[3];
     T minv_s;
@@ -637,22 +652,16 @@ struct PmlArgs {
     // values (LUT kernels): the same numbers as minv, a quarter of the bytes
     const uint8_t *mid[3];
     const T *lut;
-    // FAST with a periodic x axis (perx != 0): the x-neighbour across the
-    // boundary is read from the opposite interior plane instead of the ghost
-    // plane, as the wrap would have filled it: wa F + wb Fo in float64 (F
-    // this part, Fo the other part of a complex field or null; the Bloch
-    // phase for this part)
+    // periodic x folded into the kernel (see PmlPart); nxg = N along x
     int perx;
     int nxg;
-    const T *Fo[3];
-    double wa, wb;
 };
 
 // the ghost value wrap_kernel would write from interior element i
 template <typename T>
-__device__ __forceinline__ T xwrap_val(const PmlArgs<T> &p, int a, size_t i) {
-    double v = p.wa * (double)p.F[a][i];
-    if (p.Fo[a]) v += p.wb * (double)p.Fo[a][i];
+__device__ __forceinline__ T xwrap_val(const PmlPart<T> &f, int a, size_t i) {
+    double v = f.wa * (double)f.F[a][i];
+    if (f.Fo[a]) v += f.wb * (double)f.Fo[a][i];
     return (T)v;
 }
 
@@ -686,7 +695,7 @@ struct PmlComp {
 };
 
 template <typename T, bool PERVOX, bool LUT>
-__device__ __forceinline__ void pml_load_a(const PmlArgs<T> &p, int a, size_t idx, PmlComp<T> &c) {
+__device__ __forceinline__ void pml_load_m(const PmlArgs<T> &p, int a, size_t idx, PmlComp<T> &c) {
     if (LUT) {
         const unsigned short q = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p.mid[a] + idx));
         c.m.x = p.lut[q & 0xff];
@@ -697,22 +706,22 @@ __device__ __forceinline__ void pml_load_a(const PmlArgs<T> &p, int a, size_t id
         c.m.x = p.minv_s;
         c.m.y = p.minv_s;
     }
-    c.ao = ldo<T, true>((const T *)p.A[a], idx);
 }
 
 template <typename T>
-__device__ __forceinline__ void pml_load_uw(const PmlArgs<T> &p, int a, size_t idx, bool nu, bool nw,
+__device__ __forceinline__ void pml_load_uw(const PmlPart<T> &f, int a, size_t idx, bool nu, bool nw,
                                             PmlComp<T> &c) {
     c.nu = nu;
     c.nw = nw;
-    if (nu) c.uo = ldo<T, true>((const T *)p.U[a], idx);
-    if (nw) c.wo = ldo<T, true>((const T *)p.W[a], idx);
+    if (nu) c.uo = ldo<T, true>((const T *)f.U[a], idx);
+    if (nw) c.wo = ldo<T, true>((const T *)f.W[a], idx);
 }
 
 // Update and store one component.  so/sn/sp: own/next/prev sigma per element.
 template <typename T>
-__device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t idx, const PmlComp<T> &c, T K0,
-                                           T K1, T so0, T so1, T sn0, T sn1, T sp0, T sp1, bool v0, bool v1) {
+__device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T> &f, int a, size_t idx,
+                                           const PmlComp<T> &c, T K0, T K1, T so0, T so1, T sn0, T sn1, T sp0,
+                                           T sp1, bool v0, bool v1) {
     using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
     if (c.nc) {
@@ -738,7 +747,7 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t id
             un.y = ((T(1) - sn1) * c.uo.y + in1) / (T(1) + sn1);
             in1 = un.y - c.uo.y;
         }
-        sto<T, true>(p.U[a], idx, un);
+        sto<T, true>(f.U[a], idx, un);
     }
     V an = c.ao;
     if (c.nw) {
@@ -759,26 +768,27 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, int a, size_t id
                 an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
             }
         }
-        sto<T, true>(p.W[a], idx, wn);
+        sto<T, true>(f.W[a], idx, wn);
     } else {
         if (v0) an.x = ((T(1) - sp0) * c.ao.x + c.m.x * in0) / (T(1) + sp0);
         if (v1) an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
     }
-    sto<T, true>(p.A[a], idx, an);
+    sto<T, true>(f.A[a], idx, an);
 }
 
-// Loads of the updated field's state for the three components:
+// Loads of one part's state for the three components (the material values
+// m are loaded once per lane by pml_load_m):
 // (own, next, prev) = (x,y,z), (y,z,x), (z,x,y).
-template <typename T, bool PERVOX, bool COND, bool LUT>
-__device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T sx0, T sx1, T sgy, T sgz,
-                                             int x, int y, int z, PmlComp<T> c[3]) {
-    pml_load_a<T, PERVOX, LUT>(p, 0, idx, c[0]);
-    pml_load_a<T, PERVOX, LUT>(p, 1, idx, c[1]);
-    pml_load_a<T, PERVOX, LUT>(p, 2, idx, c[2]);
+template <typename T, bool COND>
+__device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, T sx0, T sx1,
+                                             T sgy, T sgz, int x, int y, int z, PmlComp<T> c[3]) {
+    c[0].ao = ldo<T, true>((const T *)f.A[0], idx);
+    c[1].ao = ldo<T, true>((const T *)f.A[1], idx);
+    c[2].ao = ldo<T, true>((const T *)f.A[2], idx);
     const bool px = sx0 != T(0) || sx1 != T(0), py = sgy != T(0), pz = sgz != T(0);
-    pml_load_uw(p, 0, idx, py, px, c[0]);
-    pml_load_uw(p, 1, idx, pz, py, c[1]);
-    pml_load_uw(p, 2, idx, px, pz, c[2]);
+    pml_load_uw(f, 0, idx, py, px, c[0]);
+    pml_load_uw(f, 1, idx, pz, py, c[1]);
+    pml_load_uw(f, 2, idx, px, pz, c[2]);
     const bool inbox = COND && x + 1 >= p.clo[0] && x <= p.chi[0] && y >= p.clo[1] && y <= p.chi[1] &&
                        z >= p.clo[2] && z <= p.chi[2];
     for (int a = 0; a < 3; ++a) {
@@ -792,15 +802,12 @@ __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, size_t idx, T 
     }
 }
 
-template <typename T, bool PERVOX, bool COND, bool LUT = false>
-__global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
-    int x, y, z;
-    bool v0, v1;
-    if (!pml_lane(p, x, y, z, v0, v1)) return;
+// One part of the H update at the lane's pair (x, x+1), row y, plane z.
+template <typename T, bool COND>
+__device__ __forceinline__ void pml_h_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
+                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3]) {
     const int64_t sy = p.sy, sz = p.sz;
-    const size_t idx = (size_t)x + (size_t)y * sy + (size_t)z * sz;
-    const T *Ex = p.F[0], *Ey = p.F[1], *Ez = p.F[2];
-    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    const T *Ex = f.F[0], *Ey = f.F[1], *Ez = f.F[2];
     const auto ex = ld2(Ex, idx);
     auto ey = ld2(Ey, idx), ez = ld2(Ez, idx);
     const auto ex1 = ld2(Ex, idx + sz), ey1 = ld2(Ey, idx + sz);
@@ -810,16 +817,15 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
         // E at x = N + 1 is E at x = 1 times the Bloch phase
         if (x + 1 == p.nxg) {
             const size_t i1 = idx + 2 - (size_t)p.nxg;
-            ez_x2 = xwrap_val(p, 2, i1);
-            ey_x2 = xwrap_val(p, 1, i1);
+            ez_x2 = xwrap_val(f, 2, i1);
+            ey_x2 = xwrap_val(f, 1, i1);
         } else if (x == p.nxg) {
             const size_t i1 = idx + 1 - (size_t)p.nxg;
-            ez.y = xwrap_val(p, 2, i1);
-            ey.y = xwrap_val(p, 1, i1);
+            ez.y = xwrap_val(f, 2, i1);
+            ey.y = xwrap_val(f, 1, i1);
         }
     }
-    PmlComp<T> c[3];
-    pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
+    pml_load_all<T, COND>(p, f, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (ey1.x - ey.x) - iy * (ez_y1.x - ez.x));
     const T Kx1 = dt * (iz * (ey1.y - ey.y) - iy * (ez_y1.y - ez.y));
@@ -827,20 +833,17 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     const T Ky1 = dt * (ix * (ez_x2 - ez.y) - iz * (ex1.y - ex.y));
     const T Kz0 = dt * (iy * (ex_y1.x - ex.x) - ix * (ey.y - ey.x));
     const T Kz1 = dt * (iy * (ex_y1.y - ex.y) - ix * (ey_x2 - ey.y));
-    pml_update(p, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_update(p, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
 }
 
-template <typename T, bool PERVOX, bool COND, bool LUT = false>
-__global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
-    int x, y, z;
-    bool v0, v1;
-    if (!pml_lane(p, x, y, z, v0, v1)) return;
+// One part of the E update at the lane's pair.
+template <typename T, bool COND>
+__device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
+                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3]) {
     const int64_t sy = p.sy, sz = p.sz;
-    const size_t idx = (size_t)x + (size_t)y * sy + (size_t)z * sz;
-    const T *Hx = p.F[0], *Hy = p.F[1], *Hz = p.F[2];
-    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    const T *Hx = f.F[0], *Hy = f.F[1], *Hz = f.F[2];
     const auto hx = ld2(Hx, idx), hy = ld2(Hy, idx), hz = ld2(Hz, idx);
     const auto hxm = ld2(Hx, idx - sz), hym = ld2(Hy, idx - sz);
     const auto hz_ym = ld2(Hz, idx - sy), hx_ym = ld2(Hx, idx - sy);
@@ -848,11 +851,10 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     if (p.perx && x == 1) {
         // H at x = 0 is H at x = N times the Bloch phase
         const size_t i0 = idx - 1 + (size_t)p.nxg;
-        hz_xm = xwrap_val(p, 2, i0);
-        hy_xm = xwrap_val(p, 1, i0);
+        hz_xm = xwrap_val(f, 2, i0);
+        hy_xm = xwrap_val(f, 1, i0);
     }
-    PmlComp<T> c[3];
-    pml_load_all<T, PERVOX, COND, LUT>(p, idx, sx0, sx1, sgy, sgz, x, y, z, c);
+    pml_load_all<T, COND>(p, f, idx, sx0, sx1, sgy, sgz, x, y, z, c);
     const T dt = p.dt, ix = p.inv[0], iy = p.inv[1], iz = p.inv[2];
     const T Kx0 = dt * (iz * (hym.x - hy.x) - iy * (hz_ym.x - hz.x));
     const T Kx1 = dt * (iz * (hym.y - hy.y) - iy * (hz_ym.y - hz.y));
@@ -860,9 +862,43 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     const T Ky1 = dt * (ix * (hz.x - hz.y) - iz * (hxm.y - hx.y));
     const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
     const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
-    pml_update(p, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_update(p, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_update(p, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
+    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
+    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+}
+
+// TWO: complex fields, p.f[0] the real and p.f[1] the imaginary part (no
+// conductivity: its C stage is per part).
+template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false>
+__global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
+    static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
+    int x, y, z;
+    bool v0, v1;
+    if (!pml_lane(p, x, y, z, v0, v1)) return;
+    const size_t idx = (size_t)x + (size_t)y * p.sy + (size_t)z * p.sz;
+    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    PmlComp<T> c[3];
+    pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
+    pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
+    pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
+    pml_h_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    if (TWO) pml_h_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+}
+
+template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false>
+__global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
+    static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
+    int x, y, z;
+    bool v0, v1;
+    if (!pml_lane(p, x, y, z, v0, v1)) return;
+    const size_t idx = (size_t)x + (size_t)y * p.sy + (size_t)z * p.sz;
+    const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    PmlComp<T> c[3];
+    pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
+    pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
+    pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
+    pml_e_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    if (TWO) pml_e_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
 }
 
 // FAST mode sources.  With T eliminated the reference's E = m (T + S_n)

@@ -1021,7 +1021,10 @@ struct Ctx final : CtxBase {
     }
 
     void launch_half(int e) {
-        for (int part = 0; part < nparts; ++part) launch_half_part(e, part);
+        // complex fields without conductivity: one PML launch updates both parts
+        bool fuse = nparts == 2 && plan[e].pblocks > 0;
+        for (int a = 0; a < 3; ++a) fuse = fuse && !(e ? fp[0].CD[a] : fp[0].CB[a]);
+        for (int part = 0; part < nparts; ++part) launch_half_part(e, part, fuse);
         const bool electric = e == 1;
         // FAST mode sources (real part only: S += Re(f A),
         // src/Sources/Sources.jl:346-357) while sources are or were on the
@@ -1164,7 +1167,7 @@ struct Ctx final : CtxBase {
         }
     }
 
-    void launch_half_part(int e, int part) {
+    void launch_half_part(int e, int part, bool fuse) {
         const Plan &P = plan[e];
         const bool electric = e == 1;
         const Part &f = fp[part];
@@ -1218,17 +1221,22 @@ struct Ctx final : CtxBase {
                 evs.push_back(ev);
             }
         }
-        if (P.pblocks) {
+        if (P.pblocks && !(fuse && part == 1)) {
             if (timing) {
                 ev = take_event(2 * e + 1);
                 HIPCHK(hipEventRecord(ev.a, st));
             }
             kh::PmlArgs<T> q{};
+            for (int k = 0; k < (fuse ? 2 : 1); ++k) {
+                const Part &fk = fp[fuse ? k : part];
+                for (int a = 0; a < 3; ++a) {
+                    q.f[k].A[a] = electric ? fk.E[a] : fk.H[a];
+                    q.f[k].F[a] = electric ? fk.H[a] : fk.E[a];
+                    q.f[k].U[a] = electric ? fk.UD[a] : fk.UB[a];
+                    q.f[k].W[a] = electric ? fk.WD[a] : fk.WB[a];
+                }
+            }
             for (int a = 0; a < 3; ++a) {
-                q.A[a] = electric ? f.E[a] : f.H[a];
-                q.F[a] = electric ? f.H[a] : f.E[a];
-                q.U[a] = electric ? f.UD[a] : f.UB[a];
-                q.W[a] = electric ? f.WD[a] : f.WB[a];
                 q.sig[a] = electric ? sigD[a] : sigB[a];
                 q.minv[a] = electric ? epsi[a] : mui[a];
                 q.inv[a] = T(1) / dx[a];
@@ -1248,10 +1256,13 @@ struct Ctx final : CtxBase {
                 const double *ph = electric ? ph_rev[0] : ph_fwd[0];
                 const bool cplx = nparts == 2 && !(ph_fwd[0][0] == 1.0 && ph_fwd[0][1] == 0.0 &&
                                                    ph_rev[0][0] == 1.0 && ph_rev[0][1] == 0.0);
-                q.wa = cplx ? ph[0] : 1.0;
-                q.wb = cplx ? (part == 0 ? -ph[1] : ph[1]) : 0.0;
-                for (int a = 0; a < 3; ++a)
-                    q.Fo[a] = cplx ? (electric ? fp[1 - part].H[a] : fp[1 - part].E[a]) : nullptr;
+                for (int k = 0; k < (fuse ? 2 : 1); ++k) {
+                    const int pk = fuse ? k : part;
+                    q.f[k].wa = cplx ? ph[0] : 1.0;
+                    q.f[k].wb = cplx ? (pk == 0 ? -ph[1] : ph[1]) : 0.0;
+                    for (int a = 0; a < 3; ++a)
+                        q.f[k].Fo[a] = cplx ? (electric ? fp[1 - pk].H[a] : fp[1 - pk].E[a]) : nullptr;
+                }
             }
             q.minv_s = electric ? eps_s : mu_s;
             q.dt = dt;
@@ -1264,7 +1275,17 @@ struct Ctx final : CtxBase {
             const bool pervox = q.minv[0] != nullptr;
             const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
             const bool lut_on = pervox && use_lut[e];
-            if (electric) {
+            if (fuse) {
+                if (electric) {
+                    if (lut_on) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, true, true>), pgrid, pblk, 0, st, q);
+                    else if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, false, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, false, false, true>), pgrid, pblk, 0, st, q);
+                } else {
+                    if (lut_on) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, true, true>), pgrid, pblk, 0, st, q);
+                    else if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, false, true>), pgrid, pblk, 0, st, q);
+                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, false, false, true>), pgrid, pblk, 0, st, q);
+                }
+            } else if (electric) {
                 if (lut_on) {
                     if (cond) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true, true>), pgrid, pblk, 0, st, q);
                     else hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, true>), pgrid, pblk, 0, st, q);
