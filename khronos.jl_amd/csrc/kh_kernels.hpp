@@ -601,7 +601,8 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
 //   s_o == 0:  A' = ((1 - s_p) A + m in) / (1 + s_p)
 //   s_o != 0:  W' = ((1 - s_p) W + m in) / (1 + s_p),
 //              A' = A + (1 + s_o) W' - (1 - s_o) W
-// Source terms are added afterwards by source_fix_kernel.  A lane owns the x
+// Source terms are added as the values are stored (PmlArgs::src, up to
+// MAX_SRC per half step) or afterwards by source_fix_kernel.  A lane owns the x
 // pair (x, x+1) like the inner kernels; slab x ranges are widened to whole
 // pairs and the extra element is written back unchanged.  With complex
 // fields (TWO kernels) a lane updates the real part and then the imaginary
@@ -655,6 +656,15 @@ struct PmlArgs {
     // periodic x folded into the kernel (see PmlPart); nxg = N along x
     int perx;
     int nxg;
+    // FAST-mode sources of this half step applied to the part f[0] as the
+    // update stores its values: source_fix_kernel's correction, in
+    // registration order, on the cells inside each source's box (nsrc = 0:
+    // none, or the host launches source_fix_kernel instead)
+    int nsrc;
+    FixSrc<T> src[MAX_SRC];
+    const int64_t *clk;
+    int koff;
+    double cutoff;
 };
 
 // the ghost value wrap_kernel would write from interior element i
@@ -715,6 +725,47 @@ __device__ __forceinline__ void pml_load_uw(const PmlPart<T> &f, int a, size_t i
     c.nw = nw;
     if (nu) c.uo = ldo<T, true>((const T *)f.U[a], idx);
     if (nw) c.wo = ldo<T, true>((const T *)f.W[a], idx);
+}
+
+// The FAST source correction at the lane's pair (x, x+1) in row y, plane z,
+// once the update has stored its values: source by source in registration
+// order, the same loads and operations as source_fix_kernel (the lane reads
+// back what it stored), so the result is that of the separate launch.
+template <typename T>
+__device__ __forceinline__ void pml_sources(const PmlArgs<T> &p, size_t idx, int x, int y, int z, bool v0,
+                                            bool v1) {
+    for (int q = 0; q < p.nsrc; ++q) {
+        const FixSrc<T> &o = p.src[q];
+        if (y < o.s[1] || y >= o.s[1] + o.d[1] || z < o.s[2] || z >= o.s[2] + o.d[2]) continue;
+        for (int e = 0; e < 2; ++e) {
+            const int g[3] = {x + e, y, z};
+            if (!(e ? v1 : v0) || g[0] < o.s[0] || g[0] >= o.s[0] + o.d[0]) continue;
+            const int64_t ts = *p.clk + p.koff;
+            const bool now = sources_on(ts, p.dt, p.cutoff);
+            const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
+            if (!now && !before) continue;
+            const size_t i = idx + (size_t)e;
+            const size_t ci = (size_t)(g[0] - o.s[0]) +
+                              (size_t)o.d[0] * ((size_t)(y - o.s[1]) + (size_t)o.d[1] * (size_t)(z - o.s[2]));
+            const T ar = o.amp[2 * ci], ai = o.amp[2 * ci + 1];
+            const T m = o.minv ? o.minv[i] : p.minv_s;
+            const T so = o.sig_own[g[o.own]], sp = o.sig_prev[g[o.prev]];
+            const T ao = o.A[i];
+            const T wo = so == T(0) ? T(0) : o.W[i];
+            T fr = 0, fi = 0, pr = 0, pi = 0;
+            if (now) wave_value(o.wave, o.nwave, ts, fr, fi);
+            if (before) wave_value(o.wave, o.nwave, ts - 1, pr, pi);
+            const T vn = (fr * ar) - (fi * ai);
+            const T vo = (pr * ar) - (pi * ai);
+            const T dw = m * (vn - ((T(1) - sp) / (T(1) + sp)) * vo);
+            if (so == T(0)) {
+                o.A[i] = ao + dw;
+            } else {
+                o.W[i] = wo + dw;
+                o.A[i] = ao + (T(1) + so) * dw;
+            }
+        }
+    }
 }
 
 // Update and store one component.  so/sn/sp: own/next/prev sigma per element.
@@ -883,6 +934,7 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
     pml_h_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (TWO) pml_h_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
 
 template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false>
@@ -899,6 +951,7 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
     pml_e_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (TWO) pml_e_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
 
 // FAST mode sources.  With T eliminated the reference's E = m (T + S_n)

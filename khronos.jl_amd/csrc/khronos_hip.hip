@@ -838,6 +838,7 @@ struct Ctx final : CtxBase {
     Plan plan[2];
     bool planned = false;
     bool nt = false;           // nontemporal access for single-use streams (inner kernels)
+    bool src_in_pml = true;    // small FAST sources inside the PML launch (KH_SRC_LAUNCH=1: never)
     bool last_active = false;  // sources injected on the previous step
 
     static int env_int(const char *name, int def) {
@@ -886,6 +887,7 @@ struct Ctx final : CtxBase {
     void make_plans() {
         const int zb_inner = std::max(1, env_int("KH_ZB_INNER", 1));
         nt = env_int("KH_NT", 1) != 0;
+        src_in_pml = env_int("KH_SRC_LAUNCH", 0) == 0;
         for (int e = 0; e < 2; ++e) {
             Plan &P = plan[e];
             // sampled sigma back on the host
@@ -1020,20 +1022,42 @@ struct Ctx final : CtxBase {
         return p;
     }
 
+    // FAST sources the PML launch of this half step applies itself (empty:
+    // none, or source_fix_kernel launches)
+    std::vector<kh::FixSrc<T>> pml_srcs;
+
     void launch_half(int e) {
+        const bool electric = e == 1;
         // complex fields without conductivity: one PML launch updates both parts
         bool fuse = nparts == 2 && plan[e].pblocks > 0;
         for (int a = 0; a < 3; ++a) fuse = fuse && !(e ? fp[0].CD[a] : fp[0].CB[a]);
-        for (int part = 0; part < nparts; ++part) launch_half_part(e, part, fuse);
-        const bool electric = e == 1;
         // FAST mode sources (real part only: S += Re(f A),
         // src/Sources/Sources.jl:346-357) while sources are or were on the
-        // previous step; the switch is permanent and the kernel gates itself
-        // by the device timestep.  Sources share a launch unless two of them
-        // write the same cell of one component: then they go to successive
+        // previous step; the switch is permanent and the kernels gate
+        // themselves by the device timestep.  Small sources (points, lines:
+        // all of them together in at most 1024 wave-wide row segments of the
+        // PML launch) are applied by that launch as its lanes finish, in
+        // registration order; a lane in a source box waits for the waveform
+        // tables after its stores, which costs less than a launch only while
+        // few waves do it (a 192 x 160 plane made the E update 14% slower
+        // than a separate launch).  Otherwise source_fix_kernel runs after
+        // the update.
+        std::vector<kh::FixSrc<T>> list;
+        if (mode == KH_MODE_FAST && (active || last_active)) list = fix_sources(electric);
+        pml_srcs.clear();
+        if (src_in_pml && !list.empty() && plan[e].pblocks > 0 && !plan[e].inner &&
+            list.size() <= (size_t)kh::MAX_SRC) {
+            int64_t rows = 0;
+            for (const auto &o : list)
+                rows += (int64_t)o.d[1] * o.d[2] * ((o.d[0] + 127) / 128 + 1);  // a wave: 64 lanes x 2 cells
+            if (rows <= 1024) pml_srcs.swap(list);
+        }
+        for (int part = 0; part < nparts; ++part) launch_half_part(e, part, fuse);
+        pml_srcs.clear();
+        // Sources in separate launches share a launch unless two of them write
+        // the same cell of one component: then they go to successive
         // launches, in registration order.
-        if (mode == KH_MODE_FAST && (active || last_active)) {
-            std::vector<kh::FixSrc<T>> list = fix_sources(electric);
+        if (!list.empty()) {
             size_t q0 = 0;
             while (q0 < list.size()) {
                 kh::FixArgs<T> q{};
@@ -1268,6 +1292,14 @@ struct Ctx final : CtxBase {
             q.dt = dt;
             q.sy = g.sy();
             q.sz = g.sz();
+            // sources go to the real part (f[0] of a two-part launch)
+            if (part == 0) {
+                q.nsrc = (int)pml_srcs.size();
+                for (size_t k = 0; k < pml_srcs.size(); ++k) q.src[k] = pml_srcs[k];
+            }
+            q.clk = clk;
+            q.koff = koff;
+            q.cutoff = cutoff;
             q.nslab = P.npslab;
             for (int s = 0; s < P.npslab; ++s) q.slab[s] = P.pslab[s];
             q.nblock = P.pblocks;
