@@ -16,8 +16,9 @@ Prints one JSON line.  Mcell-updates/s = Nx*Ny*Nz*K / wall_s / 1e6
 planes sample, the graphs the window replays are captured before it, and
 ``config.window`` says what the window ran (graph replays, eager steps, DFT
 launches).  The roofline entry uses the E-update kernel (the dominant one):
-algorithmic bytes per launch / mean launch time from HIP events recorded
-around each launch on the library's stream, beside the measured HBM traffic
+algorithmic bytes per launch / mean launch time from the HIP event pair each
+launch stamps at its kernel's start and end on the library's stream
+(hipExtLaunchKernel), beside the measured HBM traffic
 per launch (committed counter summary) over the same time.  Also in the
 line: the 80^3 FP64 dipole's rate, a short Poisson serving stream (served vs
 one at a time, p50/p99, ideal-overlap bound) and the CPU oracle on all the
@@ -168,11 +169,11 @@ def overlap_bound(reqs):
     return bound, bound_cells, r_max, bw
 
 
-def measured_traffic(kernel_prefix: str):
-    """HBM bytes per dispatch of a kernel from the committed counter summary
-    (profiles/pmc_traffic_strip_waveguide_fp32_fast.md: FETCH_SIZE and
-    WRITE_SIZE runs scaled by a 1 GiB copy's counter readings), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_strip_waveguide_fp32_fast.md")
+def measured_traffic(kernel_prefix: str, summary: str = "pmc_traffic_strip_waveguide_fp32_fast.md"):
+    """HBM bytes per dispatch of a kernel from a committed counter summary
+    under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy's
+    counter readings), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -425,6 +426,8 @@ def metasurface_bench(args):
     per_step = {k: round(v[0] / k_steps, 5) for k, v in tm.items()}
     e_ms = tm["e_full"][0] / k_steps
     achieved = ab["E"] / (e_ms * 1e-3) / 1e9 if e_ms > 0 else 0.0
+    traffic = measured_traffic("kh::pml_e_kernel<float, true, false, true, true>",
+                               "pmc_traffic_metasurface_256x256x384_fp32.md")
     b_alg = (ab["H"] + ab["E"]) / cells
     value = cells * args.steps / wall / 1e6
     line = {
@@ -442,7 +445,10 @@ def metasurface_bench(args):
                    "launches_per_step": {k: round(v[1] / k_steps, 2) for k, v in tm.items()},
                    "hbm_fraction_whole_step": round(cells * args.steps / wall * b_alg / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_achieved": round(traffic / (e_ms * 1e-3) / 1e9, 1) if traffic and e_ms > 0 else None,
+                     "traffic_frac": (round(traffic / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                      if traffic and e_ms > 0 else None),
                      "kernel": "E update of one half step (pml_e_kernel, real and imaginary parts)",
                      "alg_bytes_per_launch": ab["E"], "kernel_ms": round(e_ms, 5)},
     }
@@ -540,8 +546,8 @@ def main(argv=None):
     wall = time.perf_counter() - t0
     counts = eng.launch_counts()
 
-    # per-kernel durations: HIP events around every launch on the library's
-    # stream (events need eager launches, so this is a separate run)
+    # per-kernel durations: the start/end events every launch stamps on the
+    # library's stream (events need eager launches, so this is a separate run)
     k_steps = min(args.steps, 200)
     eng.set_timing(k_steps * 16 + 64)
     eng.get_timing()  # clear
@@ -564,7 +570,9 @@ def main(argv=None):
         lut = os.environ.get("KH_LUT", "1") != "0" and len(vals) <= 256
         tmpl = f"{ctype}, true, false, " + ("true" if lut else "false")
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], "pml_e_kernel<" + tmpl.replace(" ", "") + ">"
-        trace_name = "kh::pml_e_kernel<" + tmpl + ">"
+        # (the summary may name the kernel with or without its trailing
+        # single-part flag)
+        trace_name = "kh::pml_e_kernel<" + tmpl
     else:
         dom, dom_bytes, dom_name = avg["e_full"], ab["E"], f"cascade_e_kernel<{ctype}>"
         trace_name = f"kh::cascade_e_kernel<{ctype}>"

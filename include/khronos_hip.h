@@ -430,7 +430,9 @@ int kh_adjoint_gradient(kh_ctx *ctx, int32_t n_monitors, const int32_t *monitor_
                         const int32_t box_lo[9], const int32_t box_n[9], double *grad);
 
 /* Kernel timing for measurement: while enabled (enable > 0), every update
- * and DFT launch is bracketed by HIP events on the context's stream; `enable`
+ * and DFT launch stamps a HIP event pair at its kernel's start and end on the
+ * context's stream (hipExtLaunchKernel: the dispatch gap before the kernel is
+ * not counted); `enable`
  * event pairs are created up front so that timing allocates nothing while it
  * runs.  kh_get_timing writes {total ms, launch count} for each of the
  * KH_TIMING_SLOTS kernel classes -- H inner box, H PML/full cascade, E inner

@@ -6,6 +6,7 @@
 // H update kernel, then due H-component DFT launches, E update (+E sources),
 // then due E-component DFT launches.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -338,6 +339,23 @@ struct Ctx final : CtxBase {
     struct EvPair { hipEvent_t a, b; int kind; };
     std::vector<EvPair> evs;   // recorded, not yet read
     std::vector<EvPair> pool;  // created up front so timing adds no allocation
+
+    // Launch on the context stream; with kernel timing on, the launch itself
+    // stamps an event pair at the kernel's start and end (the dispatch gap in
+    // front of the kernel is not counted, as a pair recorded around the
+    // launch would count it).
+    template <typename... Args, typename F = void (*)(Args...)>
+    void launch(int kind, F kernel, dim3 grid, dim3 blk, size_t shm, Args... args) {
+        if (timing) {
+            EvPair ev = take_event(kind);
+            hipExtLaunchKernelGGL(kernel, grid, blk, (std::uint32_t)shm, st, ev.a, ev.b, 0, args...);
+            HIPCHK(hipGetLastError());
+            evs.push_back(ev);
+        } else {
+            hipLaunchKernelGGL(kernel, grid, blk, shm, st, args...);
+            HIPCHK(hipGetLastError());
+        }
+    }
 
     EvPair take_event(int kind) {
         if (pool.empty()) {
@@ -797,24 +815,11 @@ struct Ctx final : CtxBase {
         q.poles = poles_dev;
         q.clk = clk;
         q.koff = koff;
-        EvPair ev{};
-        if (timing) {
-            ev = take_event(5);
-            HIPCHK(hipEventRecord(ev.a, st));
-        }
         const dim3 grid((unsigned)((mcells + 255) / 256)), blk(256);
-        if (nparts == 2) {
-            if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true, 2>), grid, blk, 0, st, q);
-            else hipLaunchKernelGGL((kh::material_kernel<T, false, 2>), grid, blk, 0, st, q);
-        } else {
-            if (mode == KH_MODE_FAST) hipLaunchKernelGGL((kh::material_kernel<T, true, 1>), grid, blk, 0, st, q);
-            else hipLaunchKernelGGL((kh::material_kernel<T, false, 1>), grid, blk, 0, st, q);
-        }
-        HIPCHK(hipGetLastError());
-        if (timing) {
-            HIPCHK(hipEventRecord(ev.b, st));
-            evs.push_back(ev);
-        }
+        void (*kfn)(kh::MatArgs<T>);
+        if (nparts == 2) kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 2> : kh::material_kernel<T, false, 2>;
+        else kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 1> : kh::material_kernel<T, false, 1>;
+        launch(5, kfn, grid, blk, 0, q);
     }
 
     double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22
@@ -1196,39 +1201,24 @@ struct Ctx final : CtxBase {
         const bool electric = e == 1;
         const Part &f = fp[part];
         dim3 blk(kh::TX, kh::TY, 1);
-        EvPair ev{};
         if (P.inner) {
-            if (timing) {
-                ev = take_event(2 * e);
-                HIPCHK(hipEventRecord(ev.a, st));
-            }
             kh::InnerArgs<T> ia = P.ia;
             for (int a = 0; a < 3; ++a) {
                 ia.A[a] = electric ? f.E[a] : f.H[a];
                 ia.F[a] = electric ? f.H[a] : f.E[a];
             }
             const bool pervox = ia.minv[0] != nullptr;
-            const dim3 grid(P.inner_blocks);
-#define KH_INNER(KER, PV, NTV) hipLaunchKernelGGL((kh::KER<T, PV, NTV>), grid, blk, 0, st, ia)
+            void (*kfn)(kh::InnerArgs<T>);
             if (electric) {
-                if (pervox) { if (nt) KH_INNER(inner_e_kernel, true, true); else KH_INNER(inner_e_kernel, true, false); }
-                else { if (nt) KH_INNER(inner_e_kernel, false, true); else KH_INNER(inner_e_kernel, false, false); }
+                if (pervox) kfn = nt ? kh::inner_e_kernel<T, true, true> : kh::inner_e_kernel<T, true, false>;
+                else kfn = nt ? kh::inner_e_kernel<T, false, true> : kh::inner_e_kernel<T, false, false>;
             } else {
-                if (pervox) { if (nt) KH_INNER(inner_h_kernel, true, true); else KH_INNER(inner_h_kernel, true, false); }
-                else { if (nt) KH_INNER(inner_h_kernel, false, true); else KH_INNER(inner_h_kernel, false, false); }
+                if (pervox) kfn = nt ? kh::inner_h_kernel<T, true, true> : kh::inner_h_kernel<T, true, false>;
+                else kfn = nt ? kh::inner_h_kernel<T, false, true> : kh::inner_h_kernel<T, false, false>;
             }
-#undef KH_INNER
-            HIPCHK(hipGetLastError());
-            if (timing) {
-                HIPCHK(hipEventRecord(ev.b, st));
-                evs.push_back(ev);
-            }
+            launch(2 * e, kfn, dim3(P.inner_blocks), blk, 0, ia);
         }
         if (P.cblocks) {
-            if (timing) {
-                ev = take_event(2 * e + 1);
-                HIPCHK(hipEventRecord(ev.a, st));
-            }
             kh::CascadeArgs<T> p = cascade_args(e, part);
             // sources add Re(f A) to the real part only
             p.nsrc = part == 0 ? srcl_n[e] : 0;
@@ -1237,19 +1227,9 @@ struct Ctx final : CtxBase {
             p.koff = koff;
             p.cutoff = cutoff;
             const dim3 cgrid(p.per_xcd * kh::NXCD), cblk(kh::CBLK);
-            if (electric) hipLaunchKernelGGL((kh::cascade_e_kernel<T>), cgrid, cblk, 0, st, p);
-            else hipLaunchKernelGGL((kh::cascade_h_kernel<T>), cgrid, cblk, 0, st, p);
-            HIPCHK(hipGetLastError());
-            if (timing) {
-                HIPCHK(hipEventRecord(ev.b, st));
-                evs.push_back(ev);
-            }
+            launch(2 * e + 1, electric ? kh::cascade_e_kernel<T> : kh::cascade_h_kernel<T>, cgrid, cblk, 0, p);
         }
         if (P.pblocks && !(fuse && part == 1)) {
-            if (timing) {
-                ev = take_event(2 * e + 1);
-                HIPCHK(hipEventRecord(ev.a, st));
-            }
             kh::PmlArgs<T> q{};
             for (int k = 0; k < (fuse ? 2 : 1); ++k) {
                 const Part &fk = fp[fuse ? k : part];
@@ -1307,44 +1287,26 @@ struct Ctx final : CtxBase {
             const bool pervox = q.minv[0] != nullptr;
             const dim3 pgrid(q.per_xcd * kh::NXCD), pblk(kh::CBLK);
             const bool lut_on = pervox && use_lut[e];
+            // template flags: per-voxel material, conductivity C stage,
+            // material bytes + table, both parts of a complex field
+            void (*kfn)(kh::PmlArgs<T>);
             if (fuse) {
-                if (electric) {
-                    if (lut_on) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, true, true>), pgrid, pblk, 0, st, q);
-                    else if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, false, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, false, false, true>), pgrid, pblk, 0, st, q);
-                } else {
-                    if (lut_on) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, true, true>), pgrid, pblk, 0, st, q);
-                    else if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, false, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, false, false, true>), pgrid, pblk, 0, st, q);
-                }
+                if (electric) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, true>
+                                  : pervox ? kh::pml_e_kernel<T, true, false, false, true>
+                                           : kh::pml_e_kernel<T, false, false, false, true>;
+                else kfn = lut_on ? kh::pml_h_kernel<T, true, false, true, true>
+                         : pervox ? kh::pml_h_kernel<T, true, false, false, true>
+                                  : kh::pml_h_kernel<T, false, false, false, true>;
             } else if (electric) {
-                if (lut_on) {
-                    if (cond) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false, true>), pgrid, pblk, 0, st, q);
-                } else if (cond) {
-                    if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, true>), pgrid, pblk, 0, st, q);
-                } else {
-                    if (pervox) hipLaunchKernelGGL((kh::pml_e_kernel<T, true, false>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_e_kernel<T, false, false>), pgrid, pblk, 0, st, q);
-                }
+                if (lut_on) kfn = cond ? kh::pml_e_kernel<T, true, true, true> : kh::pml_e_kernel<T, true, false, true>;
+                else if (cond) kfn = pervox ? kh::pml_e_kernel<T, true, true> : kh::pml_e_kernel<T, false, true>;
+                else kfn = pervox ? kh::pml_e_kernel<T, true, false> : kh::pml_e_kernel<T, false, false>;
             } else {
-                if (lut_on) {
-                    if (cond) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, true, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false, true>), pgrid, pblk, 0, st, q);
-                } else if (cond) {
-                    if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, true>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, true>), pgrid, pblk, 0, st, q);
-                } else {
-                    if (pervox) hipLaunchKernelGGL((kh::pml_h_kernel<T, true, false>), pgrid, pblk, 0, st, q);
-                    else hipLaunchKernelGGL((kh::pml_h_kernel<T, false, false>), pgrid, pblk, 0, st, q);
-                }
+                if (lut_on) kfn = cond ? kh::pml_h_kernel<T, true, true, true> : kh::pml_h_kernel<T, true, false, true>;
+                else if (cond) kfn = pervox ? kh::pml_h_kernel<T, true, true> : kh::pml_h_kernel<T, false, true>;
+                else kfn = pervox ? kh::pml_h_kernel<T, true, false> : kh::pml_h_kernel<T, false, false>;
             }
-            HIPCHK(hipGetLastError());
-            if (timing) {
-                HIPCHK(hipEventRecord(ev.b, st));
-                evs.push_back(ev);
-            }
+            launch(2 * e + 1, kfn, pgrid, pblk, 0, q);
         }
     }
 
@@ -1406,19 +1368,8 @@ struct Ctx final : CtxBase {
                 q.xrev[k] = ph_rev[0][k];
             }
             const size_t shm = (size_t)nfmax * 2 * sizeof(double);
-            EvPair ev{};
-            if (timing) {
-                ev = take_event(4);
-                HIPCHK(hipEventRecord(ev.a, st));
-            }
-            if (acc64) hipLaunchKernelGGL((kh::dft_kernel<T, double>), dim3(blocks), dim3(256), shm, st, q);
-            else hipLaunchKernelGGL((kh::dft_kernel<T, T>), dim3(blocks), dim3(256), shm, st, q);
-            HIPCHK(hipGetLastError());
+            launch(4, acc64 ? kh::dft_kernel<T, double> : kh::dft_kernel<T, T>, dim3(blocks), dim3(256), shm, q);
             ++(capturing ? cap_dfts : n_dft);
-            if (timing) {
-                HIPCHK(hipEventRecord(ev.b, st));
-                evs.push_back(ev);
-            }
         }
     }
 

@@ -143,3 +143,38 @@ def test_launch_counts_and_prepared_graphs():
         t += chunk
         for a, b in zip(_state(eager), _state(graph)):
             assert np.array_equal(a, b), t
+
+
+def test_kernel_timing_counts_and_results():
+    """kh_set_timing: every update, material and DFT launch stamps an event
+    pair at its kernel's start and end (hipExtLaunchKernel).  The counts per
+    kernel class are the launches made, the durations are positive and add up
+    to less than the wall time, and the fields equal an untimed eager run's
+    value for value."""
+    import time as _time
+    timed = _scene("f32", "fast", False)
+    plain = _scene("f32", "fast", False)
+    timed.prepare()
+    plain.prepare()
+    plain.engine.set_graph(0)
+    decs = [(md.component.electric, md.decimation) for md in timed.spec.dfts]
+    due = sum(any(e == el and t % d == 0 for e, d in decs) for t in range(40) for el in (False, True))
+    timed.engine.set_timing(True)
+    timed.engine.get_timing()
+    t0 = _time.perf_counter()
+    timed.step(40)
+    timed.engine.sync()
+    wall_ms = (_time.perf_counter() - t0) * 1e3
+    tm = timed.engine.get_timing()
+    timed.engine.set_timing(False)
+    plain.step(40)
+    _note(f"[timing] 40 steps, wall {wall_ms:.2f} ms: {tm}")
+    for a, b in zip(_state(timed), _state(plain)):
+        assert np.array_equal(a, b)
+    assert tm["h_full"][1] == 40 and tm["e_full"][1] == 40
+    assert tm["material"][1] == 40
+    assert tm["dft"][1] == due
+    assert tm["h_inner"][1] == 0 and tm["e_inner"][1] == 0
+    for k in ("h_full", "e_full", "material", "dft"):
+        assert tm[k][0] > 0, k
+    assert sum(v[0] for v in tm.values()) < wall_ms
