@@ -182,6 +182,11 @@ struct MatArgs {
     int64_t sy, sz;
     int npole;
     const PoleDev<T> *poles;
+    // per box cell: 1 where some pole's sigma or chi3 is non-zero, or null.
+    // Elsewhere P stays zero (it is only updated where its sigma is not), so
+    // fPD, the FAST corrections and the chi3 delta are zero and the pass would
+    // store what it read: such cells are skipped.
+    const uint8_t *active;
     // FAST with a periodic axis: the wrapped ghost of a corrected boundary
     // cell is rewritten from the corrected linear field
     int N[3];
@@ -1042,6 +1047,7 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ncell = (int64_t)p.n[0] * p.n[1] * p.n[2];
     if (c >= ncell) return;
+    if (p.active && !p.active[c]) return;
     const int ux = (int)(c % p.n[0]);
     const int64_t r = c / p.n[0];
     const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
@@ -1154,6 +1160,26 @@ __global__ __launch_bounds__(256) void material_kernel(MatArgs<T> p) {
         for (int q = 0; q < NP; ++q)
             for (int a = 0; a < 3; ++a) p.fpd[bnext][q][a][c] = sum[q][a];
     }
+}
+
+// The material box's active cells (see MatArgs::active), once at set-up.
+template <typename T>
+__global__ __launch_bounds__(256) void material_mask_kernel(MatArgs<T> p, uint8_t *active) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ncell = (int64_t)p.n[0] * p.n[1] * p.n[2];
+    if (c >= ncell) return;
+    const int ux = (int)(c % p.n[0]);
+    const int64_t r = c / p.n[0];
+    const int uy = (int)(r % p.n[1]), uz = (int)(r / p.n[1]);
+    const int g[3] = {p.lo[0] + ux, p.lo[1] + uy, p.lo[2] + uz};
+    bool on = p.chi3 && p.chi3[c] != T(0);
+    for (int k = 0; k < p.npole && !on; ++k) {
+        const PoleDev<T> &pl = p.poles[k];
+        const int kx = g[0] - pl.lo[0], ky = g[1] - pl.lo[1], kz = g[2] - pl.lo[2];
+        if (kx < 0 || ky < 0 || kz < 0 || kx >= pl.n[0] || ky >= pl.n[1] || kz >= pl.n[2]) continue;
+        on = pl.sigma[(size_t)kx + (size_t)pl.n[0] * ((size_t)ky + (size_t)pl.n[1] * (size_t)kz)] != T(0);
+    }
+    active[c] = on ? 1 : 0;
 }
 
 // Periodic / Bloch wrap of one field group along one axis

@@ -745,6 +745,10 @@ struct Ctx final : CtxBase {
     void write_pole(int id, int which, int comp, const void *host, size_t bytes) override {
         drop_checkpoints();
         pole_io(id, which, comp, const_cast<void *>(host), bytes, false);
+        if (mat_skip) {
+            mat_skip = false;
+            drop_graph();  // captured material launches carry the mask
+        }
     }
 
     // Material box = bounding box of chi3 != 0 and every pole's box; its
@@ -784,10 +788,30 @@ struct Ctx final : CtxBase {
         if (has_chi3 && mode == KH_MODE_FAST)
             for (int pt = 0; pt < nparts; ++pt)
                 for (int a = 0; a < 3; ++a) delta[pt][a] = (T *)dalloc(nb);
+        mat_active = (uint8_t *)dalloc((size_t)mcells);
+        kh::MatArgs<T> q = mat_args();
+        hipLaunchKernelGGL(kh::material_mask_kernel<T>, dim3((unsigned)((mcells + 255) / 256)), dim3(256), 0, st, q,
+                           mat_active);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // cells of the material box the pass updates (null: all); dropped for
+    // good once a polarization array is written from the host, which can put
+    // a non-zero P where its sigma is zero
+    uint8_t *mat_active = nullptr;
+    bool mat_skip = true;
+
     void launch_material() {
+        kh::MatArgs<T> q = mat_args();
+        const dim3 grid((unsigned)((mcells + 255) / 256)), blk(256);
+        void (*kfn)(kh::MatArgs<T>);
+        if (nparts == 2) kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 2> : kh::material_kernel<T, false, 2>;
+        else kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 1> : kh::material_kernel<T, false, 1>;
+        launch(5, kfn, grid, blk, 0, q);
+    }
+
+    kh::MatArgs<T> mat_args() const {
         kh::MatArgs<T> q{};
         for (int a = 0; a < 3; ++a) {
             for (int pt = 0; pt < nparts; ++pt) {
@@ -815,11 +839,8 @@ struct Ctx final : CtxBase {
         q.poles = poles_dev;
         q.clk = clk;
         q.koff = koff;
-        const dim3 grid((unsigned)((mcells + 255) / 256)), blk(256);
-        void (*kfn)(kh::MatArgs<T>);
-        if (nparts == 2) kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 2> : kh::material_kernel<T, false, 2>;
-        else kfn = mode == KH_MODE_FAST ? kh::material_kernel<T, true, 1> : kh::material_kernel<T, false, 1>;
-        launch(5, kfn, grid, blk, 0, q);
+        q.active = mat_skip ? mat_active : nullptr;
+        return q;
     }
 
     double round_time(int64_t n) const { return (double)((T)n * dt); }  // src/Simulation.jl:22

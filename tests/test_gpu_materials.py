@@ -140,3 +140,31 @@ def test_pole_write_read_roundtrip():
     mask[box] = True
     assert np.array_equal(Y[mask], X[mask])
     assert np.all(Y[~mask] == 0)
+
+
+@pytest.mark.parametrize("precision,mode", [("f32", "fast"), ("f64", "fast"), ("f64", "exact")])
+def test_inactive_material_cells_skipped(precision, mode):
+    """The material pass skips the cells of its box where no pole's sigma and
+    no chi3 is non-zero: P stays zero there, so fPD, the FAST corrections and
+    the chi3 delta are zero and the pass would store what it read.  A host
+    write of a polarization array turns the skip off for good (it may put a
+    non-zero P where sigma is zero).  Both runs give the same fields,
+    polarization and DFTs (equal as numbers: only the sign of a zero may
+    differ, where the full pass adds a zero correction to -0)."""
+    skip = _mat_sim(precision, mode)
+    full = _mat_sim(precision, mode)
+    skip.prepare()
+    full.prepare()
+    full.engine.write_pole(0, 0, 0, full.engine.read_pole(0, 0, 0))  # zeros back: skip off
+    _note(f"[materials] skip vs full pass, {precision} {mode}")
+    for chunk in (40, 80):
+        skip.step(chunk)
+        full.step(chunk)
+        for C in COMPS:
+            assert np.array_equal(skip.read_field(C()), full.read_field(C())), C.__name__
+        for pa, pb in zip(_poles(skip), _poles(full)):
+            for x, y in zip(pa, pb):
+                assert np.array_equal(x, y)
+        for i in range(len(skip.spec.dfts)):
+            assert np.array_equal(skip.engine.read_dft(i), full.engine.read_dft(i))
+    assert any(np.any(skip.read_field(C()) != 0) for C in COMPS)
