@@ -21,8 +21,9 @@ launch stamps at its kernel's start and end on the library's stream
 (hipExtLaunchKernel), beside the measured HBM traffic
 per launch (committed counter summary) over the same time.  Also in the
 line: the 80^3 FP64 dipole's rate, a short Poisson serving stream (served vs
-one at a time, p50/p99, ideal-overlap bound) and the CPU oracle on all the
-process's cores (at most 16) and on one core.
+one at a time, p50/p99, ideal-overlap bound), the complex metasurface's rate
+(200 steps), the 400 x 400 adjoint gradient's latency and the CPU oracle on
+all the process's cores (at most 16) and on one core.
 """
 from __future__ import annotations
 
@@ -323,11 +324,18 @@ def adjoint_bend_bench(args):
     if not torch.cuda.is_available():
         sys.stderr.write("no GPU visible\n")
         return 1
+    print(json.dumps(adjoint_line(args.steps, args.warmup, cpu=not args.no_cpu_baseline)), flush=True)
+    return 0
+
+
+def adjoint_line(steps: int, warmup: int, cpu: bool = True) -> dict:
+    """The adjoint bench line (see adjoint_bend_bench); cpu: time the
+    frequency-domain gradient on the CPU oracle beside it."""
     from khronos_hip import workloads as W
     opt, rho = W.adjoint_bend(precision="f32", setup_precision="f64")
-    for _ in range(max(1, args.warmup)):
+    for _ in range(max(1, warmup)):
         opt(rho)
-    k = max(1, args.steps)
+    k = max(1, steps)
     tf = tg = 0.0
     for _ in range(k):
         opt.update_design(rho)
@@ -345,7 +353,7 @@ def adjoint_bend_bench(args):
         "metric": "adjoint gradient latency (forward run + time-reversed sweep, all design parameters)",
         "value": round(wall * 1e3, 2),
         "unit": "ms per gradient",
-        "n_gpus": 1, "steps": k, "warmup": args.warmup,
+        "n_gpus": 1, "steps": k, "warmup": warmup,
         "ms_per_step": round(wall * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"2D waveguide bend 400x400 cells, 120x120 design parameters rho ~ U[0,1] (seed 0), "
@@ -353,7 +361,7 @@ def adjoint_bend_bench(args):
                                "FP64 setup and gradient accumulation",
                    "forward_ms": round(tf / k * 1e3, 2), "sweep_ms": round(tg / k * 1e3, 2)},
     }
-    if not args.no_cpu_baseline:
+    if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_engine import OracleEngine  # the checker, used here only as the CPU baseline
         cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
@@ -377,8 +385,8 @@ def adjoint_bend_bench(args):
                                           "frequency, the overlap sum) on the C oracle (-O3 -fopenmp, "
                                           f"{cores} threads), timed once: forward {(t1 - t0) * 1e3:.0f} ms",
                                 "gradient_shape": list(np.shape(g_ref))}
-    print(json.dumps(line), flush=True)
-    return 0
+    opt._close_forward()
+    return line
 
 
 def metasurface_bench(args):
@@ -393,29 +401,36 @@ def metasurface_bench(args):
     if not torch.cuda.is_available():
         sys.stderr.write("no GPU visible\n")
         return 1
+    print(json.dumps(metasurface_line(args.steps, args.warmup, args.graph_steps)), flush=True)
+    return 0
+
+
+def metasurface_line(steps: int, warmup: int, graph_steps: int = 16) -> dict:
+    """The metasurface bench line (see metasurface_bench)."""
+    import torch
     from khronos_hip.workloads import metasurface, algorithmic_bytes
     sim = metasurface(precision="f32", mode="fast")
     sim.prepare()
     eng = sim.engine
     cells = sim.Nx * sim.Ny * sim.Nz
     ab = algorithmic_bytes(sim.spec)
-    eng.set_graph(args.graph_steps)
+    eng.set_graph(graph_steps)
     period = dft_period(sim.spec)
-    sim.step(args.warmup)
+    sim.step(warmup)
     extra = (-sim.timestep) % period
     sim.step(extra)
-    eng.prepare_graphs(args.steps)
+    eng.prepare_graphs(steps)
     eng.sync()
     torch.cuda.synchronize()
     eng.launch_counts()
     t_start = sim.timestep
     t0 = time.perf_counter()
-    sim.step(args.steps)
+    sim.step(steps)
     eng.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     counts = eng.launch_counts()
-    k_steps = min(args.steps, 100)
+    k_steps = min(steps, 100)
     eng.set_timing(k_steps * 24 + 64)
     eng.get_timing()
     sim.step(k_steps)
@@ -429,21 +444,21 @@ def metasurface_bench(args):
     traffic = measured_traffic("kh::pml_e_kernel<float, true, false, true, true>",
                                "pmc_traffic_metasurface_256x256x384_fp32.md")
     b_alg = (ab["H"] + ab["E"]) / cells
-    value = cells * args.steps / wall / 1e6
+    value = cells * steps / wall / 1e6
     line = {
         "metric": "Mcell-updates/s (complex FP32 3D Yee + CPML + Lorentz ADE + chi3, Bloch x/y)",
-        "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5), "higher_is_better": True,
+        "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(wall / steps * 1e3, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "metasurface 256x256x384, Bloch x/y (seeded kx, ky), 16-cell CPML z, 8x8 cylinders "
                                "(radii U[0.2,0.4]) with 2 Lorentz poles + chi3 1e-3, plane-wave Gaussian pulse, "
                                "complex FP32 fields, 100-frequency flux plane (f64 accumulators)",
-                   "cells": cells, "warmup_steps_run": args.warmup + extra,
-                   "window": {"timesteps": [t_start, t_start + args.steps - 1], "dft_period": period, **counts},
+                   "cells": cells, "warmup_steps_run": warmup + extra,
+                   "window": {"timesteps": [t_start, t_start + steps - 1], "dft_period": period, **counts},
                    "B_alg_bytes_per_cell_update": round(b_alg, 2),
                    "kernel_ms_per_step": per_step,
                    "launches_per_step": {k: round(v[1] / k_steps, 2) for k, v in tm.items()},
-                   "hbm_fraction_whole_step": round(cells * args.steps / wall * b_alg / 1e9 / HBM_PEAK_GBS, 4)},
+                   "hbm_fraction_whole_step": round(cells * steps / wall * b_alg / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_achieved": round(traffic / (e_ms * 1e-3) / 1e9, 1) if traffic and e_ms > 0 else None,
@@ -452,8 +467,8 @@ def metasurface_bench(args):
                      "kernel": "E update of one half step (pml_e_kernel, real and imaginary parts)",
                      "alg_bytes_per_launch": ab["E"], "kernel_ms": round(e_ms, 5)},
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    sim.engine.close()
+    return line
 
 
 def main(argv=None):
@@ -627,6 +642,18 @@ def main(argv=None):
         line["config1_dipole_80_f64"] = config1_leg()
         sys.stderr.write(f"serving leg ({args.leg_requests} Poisson requests)\n")
         line["serving"] = serving_leg(args.leg_requests, args.leg_rate)
+        sys.stderr.write("metasurface leg (256x256x384 complex FP32)\n")
+        m = metasurface_line(200, 20)
+        line["metasurface"] = {"workload": m["config"]["workload"], "value": m["value"], "unit": m["unit"],
+                               "ms_per_step": m["ms_per_step"], "steps": m["steps"],
+                               "kernel_ms_per_step": m["config"]["kernel_ms_per_step"],
+                               "roofline_frac": m["roofline"]["frac"], "traffic_frac": m["roofline"]["traffic_frac"],
+                               "full_line": "python bench.py --metasurface"}
+        sys.stderr.write("adjoint leg (400x400 bend, all 14400 design parameters)\n")
+        a = adjoint_line(3, 1, cpu=False)
+        line["adjoint_bend"] = {"workload": a["config"]["workload"], "value": a["value"], "unit": a["unit"],
+                                "forward_ms": a["config"]["forward_ms"], "sweep_ms": a["config"]["sweep_ms"],
+                                "full_line": "python bench.py --adjoint (with its timed CPU baseline)"}
     if not args.no_cpu_baseline:
         spec = sim.spec
         eng.close()
