@@ -595,6 +595,17 @@ def main(argv=None):
     traffic = measured_traffic(trace_name) if args.precision == "f32" else None
     b_alg = (ab["H"] + ab["E"]) / cells
     step_kernel_ms = sum(v[0] for v in tm.values()) / k_steps
+    # the H update beside it (FAST whole-grid PML kernel, scalar mu)
+    h_ms = avg["h_full"]
+    roofline_h = None
+    if args.mode == "fast" and h_ms > 0 and avg["h_inner"] == 0:
+        h_traffic = measured_traffic(f"kh::pml_h_kernel<{ctype}, false, false, false") if args.precision == "f32" \
+            else None
+        roofline_h = {"kernel": f"pml_h_kernel<{ctype},false,false,false>", "alg_bytes_per_launch": ab["H"],
+                      "kernel_ms": round(h_ms, 5), "achieved": round(ab["H"] / (h_ms * 1e-3) / 1e9, 1),
+                      "frac": round(ab["H"] / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": h_traffic,
+                      "traffic_frac": (round(h_traffic / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                       if h_traffic else None)}
 
     line = {
         "metric": f"Mcell-updates/s ({pname} 3D Yee + CPML time step)",
@@ -637,6 +648,8 @@ def main(argv=None):
             "kernel_ms": round(dom, 5),
         },
     }
+    if roofline_h:
+        line["roofline_h_update"] = roofline_h
     if not args.no_legs:
         sys.stderr.write("config-1 leg (80^3 FP64 dipole)\n")
         line["config1_dipole_80_f64"] = config1_leg()
