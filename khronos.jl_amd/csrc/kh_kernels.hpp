@@ -613,6 +613,14 @@ __global__ __launch_bounds__(256) void inner_e_kernel(InnerArgs<T> p) {
 // fields (TWO kernels) a lane updates the real part and then the imaginary
 // part of its pair: the material bytes, sigma values and indexing are read
 // and computed once for both.
+// A source box whose FAST corrections for a block of steps were computed
+// ahead of the updates (source_dw_kernel): the block step's row holds the
+// boxes one after another, each dense with x fastest, from `off` on.
+struct PreBox {
+    int s[3], d[3];
+    int64_t off;
+};
+
 struct PSlab {
     int lo[3];      // first cell: lo[0] odd
     int n[3];       // extent: n[0] even
@@ -670,6 +678,15 @@ struct PmlArgs {
     const int64_t *clk;
     int koff;
     double cutoff;
+    // or: the half step's sources, all on component pre_comp with pairwise
+    // disjoint boxes, as corrections computed for the block of steps ahead of
+    // time: the value at box cell i of block step koff is
+    // pre_dw[koff * pre_cells + pre[q].off + i], loaded with the lane's other
+    // values and added to the updated value (and W) before the store
+    int pre_n, pre_comp;
+    PreBox pre[MAX_SRC];
+    const T *pre_dw;
+    int64_t pre_cells;
 };
 
 // the ghost value wrap_kernel would write from interior element i
@@ -732,6 +749,28 @@ __device__ __forceinline__ void pml_load_uw(const PmlPart<T> &f, int a, size_t i
     if (nw) c.wo = ldo<T, true>((const T *)f.W[a], idx);
 }
 
+// The precomputed corrections of the lane's pair (x, x+1) in row y, plane z
+// (at most one box holds a cell).
+template <typename T>
+__device__ __forceinline__ void pml_pre_load(const PmlArgs<T> &p, int x, int y, int z, bool v0, bool v1, T &dw0,
+                                             T &dw1, bool &h0, bool &h1) {
+    const T *row = p.pre_dw + (int64_t)p.koff * p.pre_cells;
+    for (int q = 0; q < p.pre_n; ++q) {
+        const PreBox &b = p.pre[q];
+        if (y < b.s[1] || y >= b.s[1] + b.d[1] || z < b.s[2] || z >= b.s[2] + b.d[2]) continue;
+        const int64_t r = b.off + (int64_t)b.d[0] * ((int64_t)(y - b.s[1]) + (int64_t)b.d[1] * (z - b.s[2]));
+        const int i0 = x - b.s[0];
+        if (v0 && i0 >= 0 && i0 < b.d[0]) {
+            dw0 = row[r + i0];
+            h0 = true;
+        }
+        if (v1 && i0 + 1 >= 0 && i0 + 1 < b.d[0]) {
+            dw1 = row[r + i0 + 1];
+            h1 = true;
+        }
+    }
+}
+
 // The FAST source correction at the lane's pair (x, x+1) in row y, plane z,
 // once the update has stored its values: source by source in registration
 // order, the same loads and operations as source_fix_kernel (the lane reads
@@ -774,10 +813,13 @@ __device__ __forceinline__ void pml_sources(const PmlArgs<T> &p, size_t idx, int
 }
 
 // Update and store one component.  so/sn/sp: own/next/prev sigma per element.
+// dw0/dw1 with h0/h1: a precomputed source correction of the element, added
+// as source_fix_kernel would add it after the store.
 template <typename T>
 __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T> &f, int a, size_t idx,
                                            const PmlComp<T> &c, T K0, T K1, T so0, T so1, T sn0, T sn1, T sp0,
-                                           T sp1, bool v0, bool v1) {
+                                           T sp1, bool v0, bool v1, T dw0 = T(0), T dw1 = T(0), bool h0 = false,
+                                           bool h1 = false) {
     using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
     if (c.nc) {
@@ -806,8 +848,8 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T>
         sto<T, true>(f.U[a], idx, un);
     }
     V an = c.ao;
+    V wn = c.wo;  // loaded and stored with the W stage only
     if (c.nw) {
-        V wn = c.wo;
         if (v0) {
             if (so0 != T(0)) {
                 wn.x = ((T(1) - sp0) * c.wo.x + c.m.x * in0) / (T(1) + sp0);
@@ -824,11 +866,28 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T>
                 an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
             }
         }
-        sto<T, true>(f.W[a], idx, wn);
     } else {
         if (v0) an.x = ((T(1) - sp0) * c.ao.x + c.m.x * in0) / (T(1) + sp0);
         if (v1) an.y = ((T(1) - sp1) * c.ao.y + c.m.y * in1) / (T(1) + sp1);
     }
+    // (a non-zero own-axis sigma implies the W stage)
+    if (h0) {
+        if (so0 == T(0)) {
+            an.x = an.x + dw0;
+        } else {
+            wn.x = wn.x + dw0;
+            an.x = an.x + (T(1) + so0) * dw0;
+        }
+    }
+    if (h1) {
+        if (so1 == T(0)) {
+            an.y = an.y + dw1;
+        } else {
+            wn.y = wn.y + dw1;
+            an.y = an.y + (T(1) + so1) * dw1;
+        }
+    }
+    if (c.nw) sto<T, true>(f.W[a], idx, wn);
     sto<T, true>(f.A[a], idx, an);
 }
 
@@ -861,7 +920,8 @@ __device__ __forceinline__ void pml_load_all(const PmlArgs<T> &p, const PmlPart<
 // One part of the H update at the lane's pair (x, x+1), row y, plane z.
 template <typename T, bool COND>
 __device__ __forceinline__ void pml_h_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
-                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3]) {
+                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3],
+                                           T dw0 = T(0), T dw1 = T(0), bool h0 = false, bool h1 = false) {
     const int64_t sy = p.sy, sz = p.sz;
     const T *Ex = f.F[0], *Ey = f.F[1], *Ez = f.F[2];
     const auto ex = ld2(Ex, idx);
@@ -889,15 +949,20 @@ __device__ __forceinline__ void pml_h_part(const PmlArgs<T> &p, const PmlPart<T>
     const T Ky1 = dt * (ix * (ez_x2 - ez.y) - iz * (ex1.y - ex.y));
     const T Kz0 = dt * (iy * (ex_y1.x - ex.x) - ix * (ey.y - ey.x));
     const T Kz1 = dt * (iy * (ex_y1.y - ex.y) - ix * (ey_x2 - ey.y));
-    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    const int pc = p.pre_comp;
+    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1, dw0, dw1, h0 && pc == 0,
+               h1 && pc == 0);
+    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1, dw0, dw1, h0 && pc == 1,
+               h1 && pc == 1);
+    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1, dw0, dw1, h0 && pc == 2,
+               h1 && pc == 2);
 }
 
 // One part of the E update at the lane's pair.
 template <typename T, bool COND>
 __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
-                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3]) {
+                                           int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3],
+                                           T dw0 = T(0), T dw1 = T(0), bool h0 = false, bool h1 = false) {
     const int64_t sy = p.sy, sz = p.sz;
     const T *Hx = f.F[0], *Hy = f.F[1], *Hz = f.F[2];
     const auto hx = ld2(Hx, idx), hy = ld2(Hy, idx), hz = ld2(Hz, idx);
@@ -918,43 +983,57 @@ __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T>
     const T Ky1 = dt * (ix * (hz.x - hz.y) - iz * (hxm.y - hx.y));
     const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
     const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
-    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1);
-    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1);
-    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1);
+    const int pc = p.pre_comp;
+    pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1, dw0, dw1, h0 && pc == 0,
+               h1 && pc == 0);
+    pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1, dw0, dw1, h0 && pc == 1,
+               h1 && pc == 1);
+    pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1, dw0, dw1, h0 && pc == 2,
+               h1 && pc == 2);
 }
 
 // TWO: complex fields, p.f[0] the real and p.f[1] the imaginary part (no
-// conductivity: its C stage is per part).
-template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false>
+// conductivity: its C stage is per part).  PRE: the launch adds precomputed
+// source corrections (real fields without conductivity; a separate variant so
+// that the others keep their register count).
+template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false>
 __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
+    static_assert(!(PRE && (COND || TWO)), "precomputed sources: real fields without conductivity");
     int x, y, z;
     bool v0, v1;
     if (!pml_lane(p, x, y, z, v0, v1)) return;
     const size_t idx = (size_t)x + (size_t)y * p.sy + (size_t)z * p.sz;
     const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    T dw0 = T(0), dw1 = T(0);
+    bool h0 = false, h1 = false;
+    if (PRE) pml_pre_load(p, x, y, z, v0, v1, dw0, dw1, h0, h1);
     PmlComp<T> c[3];
     pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
     pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
-    pml_h_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    pml_h_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
     if (TWO) pml_h_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
 
-template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false>
+template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false>
 __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
+    static_assert(!(PRE && (COND || TWO)), "precomputed sources: real fields without conductivity");
     int x, y, z;
     bool v0, v1;
     if (!pml_lane(p, x, y, z, v0, v1)) return;
     const size_t idx = (size_t)x + (size_t)y * p.sy + (size_t)z * p.sz;
     const T sx0 = p.sig[0][x], sx1 = p.sig[0][x + 1], sgy = p.sig[1][y], sgz = p.sig[2][z];
+    T dw0 = T(0), dw1 = T(0);
+    bool h0 = false, h1 = false;
+    if (PRE) pml_pre_load(p, x, y, z, v0, v1, dw0, dw1, h0, h1);
     PmlComp<T> c[3];
     pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
     pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
-    pml_e_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
+    pml_e_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
     if (TWO) pml_e_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
@@ -1022,6 +1101,67 @@ __global__ __launch_bounds__(256) void source_fix_kernel(FixArgs<T> p) {
         o.W[idx] = wo + dw;
         o.A[idx] = ao + (T(1) + so) * dw;
     }
+}
+
+// The FAST corrections of a half step's sources for the nsteps steps of a
+// block, ahead of the updates: row k (block step k, timestep = device clock +
+// k) holds the boxes one after another; each value is the dw source_fix_kernel
+// would add at that cell and step, or -0 (the exact identity of an addition)
+// where the sources are off.  Cells outside 1..N are never read.
+template <typename T>
+struct PreSrcArgs {
+    int n;
+    PreBox box[MAX_SRC];
+    const T *amp[MAX_SRC];
+    const double *wave[MAX_SRC];
+    int64_t nwave[MAX_SRC];
+    int prev;
+    const T *minv;        // per-voxel or null (minv_s)
+    T minv_s;
+    const T *sig_prev;
+    T dt;
+    double cutoff;
+    const int64_t *clk;
+    int nsteps;
+    int64_t cells;        // per row
+    int n0, n1, n2;
+    int64_t sy, sz;
+    T *dw;                // nsteps rows
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void source_dw_kernel(PreSrcArgs<T> p) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)p.nsteps * p.cells) return;
+    const int k = (int)(t / p.cells);
+    const int64_t ci = t - (int64_t)k * p.cells;
+    int q = 0;
+    while (q + 1 < p.n && ci >= p.box[q + 1].off) ++q;
+    const PreBox &b = p.box[q];
+    const int64_t li = ci - b.off;
+    const int ix = (int)(li % b.d[0]);
+    const int64_t r = li / b.d[0];
+    const int iy = (int)(r % b.d[1]), iz = (int)(r / b.d[1]);
+    const int g[3] = {b.s[0] + ix, b.s[1] + iy, b.s[2] + iz};
+    T out = -T(0);
+    if (g[0] >= 1 && g[0] <= p.n0 && g[1] >= 1 && g[1] <= p.n1 && g[2] >= 1 && g[2] <= p.n2) {
+        const size_t idx = (size_t)g[0] + (size_t)g[1] * p.sy + (size_t)g[2] * p.sz;
+        const T ar = p.amp[q][2 * li], ai = p.amp[q][2 * li + 1];
+        const T m = p.minv ? p.minv[idx] : p.minv_s;
+        const T sp = p.sig_prev[g[p.prev]];
+        const int64_t ts = *p.clk + k;
+        const bool now = sources_on(ts, p.dt, p.cutoff);
+        const bool before = ts >= 1 && sources_on(ts - 1, p.dt, p.cutoff);
+        if (now || before) {
+            T fr = 0, fi = 0, pr = 0, pi = 0;
+            if (now) wave_value(p.wave[q], p.nwave[q], ts, fr, fi);
+            if (before) wave_value(p.wave[q], p.nwave[q], ts - 1, pr, pi);
+            const T vn = (fr * ar) - (fi * ai);
+            const T vo = (pr * ar) - (pi * ai);
+            out = m * (vn - ((T(1) - sp) / (T(1) + sp)) * vo);
+        }
+    }
+    p.dw[t] = out;
 }
 
 // End of a block of steps: advance the device timestep (one lane writes).

@@ -1052,6 +1052,119 @@ struct Ctx final : CtxBase {
     // none, or source_fix_kernel launches)
     std::vector<kh::FixSrc<T>> pml_srcs;
 
+    // all of the sources together in at most 1024 wave-wide row segments of
+    // the PML launch (a wave: 64 lanes x 2 cells)
+    static bool pml_sources_small(const std::vector<kh::FixSrc<T>> &list) {
+        int64_t rows = 0;
+        for (const auto &o : list) rows += (int64_t)o.d[1] * o.d[2] * ((o.d[0] + 127) / 128 + 1);
+        return rows <= 1024;
+    }
+
+    // Sources precomputed per block of steps: a half step whose sources (at
+    // most MAX_SRC) are all on one component with pairwise disjoint boxes has
+    // their corrections for the whole block computed by one launch at the
+    // block's start (source_dw_kernel, from the device clock), and the PML
+    // launch adds them in registers before its stores: no per-step source
+    // launch and no wait on the waveform tables inside the update.
+    struct PrePlan {
+        bool ok = false;
+        int comp = 0, n = 0, cap = 0;  // cap: steps a buffer row set holds
+        int64_t cells = 0;
+        kh::PreBox box[kh::MAX_SRC];
+        T *buf = nullptr;
+        size_t bytes = 0;
+    };
+    PrePlan pre[2];
+    uint64_t pre_version = 0;
+    bool pre_ready[2] = {false, false};  // the current block's corrections are computed
+
+    void plan_pre() {
+        if (pre_version == src_version) return;
+        pre_version = src_version;
+        for (int e = 0; e < 2; ++e) {
+            PrePlan &P = pre[e];
+            P.ok = false;
+            // (real fields without conductivity: the kernels' PRE variants)
+            const bool cond = e ? (msig[0] || msig[1] || msig[2]) : (msig[3] || msig[4] || msig[5]);
+            if (mode != KH_MODE_FAST || !src_in_pml || nparts != 1 || cond || plan[e].pblocks == 0 || plan[e].inner)
+                continue;
+            const std::vector<kh::FixSrc<T>> list = fix_sources(e == 1);
+            if (list.empty() || list.size() > (size_t)kh::MAX_SRC) continue;
+            bool one = true;
+            for (size_t i = 0; i < list.size() && one; ++i) {
+                one = list[i].own == list[0].own;
+                for (size_t j = 0; j < i && one; ++j) one = !boxes_meet(list[i], list[j]);
+            }
+            if (!one) continue;
+            // small sources stay with the end-of-lane form (no extra launch)
+            if (pml_sources_small(list)) continue;
+            int64_t cells = 0;
+            for (size_t i = 0; i < list.size(); ++i) {
+                kh::PreBox &b = P.box[i];
+                for (int k = 0; k < 3; ++k) { b.s[k] = list[i].s[k]; b.d[k] = list[i].d[k]; }
+                b.off = cells;
+                cells += (int64_t)b.d[0] * b.d[1] * b.d[2];
+            }
+            // rows for a main graph block, a short block or a checkpoint
+            // interval, within 64 MiB
+            int want = std::max(std::max(graph_len, graph_steps), 1);
+            if (ckpt_every > 0 && ckpt_every <= 1024) want = std::max(want, ckpt_every);
+            const int64_t fit = ((int64_t)64 << 20) / (cells * (int64_t)sizeof(T));
+            const int cap = (int)std::min<int64_t>(want, fit);
+            if (cap < 1) continue;
+            const size_t bytes = (size_t)cells * cap * sizeof(T);
+            if (bytes > P.bytes) {
+                // a new buffer: graphs captured with the old one are dropped
+                drop_graph();
+                P.buf = (T *)dalloc(bytes);
+                P.bytes = bytes;
+            }
+            P.comp = list[0].own;
+            P.n = (int)list.size();
+            P.cells = cells;
+            P.cap = cap;
+            P.ok = true;
+        }
+    }
+
+    // corrections for the next len steps (block step k: device clock + k);
+    // plan_pre() has run since the sources last changed (it allocates, so
+    // never inside a capture)
+    void launch_pre(int len) {
+        for (int e = 0; e < 2; ++e) {
+            pre_ready[e] = false;
+            const PrePlan &P = pre[e];
+            if (!P.ok || !(active || last_active) || len > P.cap) continue;
+            const std::vector<kh::FixSrc<T>> list = fix_sources(e == 1);
+            kh::PreSrcArgs<T> a{};
+            a.n = P.n;
+            for (int i = 0; i < P.n; ++i) {
+                a.box[i] = P.box[i];
+                a.amp[i] = list[i].amp;
+                a.wave[i] = list[i].wave;
+                a.nwave[i] = list[i].nwave;
+            }
+            a.prev = list[0].prev;
+            a.minv = list[0].minv;
+            a.minv_s = e ? eps_s : mu_s;
+            a.sig_prev = list[0].sig_prev;
+            a.dt = dt;
+            a.cutoff = cutoff;
+            a.clk = clk;
+            a.nsteps = len;
+            a.cells = P.cells;
+            a.n0 = g.n[0]; a.n1 = g.n[1]; a.n2 = g.n[2];
+            a.sy = g.sy();
+            a.sz = g.sz();
+            a.dw = P.buf;
+            const int64_t total = (int64_t)len * P.cells;
+            hipLaunchKernelGGL(kh::source_dw_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+            HIPCHK(hipGetLastError());
+            pre_ready[e] = true;
+        }
+    }
+    void end_pre() { pre_ready[0] = pre_ready[1] = false; }
+
     void launch_half(int e) {
         const bool electric = e == 1;
         // complex fields without conductivity: one PML launch updates both parts
@@ -1071,12 +1184,11 @@ struct Ctx final : CtxBase {
         std::vector<kh::FixSrc<T>> list;
         if (mode == KH_MODE_FAST && (active || last_active)) list = fix_sources(electric);
         pml_srcs.clear();
-        if (src_in_pml && !list.empty() && plan[e].pblocks > 0 && !plan[e].inner &&
-            list.size() <= (size_t)kh::MAX_SRC) {
-            int64_t rows = 0;
-            for (const auto &o : list)
-                rows += (int64_t)o.d[1] * o.d[2] * ((o.d[0] + 127) / 128 + 1);  // a wave: 64 lanes x 2 cells
-            if (rows <= 1024) pml_srcs.swap(list);
+        if (pre_ready[e]) {
+            list.clear();  // the PML launch adds the precomputed corrections
+        } else if (src_in_pml && !list.empty() && plan[e].pblocks > 0 && !plan[e].inner &&
+                   list.size() <= (size_t)kh::MAX_SRC && pml_sources_small(list)) {
+            pml_srcs.swap(list);
         }
         for (int part = 0; part < nparts; ++part) launch_half_part(e, part, fuse);
         pml_srcs.clear();
@@ -1297,6 +1409,14 @@ struct Ctx final : CtxBase {
             if (part == 0) {
                 q.nsrc = (int)pml_srcs.size();
                 for (size_t k = 0; k < pml_srcs.size(); ++k) q.src[k] = pml_srcs[k];
+                if (pre_ready[e]) {
+                    const PrePlan &PP = pre[e];
+                    q.pre_n = PP.n;
+                    q.pre_comp = PP.comp;
+                    for (int k = 0; k < PP.n; ++k) q.pre[k] = PP.box[k];
+                    q.pre_dw = PP.buf;
+                    q.pre_cells = PP.cells;
+                }
             }
             q.clk = clk;
             q.koff = koff;
@@ -1318,6 +1438,13 @@ struct Ctx final : CtxBase {
                 else kfn = lut_on ? kh::pml_h_kernel<T, true, false, true, true>
                          : pervox ? kh::pml_h_kernel<T, true, false, false, true>
                                   : kh::pml_h_kernel<T, false, false, false, true>;
+            } else if (q.pre_n > 0) {
+                if (electric) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, true>
+                                  : pervox ? kh::pml_e_kernel<T, true, false, false, false, true>
+                                           : kh::pml_e_kernel<T, false, false, false, false, true>;
+                else kfn = lut_on ? kh::pml_h_kernel<T, true, false, true, false, true>
+                         : pervox ? kh::pml_h_kernel<T, true, false, false, false, true>
+                                  : kh::pml_h_kernel<T, false, false, false, false, true>;
             } else if (electric) {
                 if (lut_on) kfn = cond ? kh::pml_e_kernel<T, true, true, true> : kh::pml_e_kernel<T, true, false, true>;
                 else if (cond) kfn = pervox ? kh::pml_e_kernel<T, true, true> : kh::pml_e_kernel<T, false, true>;
@@ -1516,11 +1643,13 @@ struct Ctx final : CtxBase {
         capturing = true;
         cap_dfts = 0;
         try {
+            launch_pre(len);
             for (int i = 0; i < len; ++i) launch_step(i);
             launch_tick(len);
         } catch (const KhError &) {
             ok = false;
         }
+        end_pre();
         capturing = false;
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(st, &gr);
@@ -1601,6 +1730,7 @@ struct Ctx final : CtxBase {
         }
         stepped = true;
         sync_source_lists();
+        plan_pre();
         if (xfold && n > 0) xghost_stale = true;
         int64_t done = 0;
         while (done < n) {
@@ -1620,7 +1750,9 @@ struct Ctx final : CtxBase {
                 done += sg->len;
                 continue;
             }
+            launch_pre(1);
             launch_step(0);
+            end_pre();
             launch_tick(1);
             end_step();
             ++n_eager;
@@ -1638,6 +1770,7 @@ struct Ctx final : CtxBase {
             plan_graph_len();
         }
         sync_source_lists();
+        plan_pre();
         const int64_t ts0 = ts;
         const bool a0 = active, l0 = last_active;
         int64_t done = 0;
@@ -2321,11 +2454,13 @@ struct Ctx final : CtxBase {
                     c->restore_state(c->adj_final, *fin);
                 } catch (...) {
                 }
+                c->end_pre();
                 c->dfts.swap(*saved);
                 c->plan_graph_len();
                 c->drop_graph();
             }
         } back{this, &saved, &fin};
+        plan_pre();
 
         kh::BoxCopy<T> bc{};
         unsigned bmax = 1;
@@ -2347,10 +2482,15 @@ struct Ctx final : CtxBase {
         // for every interval of that length (the device clock drives every
         // time-dependent quantity, the snapshots go to the same slots).
         auto interval = [&](int64_t len, bool captured) {
+            if (captured) launch_pre((int)len);
             for (int64_t i = 0; i < len; ++i) {
-                if (!captured) begin_step();
+                if (!captured) {
+                    begin_step();
+                    launch_pre(1);
+                }
                 launch_step(captured ? (int)i : 0);
                 if (!captured) {
+                    end_pre();
                     launch_tick(1);
                     end_step();
                 }
@@ -2361,7 +2501,10 @@ struct Ctx final : CtxBase {
                     HIPCHK(hipGetLastError());
                 }
             }
-            if (captured) launch_tick(len);
+            if (captured) {
+                end_pre();
+                launch_tick(len);
+            }
             for (int64_t i = len - 1; i >= 0; --i) {
                 launch_adj_seed(mons, off, true, i - len);
                 launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
@@ -2394,6 +2537,7 @@ struct Ctx final : CtxBase {
             } catch (const KhError &) {
                 ok = false;
             }
+            end_pre();
             hipGraph_t gr = nullptr;
             const hipError_t e = hipStreamEndCapture(st, &gr);
             if (!ok || e != hipSuccess || !gr) {
