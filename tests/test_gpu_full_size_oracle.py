@@ -8,10 +8,13 @@ device's EXACT mode):
   with FP64 DFT accumulators: within 1e-4 of each spectrum's peak (the
   precision form's bar); FP64 FAST: within 1e-9.
 * strip waveguide, 300 steps, FP32: fields and DFT accumulators against a
-  live FP32 oracle run, rel-L2 within 1e-5 (fields) and 1e-4 (DFTs).
+  live FP32 oracle run, rel-L2 within 1e-5 (fields) and 1e-4 (DFTs); the
+  device's EXACT mode equal to the oracle value for value (DFTs within
+  1e-13).
 * metasurface (256 x 256 x 384, Bloch x/y, 16-cell PML z, complex FP32, 2
   Lorentz poles + chi3 cylinders, 100-frequency flux plane), 80 steps:
-  fields and DFTs against a live FP32 oracle run at the same tolerances.  In
+  fields and DFTs against a live FP32 oracle run at the same tolerances, and
+  EXACT equal to it value for value.  In
   80 steps the pulse has not reached the cylinders; their ADE and chi3
   updates are compared with the oracle on the 2 x 2 lattice
   (test_gpu_metasurface.py).
@@ -70,15 +73,31 @@ def test_strip_waveguide_flux_against_fp64_oracle_fixture():
         assert err <= bar, (precision, err)
 
 
+def _exact_equals_oracle(x, o, tag):
+    """The device's EXACT mode against the oracle at full size: every field
+    value equal, DFT accumulators within 1e-13 (relative L2)."""
+    for C in COMPS:
+        assert np.array_equal(x.read_field(C()), o.read_field(C())), (tag, C.__name__)
+    for i in range(len(o.spec.dfts)):
+        a, b = x.engine.read_dft(i), o.engine.read_dft(i)
+        nb = float(np.linalg.norm(b))
+        assert (not np.any(a)) if nb == 0 else float(np.linalg.norm(a - b) / nb) <= 1e-13, (tag, i)
+    _note(f"[full-size] {tag}: device EXACT equals the oracle value for value")
+
+
 def test_strip_waveguide_fp32_fields_against_live_oracle():
     steps = 300
     g = strip_waveguide(precision="f32", mode="fast")
+    x = strip_waveguide(precision="f32", mode="exact")
     o = strip_waveguide(precision="f32", mode="exact", engine=OracleEngine)
-    _note(f"[full-size] strip waveguide FP32 {steps} steps: device FAST and CPU oracle")
+    _note(f"[full-size] strip waveguide FP32 {steps} steps: device FAST and EXACT, CPU oracle")
     g.step(steps)
+    x.step(steps)
     t0 = time.perf_counter()
     o.step(steps)
     _note(f"[full-size] oracle {steps} steps in {time.perf_counter() - t0:.1f} s")
+    _exact_equals_oracle(x, o, "strip waveguide")
+    x.engine.close()
     fa = [g.read_field(C()) for C in COMPS]
     fb = [o.read_field(C()) for C in COMPS]
     err = _rel(fa, fb)
@@ -102,12 +121,17 @@ def test_strip_waveguide_fp32_fields_against_live_oracle():
 def test_metasurface_full_size_against_live_oracle():
     steps = 80
     g = metasurface(mode="fast")
+    x = metasurface(mode="exact")
     o = metasurface(mode="exact", engine=OracleEngine)
-    _note(f"[full-size] metasurface {g.Nx}x{g.Ny}x{g.Nz} complex FP32 {steps} steps: device FAST and CPU oracle")
+    _note(f"[full-size] metasurface {g.Nx}x{g.Ny}x{g.Nz} complex FP32 {steps} steps: device FAST and EXACT, "
+          "CPU oracle")
     g.step(steps)
+    x.step(steps)
     t0 = time.perf_counter()
     o.step(steps)
     _note(f"[full-size] oracle {steps} steps in {time.perf_counter() - t0:.1f} s")
+    _exact_equals_oracle(x, o, "metasurface")
+    x.engine.close()
     fa = [g.read_field(C()) for C in COMPS]
     fb = [o.read_field(C()) for C in COMPS]
     err = _rel(fa, fb)
