@@ -1173,14 +1173,18 @@ struct Ctx final : CtxBase {
         // FAST mode sources (real part only: S += Re(f A),
         // src/Sources/Sources.jl:346-357) while sources are or were on the
         // previous step; the switch is permanent and the kernels gate
-        // themselves by the device timestep.  Small sources (points, lines:
-        // all of them together in at most 1024 wave-wide row segments of the
-        // PML launch) are applied by that launch as its lanes finish, in
-        // registration order; a lane in a source box waits for the waveform
-        // tables after its stores, which costs less than a launch only while
-        // few waves do it (a 192 x 160 plane made the E update 14% slower
-        // than a separate launch).  Otherwise source_fix_kernel runs after
-        // the update.
+        // themselves by the device timestep.  Three forms, the same
+        // operations on the same values:
+        //  * corrections precomputed for the block (plan_pre: real fields,
+        //    one component, disjoint boxes, not small), added by the PML
+        //    launch before its stores;
+        //  * small sources (points, lines: all of them together in at most
+        //    1024 wave-wide row segments of the PML launch) applied by that
+        //    launch as its lanes finish, in registration order; a lane in a
+        //    source box waits for the waveform tables after its stores, which
+        //    costs less than a launch only while few waves do it (a 192 x 160
+        //    plane made the E update 14% slower than a separate launch);
+        //  * otherwise source_fix_kernel after the update.
         std::vector<kh::FixSrc<T>> list;
         if (mode == KH_MODE_FAST && (active || last_active)) list = fix_sources(electric);
         pml_srcs.clear();
