@@ -106,6 +106,67 @@ def config1_leg(steps: int = 500):
             "reference_gh200": 640.1, "reference_source": "benchmark/dipole.yml:380 (Khronos.jl, FP64, 80^3)"}
 
 
+def dipole_bench(args):
+    """The 80^3 FP64 dipole (vacuum, 10-cell PML, Ez Gaussian dipole,
+    5-frequency DFT plane) with its kernels: the timed window starts after
+    ``--warmup`` steps (10 in run_benchmark, src/Simulation.jl:492-527) and
+    runs ``--steps`` steps (490: to step 500, the reference's run, sources on
+    throughout); then 100 eager steps give the per-kernel times.  The roofline
+    entry is the E update against its algorithmic bytes."""
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    from khronos_hip.workloads import algorithmic_bytes, dipole_80
+    sim = dipole_80(precision="f64", mode="fast")
+    sim.prepare()
+    eng = sim.engine
+    eng.set_graph(args.graph_steps)
+    cells = sim.Nx * sim.Ny * sim.Nz
+    ab = algorithmic_bytes(sim.spec)
+    sim.step(args.warmup)
+    eng.prepare_graphs(args.steps)
+    eng.sync()
+    eng.launch_counts()
+    t_start = sim.timestep
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    counts = eng.launch_counts()
+    k_steps = 100
+    eng.set_timing(k_steps * 8 + 64)
+    eng.get_timing()
+    sim.step(k_steps)
+    eng.sync()
+    tm = eng.get_timing()
+    eng.set_timing(0)
+    avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in tm.items()}
+    e_ms = avg["e_full"]
+    b_alg = (ab["H"] + ab["E"]) / cells
+    line = {
+        "metric": "Mcell-updates/s (FP64 3D Yee + CPML time step, 80^3 dipole)",
+        "value": round(cells * args.steps / wall / 1e6, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "dipole 80^3 FP64, 10-cell PML, Ez Gaussian dipole (fcen 1, fwidth 0.2), "
+                               "5-frequency DFT plane", "cells": cells, "graph_steps": args.graph_steps,
+                   "window": {"timesteps": [t_start, t_start + args.steps - 1], **counts},
+                   "B_alg_bytes_per_cell_update": round(b_alg, 2),
+                   "kernel_ms": {k: round(v, 5) for k, v in avg.items()},
+                   "hbm_fraction_whole_step": round(cells * args.steps / wall * b_alg / 1e9 / HBM_PEAK_GBS, 4),
+                   "reference_gh200": 640.1, "reference_source": "benchmark/dipole.yml:380 (Khronos.jl, FP64, 80^3)"},
+        "roofline": {"bound": "hbm", "achieved": round(ab["E"] / (e_ms * 1e-3) / 1e9, 1) if e_ms > 0 else 0.0,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ab["E"] / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if e_ms > 0 else 0.0,
+                     "traffic": None, "kernel": "pml_e_kernel<double,false,false,false>",
+                     "alg_bytes_per_launch": ab["E"], "kernel_ms": round(e_ms, 5)},
+    }
+    eng.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def serving_leg(n_requests: int, rate: float, seed: int = 0):
     """A short Poisson stream of the serving workload, continuous batching vs
     one at a time, with the ideal-overlap bound: every request's algorithmic
@@ -487,6 +548,8 @@ def main(argv=None):
     ap.add_argument("--near2far", action="store_true", help="near-to-far projection instead of the headline benchmark")
     ap.add_argument("--metasurface", action="store_true",
                     help="the periodic dispersive metasurface (complex fields) instead of the headline benchmark")
+    ap.add_argument("--dipole", action="store_true",
+                    help="the 80^3 FP64 dipole with its kernel times (pass --steps 490 --warmup 10 for its run)")
     ap.add_argument("--adjoint", action="store_true",
                     help="time-reversed adjoint gradient of the 2D bend instead of the headline benchmark")
     ap.add_argument("--requests", type=int, default=48)
@@ -494,7 +557,7 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--no-legs", action="store_true",
-                    help="headline only: skip the config-1 dipole and the short serving stream")
+                    help="headline only: skip the legs (80^3 dipole, serving stream, metasurface, adjoint)")
     ap.add_argument("--leg-requests", type=int, default=24, help="requests of the serving leg")
     ap.add_argument("--leg-rate", type=float, default=8.0, help="Poisson arrival rate of the serving leg")
     args = ap.parse_args(argv)
@@ -504,6 +567,8 @@ def main(argv=None):
         return near2far_bench(args)
     if args.adjoint:
         return adjoint_bend_bench(args)
+    if args.dipole:
+        return dipole_bench(args)
     if args.metasurface:
         return metasurface_bench(args)
     if args.calibrate_copy:
