@@ -130,26 +130,27 @@ def test_complex_field_write_read_roundtrip():
         assert np.array_equal(back[box], F[box])
 
 
-@pytest.mark.parametrize("cell_x", [1.2, 1.3])
+@pytest.mark.parametrize("cell_x,cell_y", [(1.2, 1.0), (1.3, 1.0), (1.2, 1.1)])
 @pytest.mark.parametrize("bloch", [True, False])
-def test_folded_x_wrap_equals_ghost_fill(cell_x, bloch):
+def test_folded_x_wrap_equals_ghost_fill(cell_x, cell_y, bloch):
     """FAST with a periodic x axis: the updates read the x-neighbour across
     the boundary from the opposite interior plane (times the Bloch phase)
     and the x ghost planes are filled only before a field is read; the DFT
     samples of monitors that span the ghost plane are formed from the
     opposite plane.  A context whose fields were written keeps filling the
     ghosts every half step.  Both give the same fields (ghosts included),
-    polarization and DFTs value for value, for an even and an odd Nx, with a
+    polarization and DFTs value for value, for even and odd Nx and Ny, with a
     source on the wrapped boundary planes."""
     from khronos_hip.grid import component_code
 
     def mk():
         s = _bloch_sim("f32", "fast", bloch=bloch)
-        s2 = kh.Simulation(cell_size=[cell_x, 1.0, 1.8], resolution=10, sources=s.sources, geometry=s.geometry,
-                           monitors=[kh.FluxMonitor(center=[0.0, 0.0, 0.55], size=[cell_x, 1.0, 0.0],
+        s2 = kh.Simulation(cell_size=[cell_x, cell_y, 1.8], resolution=10, sources=s.sources, geometry=s.geometry,
+                           monitors=[kh.FluxMonitor(center=[0.0, 0.0, 0.55], size=[cell_x, cell_y, 0.0],
                                                     frequencies=[0.8, 1.0, 1.2]),
                                      kh.DFTMonitor(component=kh.Ey(), center=[0.0, 0.0, 0.2],
-                                                   size=[cell_x, 1.0, 0.0], frequencies=[0.9, 1.1], decimation=2)],
+                                                   size=[cell_x, cell_y, 0.0], frequencies=[0.9, 1.1],
+                                                   decimation=2)],
                            boundaries=[[0.0, 0.0], [0.0, 0.0], [0.3, 0.3]], boundary_conditions=s.boundary_conditions,
                            precision="f32", mode="fast")
         s2.prepare()
@@ -159,7 +160,7 @@ def test_folded_x_wrap_equals_ghost_fill(cell_x, bloch):
     z = fill.read_field(kh.Ez())
     fill.engine.write_field(component_code(kh.Ez()), z)  # a field write keeps the per-step ghost fill
     _note(f"[periodic] folded x wrap vs ghost fill, bloch={bloch}, grid {fold.Nx}x{fold.Ny}x{fold.Nz}")
-    assert fold.Nx % 2 == (1 if cell_x == 1.3 else 0)
+    assert fold.Nx % 2 == (1 if cell_x == 1.3 else 0) and fold.Ny % 2 == (1 if cell_y == 1.1 else 0)
     done = 0
     for chunk in (23, 40, 17, 60):
         fold.step(chunk)
