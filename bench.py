@@ -167,6 +167,61 @@ def dipole_bench(args):
     return 0
 
 
+def concurrency_bench(args):
+    """Serving form, small grids: ``--batch`` same-shape requests of the
+    serving workload stepped concurrently (each context on its own stream,
+    all their graphs queued from one host thread) against one
+    context holding the same cells in one grid (n x n x batch*n: what one
+    launch batching the requests would stream) and against one request
+    alone, by algorithmic bytes moved per second (the PML share differs
+    between the shapes, so cell rates alone would not compare)."""
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    from khronos_hip import serving
+    from khronos_hip.workloads import algorithmic_bytes
+    K, steps, warm = args.batch, args.steps, 64
+
+    def ready(s):
+        s.prepare()
+        s.step(warm)
+        s.engine.prepare_graphs(steps)
+        s.engine.sync()
+        ab = algorithmic_bytes(s.spec)
+        return ab["H"] + ab["E"]
+
+    def timed(sims):
+        per_step = [ready(s) for s in sims]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in sims:  # queued asynchronously, each on its own stream
+            s.step(steps)
+        for s in sims:
+            s.engine.sync()
+        wall = time.perf_counter() - t0
+        cells = sum(s.Nx * s.Ny * s.Nz for s in sims)
+        for s in sims:
+            s.engine.close()
+        return {"GBps": round(sum(per_step) * steps / wall / 1e9, 1),
+                "Mcell_updates_per_s": round(cells * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4)}
+
+    rows = []
+    for n in args.grids:
+        mk = lambda i, shape: serving.serving_request(i, np.random.default_rng([n, i]), grids=[shape]).make()
+        alone = timed([mk(0, (n, n, n))])
+        conc = timed([mk(i, (n, n, n)) for i in range(K)])
+        one = timed([mk(0, (n, n, K * n))])
+        rows.append({"grid": f"{n}^3", "alone": alone, f"{K}_concurrent": conc, f"one_grid_{n}x{n}x{K * n}": one,
+                     "concurrent_over_one_grid": round(conc["GBps"] / one["GBps"], 3)})
+        sys.stderr.write(f"{n}^3: alone {alone['GBps']} GB/s, {K} concurrent {conc['GBps']} GB/s, "
+                         f"one {n}x{n}x{K * n} grid {one['GBps']} GB/s\n")
+    print(json.dumps({"metric": "algorithmic GB/s of same-shape serving requests: concurrent contexts vs one grid",
+                      "unit": "GB/s", "n_gpus": 1, "steps": steps, "batch": K, "dtype": "f32", "data": "synthetic",
+                      "rows": rows}), flush=True)
+    return 0
+
+
 def serving_leg(n_requests: int, rate: float, seed: int = 0):
     """A short Poisson stream of the serving workload, continuous batching vs
     one at a time, with the ideal-overlap bound: every request's algorithmic
@@ -550,6 +605,10 @@ def main(argv=None):
                     help="the periodic dispersive metasurface (complex fields) instead of the headline benchmark")
     ap.add_argument("--dipole", action="store_true",
                     help="the 80^3 FP64 dipole with its kernel times (pass --steps 490 --warmup 10 for its run)")
+    ap.add_argument("--concurrency", action="store_true",
+                    help="serving: same-shape requests concurrently vs one grid of the same cells")
+    ap.add_argument("--batch", type=int, default=6, help="requests per group (--concurrency)")
+    ap.add_argument("--grids", type=int, nargs="+", default=[64, 96, 128], help="cube edges (--concurrency)")
     ap.add_argument("--adjoint", action="store_true",
                     help="time-reversed adjoint gradient of the 2D bend instead of the headline benchmark")
     ap.add_argument("--requests", type=int, default=48)
@@ -569,6 +628,8 @@ def main(argv=None):
         return adjoint_bend_bench(args)
     if args.dipole:
         return dipole_bench(args)
+    if args.concurrency:
+        return concurrency_bench(args)
     if args.metasurface:
         return metasurface_bench(args)
     if args.calibrate_copy:
