@@ -134,6 +134,45 @@ def test_captured_sweep_equals_eager_sweep(mode):
         assert np.any(g_graph != 0)
 
 
+def plane3d_problem(mode="fast", until=12.0):
+    """3D, PML on every face: an Ez sheet source of 38 x 28 cells (too large
+    for the end-of-lane form, so its corrections are computed per block of
+    steps), a 5x4 design block and an Ez objective plane."""
+    pml = 0.3
+    src = [kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15), component=kh.Ez(),
+                            center=[-1.2, 0.0, 0.0], size=[0.0, 2.8, 2.8])]
+    sim = kh.Simulation(cell_size=[3.2, 3.0, 3.0], resolution=10, sources=src,
+                        boundaries=[[pml, pml]] * 3, precision="f64", mode=mode)
+    dr = A.DesignRegion(volume=Volume([0, 0, 0], [1.0, 1.2, 0.5]), design_parameters=np.full(20, 0.5),
+                        grid_size=(5, 4), ε_min=1.0, ε_max=4.0)
+    objs = [A.FourierFieldsObjective(volume=Volume([0.9, 0, 0], [0, 1.2, 0.5]), component=kh.Ez())]
+    f = lambda E1: torch.sum(torch.abs(E1) ** 2)
+    return A.OptimizationProblem(sim, [f], objs, [dr], [0.45, 0.55], until=until)
+
+
+def test_block_precomputed_sources_in_the_sweep(monkeypatch):
+    """A plane source whose FAST corrections are computed per block of steps
+    and added by the PML update: the forward run and the sweep (captured
+    intervals, and the eager ones of a partial last interval) give the same
+    objective and gradient bit for bit as with every source in a separate
+    launch (KH_SRC_LAUNCH=1, read when a context plans its first step)."""
+    rho = 0.3 + 0.4 * np.random.default_rng(8).random(20)
+    out = []
+    for env in (None, "1"):
+        if env:
+            monkeypatch.setenv("KH_SRC_LAUNCH", env)
+        opt = _time_reversed(plane3d_problem(), 70)
+        f, g = opt(rho)
+        n = opt._fwd_sim.engine.timestep()
+        out.append((f, g))
+        opt._close_forward()
+        monkeypatch.delenv("KH_SRC_LAUNCH", raising=False)
+    _note(f"[adjoint-tr] plane source, block-precomputed vs separate launches, {n} steps, intervals of 70")
+    assert n % 70 != 0
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.any(out[0][1] != 0)
+
+
 def test_forward_state_and_monitors_restored():
     opt = _time_reversed(slab2d_problem(engine=None, precision="f64", mode="fast"), 50)
     rho = 0.3 + 0.4 * np.random.default_rng(4).random(24)
