@@ -2501,6 +2501,39 @@ __global__ __launch_bounds__(256) void smooth_kernel(const RasterObj *objs, int 
 // shape, and the other way round.  Cells 1..N of every axis, as the forward
 // update; ghost entries of Kbar stay zero.
 template <typename T>
+struct AdjSeed {
+    const double *g;
+    T *lam;
+    const double *freqs;
+    int s[3], d[3];
+    int nf, dec;
+    int first;
+};
+
+// The seed of monitor m at grid cell (gx, gy, gz) of the step at ts, as
+// adjoint_seed_kernel adds it (false: the cell is not on the monitor or the
+// step is not sampled).
+template <typename T>
+__device__ __forceinline__ bool adj_seed_at(const AdjSeed<T> &m, int gx, int gy, int gz, int64_t ts, T dt,
+                                            int electric, double &v) {
+    if (m.dec > 1 && (ts % m.dec) != 0) return false;
+    const int ix = gx - m.s[0], iy = gy - m.s[1], iz = gz - m.s[2];
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= m.d[0] || iy >= m.d[1] || iz >= m.d[2]) return false;
+    const int64_t ncell = (int64_t)m.d[0] * m.d[1] * m.d[2];
+    const int64_t cell = (int64_t)ix + (int64_t)m.d[0] * ((int64_t)iy + (int64_t)m.d[1] * iz);
+    const double theta = dft_theta(ts, dt, electric);
+    const double dts = (double)dt;
+    v = 0.0;
+    const double *gp = m.g + 2 * cell;
+    for (int k = 0; k < m.nf; ++k, gp += 2 * ncell) {
+        double sn, cs;
+        sincos(m.freqs[k] * theta, &sn, &cs);
+        v += gp[0] * (dts * cs) + gp[1] * (dts * sn);
+    }
+    return true;
+}
+
+template <typename T>
 struct AdjArgs {
     T *lA[3], *lT[3], *lU[3], *lW[3];
     T *Kout[3];
@@ -2516,6 +2549,13 @@ struct AdjArgs {
     int glo[3][3], gn[3][3];
     const T *fwd[3];
     double *grad[3];
+    // the objective's seeds of this step (adjoint_seed_kernel's, added to
+    // lambda_A of their component in list order before the transpose reads
+    // it), ts = device clock + koff
+    int nseed;
+    AdjSeed<T> seed[MAX_DFT];
+    const int64_t *clk;
+    int64_t koff;
 };
 
 template <typename T, bool ELEC>
@@ -2549,7 +2589,16 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
     for (int a = 0; a < 3; ++a) {
         const T so = sg[a], sn = sg[(a + 1) % 3], sp = sg[(a + 2) % 3];
         const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
-        const T ap = p.lA[a][idx] + add[a];
+        T la = p.lA[a][idx];
+        if (p.nseed > 0) {
+            const int64_t ts = *p.clk + p.koff;
+            for (int q = 0; q < p.nseed; ++q) {
+                double v;
+                if (p.seed[q].lam == p.lA[a] && adj_seed_at(p.seed[q], gx, gy, gz, ts, p.dt, ELEC ? 1 : 0, v))
+                    la = (T)((double)la + v);
+            }
+        }
+        const T ap = la + add[a];
         T wh;
         if (so == T(0)) {
             wh = ap;
@@ -2596,15 +2645,6 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
 // ts = device timestep + koff (the reverse walk leaves the clock at the end
 // of the interval it walks, so one launch sequence serves every interval);
 // each monitor skips the steps its decimation does not sample.
-template <typename T>
-struct AdjSeed {
-    const double *g;
-    T *lam;
-    const double *freqs;
-    int s[3], d[3];
-    int nf, dec;
-    int first;
-};
 
 template <typename T>
 struct AdjSeedArgs {

@@ -2335,8 +2335,29 @@ struct Ctx final : CtxBase {
         }
     }
 
-    void launch_adj_half(int e, const int (*glo)[3], const int (*gn)[3], const size_t *goff, const T *seg) {
+    // The transposed half step of group e (1: E); with `mons`, the group's
+    // objective seeds of the step at device clock + koff are added by the
+    // same launch (false: too many monitors for one launch, seed separately).
+    bool launch_adj_half(int e, const int (*glo)[3], const int (*gn)[3], const size_t *goff, const T *seg,
+                         const std::vector<const Dft *> *mons = nullptr, const std::vector<size_t> *off = nullptr,
+                         int64_t koff = 0) {
         kh::AdjArgs<T> q{};
+        if (mons) {
+            for (size_t i = 0; i < mons->size(); ++i) {
+                const Dft &m = *(*mons)[i];
+                if ((m.comp < 3) != (e == 1)) continue;
+                if (q.nseed == kh::MAX_DFT) return false;
+                kh::AdjSeed<T> &x = q.seed[q.nseed++];
+                x.g = adj_cot + (*off)[i];
+                x.lam = adjA[e][m.comp % 3];
+                x.freqs = m.freqs;
+                for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
+                x.nf = m.nf;
+                x.dec = m.dec;
+            }
+            q.clk = clk;
+            q.koff = koff;
+        }
         for (int a = 0; a < 3; ++a) {
             q.lA[a] = adjA[e][a];
             q.lT[a] = adjT[e][a];
@@ -2364,6 +2385,7 @@ struct Ctx final : CtxBase {
         if (e) hipLaunchKernelGGL((kh::adjoint_half_kernel<T, true>), dim3(blocks), dim3(256), 0, st, q);
         else hipLaunchKernelGGL((kh::adjoint_half_kernel<T, false>), dim3(blocks), dim3(256), 0, st, q);
         HIPCHK(hipGetLastError());
+        return true;
     }
 
     // dJ/d(eps^-1) of a real objective J of DFT monitors, by the discrete
@@ -2510,10 +2532,16 @@ struct Ctx final : CtxBase {
                 launch_tick(len);
             }
             for (int64_t i = len - 1; i >= 0; --i) {
-                launch_adj_seed(mons, off, true, i - len);
-                launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
-                launch_adj_seed(mons, off, false, i - len);
-                launch_adj_half(0, glo, gn, goff, nullptr);
+                // the seeds go with the transposed half step (or before it,
+                // as their own launches, past MAX_DFT monitors)
+                if (!launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot, &mons, &off, i - len)) {
+                    launch_adj_seed(mons, off, true, i - len);
+                    launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
+                }
+                if (!launch_adj_half(0, glo, gn, goff, nullptr, &mons, &off, i - len)) {
+                    launch_adj_seed(mons, off, false, i - len);
+                    launch_adj_half(0, glo, gn, goff, nullptr);
+                }
             }
         };
         // graphs of a full interval, with and without the FAST source
