@@ -422,7 +422,9 @@ int kh_set_checkpoints(kh_ctx *ctx, int32_t every);
  * those of the forward run again on return.  Unless graphs are off
  * (kh_set_graph 0) or kernel timing is on, each full interval of up to 1024
  * steps (re-run, snapshots, walk back) replays one graph captured for the
- * call; the result is the same bit for bit.
+ * call; the result is the same bit for bit.  In FAST mode the re-run's E
+ * update writes the design-box snapshots itself unless an E source writes a
+ * box cell of its component (KH_SNAP_LAUNCH=1: always a separate copy).
  * Replaces: the pullback of simulate (src/Adjoint/ChainRulesIntegration.jl:
  * 38-60) with calculate_gradient! (src/Adjoint/GradientKernel.jl:35-109),
  * as dJ/d(eps^-1) per voxel before the design-grid contraction. */

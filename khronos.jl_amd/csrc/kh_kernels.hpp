@@ -687,6 +687,12 @@ struct PmlArgs {
     PreBox pre[MAX_SRC];
     const T *pre_dw;
     int64_t pre_cells;
+    // SNAP kernels (E update of the adjoint's re-run): each component's
+    // stored value (W where the own-axis sigma is non-zero, else A) on the
+    // cells of box snap_lo[a] .. + snap_n[a] also goes to snap[a], dense x
+    // fastest, as box_copy_kernel would copy it after the step (null: none)
+    T *snap[3];
+    int snap_lo[3][3], snap_n[3][3];
 };
 
 // the ghost value wrap_kernel would write from interior element i
@@ -819,7 +825,7 @@ template <typename T>
 __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T> &f, int a, size_t idx,
                                            const PmlComp<T> &c, T K0, T K1, T so0, T so1, T sn0, T sn1, T sp0,
                                            T sp1, bool v0, bool v1, T dw0 = T(0), T dw1 = T(0), bool h0 = false,
-                                           bool h1 = false) {
+                                           bool h1 = false, typename V2<T>::t *kept = nullptr) {
     using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
     if (c.nc) {
@@ -889,6 +895,24 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T>
     }
     if (c.nw) sto<T, true>(f.W[a], idx, wn);
     sto<T, true>(f.A[a], idx, an);
+    if (kept) {
+        kept->x = so0 != T(0) ? wn.x : an.x;
+        kept->y = so1 != T(0) ? wn.y : an.y;
+    }
+}
+
+// The design-box snapshot of component a at the lane's pair (SNAP kernels).
+template <typename T>
+__device__ __forceinline__ void pml_snap(const PmlArgs<T> &p, int a, int x, int y, int z, bool v0, bool v1,
+                                         typename V2<T>::t v) {
+    T *o = p.snap[a];
+    if (!o) return;
+    const int *lo = p.snap_lo[a], *n = p.snap_n[a];
+    if (y < lo[1] || y >= lo[1] + n[1] || z < lo[2] || z >= lo[2] + n[2]) return;
+    const int64_t r = (int64_t)n[0] * ((int64_t)(y - lo[1]) + (int64_t)n[1] * (z - lo[2]));
+    const int i0 = x - lo[0];
+    if (v0 && i0 >= 0 && i0 < n[0]) o[r + i0] = v.x;
+    if (v1 && i0 + 1 >= 0 && i0 + 1 < n[0]) o[r + i0 + 1] = v.y;
 }
 
 // Loads of one part's state for the three components (the material values
@@ -959,7 +983,7 @@ __device__ __forceinline__ void pml_h_part(const PmlArgs<T> &p, const PmlPart<T>
 }
 
 // One part of the E update at the lane's pair.
-template <typename T, bool COND>
+template <typename T, bool COND, bool SNAP = false>
 __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
                                            int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3],
                                            T dw0 = T(0), T dw1 = T(0), bool h0 = false, bool h1 = false) {
@@ -984,18 +1008,22 @@ __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T>
     const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
     const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
     const int pc = p.pre_comp;
+    typename V2<T>::t kept[3];
     pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1, dw0, dw1, h0 && pc == 0,
-               h1 && pc == 0);
+               h1 && pc == 0, SNAP ? &kept[0] : nullptr);
     pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1, dw0, dw1, h0 && pc == 1,
-               h1 && pc == 1);
+               h1 && pc == 1, SNAP ? &kept[1] : nullptr);
     pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1, dw0, dw1, h0 && pc == 2,
-               h1 && pc == 2);
+               h1 && pc == 2, SNAP ? &kept[2] : nullptr);
+    if (SNAP)
+        for (int a = 0; a < 3; ++a) pml_snap(p, a, x, y, z, v0, v1, kept[a]);
 }
 
 // TWO: complex fields, p.f[0] the real and p.f[1] the imaginary part (no
 // conductivity: its C stage is per part).  PRE: the launch adds precomputed
 // source corrections (real fields without conductivity; a separate variant so
-// that the others keep their register count).
+// that the others keep their register count).  SNAP (E update, real fields
+// without conductivity): the launch also writes the design-box snapshot.
 template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false>
 __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
@@ -1017,10 +1045,12 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
 
-template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false>
+template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false,
+          bool SNAP = false>
 __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
     static_assert(!(PRE && (COND || TWO)), "precomputed sources: real fields without conductivity");
+    static_assert(!(SNAP && (COND || TWO)), "snapshots: real fields without conductivity");
     int x, y, z;
     bool v0, v1;
     if (!pml_lane(p, x, y, z, v0, v1)) return;
@@ -1033,7 +1063,7 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
     pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
-    pml_e_part<T, COND>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
+    pml_e_part<T, COND, SNAP>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
     if (TWO) pml_e_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }

@@ -1165,6 +1165,33 @@ struct Ctx final : CtxBase {
     }
     void end_pre() { pre_ready[0] = pre_ready[1] = false; }
 
+    // design-box snapshot written by the E update of the steps launched
+    // while set (the adjoint's re-run; null: none)
+    T *snap_out[3] = {nullptr, nullptr, nullptr};
+    int snap_lo[3][3] = {}, snap_n[3][3] = {};
+
+    // the E update can write the snapshot of these boxes itself: FAST mode
+    // with the whole-grid PML launch, real fields, no conductivity, and no E
+    // source writing a box cell of its component (sources applied after the
+    // update's stores would change the value after the snapshot was taken);
+    // KH_SNAP_LAUNCH=1 keeps the separate box_copy_kernel
+    bool snap_fusable(const int lo[3][3], const int n[3][3]) const {
+        if (env_int("KH_SNAP_LAUNCH", 0) != 0) return false;
+        if (mode != KH_MODE_FAST || nparts != 1 || plan[1].pblocks == 0 || plan[1].inner || plan[1].cblocks)
+            return false;
+        for (int a = 0; a < 3; ++a)
+            if (fp[0].CD[a]) return false;
+        for (auto &o : srcs) {
+            if (o.comp >= 3) continue;
+            const int a = o.comp;
+            bool meet = n[a][0] > 0 && n[a][1] > 0 && n[a][2] > 0;
+            for (int k = 0; k < 3 && meet; ++k)
+                meet = o.s[k] < lo[a][k] + n[a][k] && lo[a][k] < o.s[k] + o.d[k];
+            if (meet) return false;
+        }
+        return true;
+    }
+
     void launch_half(int e) {
         const bool electric = e == 1;
         // complex fields without conductivity: one PML launch updates both parts
@@ -1422,6 +1449,15 @@ struct Ctx final : CtxBase {
                     q.pre_cells = PP.cells;
                 }
             }
+            const bool snap = electric && part == 0 && !fuse && !cond && (snap_out[0] || snap_out[1] || snap_out[2]);
+            if (snap)
+                for (int a = 0; a < 3; ++a) {
+                    q.snap[a] = snap_out[a];
+                    for (int k = 0; k < 3; ++k) {
+                        q.snap_lo[a][k] = snap_lo[a][k];
+                        q.snap_n[a][k] = snap_n[a][k];
+                    }
+                }
             q.clk = clk;
             q.koff = koff;
             q.cutoff = cutoff;
@@ -1435,7 +1471,14 @@ struct Ctx final : CtxBase {
             // template flags: per-voxel material, conductivity C stage,
             // material bytes + table, both parts of a complex field
             void (*kfn)(kh::PmlArgs<T>);
-            if (fuse) {
+            if (snap) {
+                if (q.pre_n > 0) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, true, true>
+                                     : pervox ? kh::pml_e_kernel<T, true, false, false, false, true, true>
+                                              : kh::pml_e_kernel<T, false, false, false, false, true, true>;
+                else kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, false, true>
+                         : pervox ? kh::pml_e_kernel<T, true, false, false, false, false, true>
+                                  : kh::pml_e_kernel<T, false, false, false, false, false, true>;
+            } else if (fuse) {
                 if (electric) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, true>
                                   : pervox ? kh::pml_e_kernel<T, true, false, false, true>
                                            : kh::pml_e_kernel<T, false, false, false, true>;
@@ -2507,20 +2550,37 @@ struct Ctx final : CtxBase {
         // tick per step.  Captured: the launches of a whole interval, replayed
         // for every interval of that length (the device clock drives every
         // time-dependent quantity, the snapshots go to the same slots).
+        // the re-run's E update writes the snapshots itself where it can
+        const bool fused_snap = gtot && snap_fusable(glo, gn);
+        struct SnapOff {
+            Ctx *c;
+            ~SnapOff() {
+                for (int a = 0; a < 3; ++a) c->snap_out[a] = nullptr;
+            }
+        } snap_off{this};
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) {
+                snap_lo[a][k] = glo[a][k];
+                snap_n[a][k] = gn[a][k];
+            }
         auto interval = [&](int64_t len, bool captured) {
             if (captured) launch_pre((int)len);
             for (int64_t i = 0; i < len; ++i) {
+                if (fused_snap)
+                    for (int a = 0; a < 3; ++a)
+                        snap_out[a] = gn[a][0] > 0 ? adj_seg + (size_t)i * gtot + goff[a] : nullptr;
                 if (!captured) {
                     begin_step();
                     launch_pre(1);
                 }
                 launch_step(captured ? (int)i : 0);
+                for (int a = 0; a < 3; ++a) snap_out[a] = nullptr;
                 if (!captured) {
                     end_pre();
                     launch_tick(1);
                     end_step();
                 }
-                if (gtot) {
+                if (gtot && !fused_snap) {
                     for (int a = 0; a < 3; ++a)
                         bc.out[a] = gn[a][0] > 0 ? adj_seg + (size_t)i * gtot + goff[a] : nullptr;
                     hipLaunchKernelGGL((kh::box_copy_kernel<T>), dim3(bmax, 3), dim3(256), 0, st, bc);

@@ -394,3 +394,47 @@ def test_design_region_inside_the_x_pml(mode):
     _note(f"[adjoint-tr] 2D PML design {mode} (adjoint, fd, rel): {res}")
     assert np.any(g != 0)
     assert max(r[2] for r in res) < 1e-6, res
+
+
+def _box3d_source_in_design(precision="f32"):
+    """box3d_problem with its Ez sheet source moved inside the design block:
+    the source writes design-box cells of Ez after the E update's stores, so
+    the re-run keeps the separate snapshot copy."""
+    opt = box3d_problem(precision=precision, until=20.0)
+    opt.sim.sources[0] = kh.UniformSource(time_profile=kh.GaussianPulseSource(fcen=0.5, fwidth=0.15),
+                                          component=kh.Ez(), center=[0.1, 0.0, 0.0], size=[0.0, 1.0, 0.4])
+    return opt
+
+
+@pytest.mark.parametrize("case", ["2d_x_pml_f64", "3d_f32", "3d_plane_source_f64", "3d_source_in_design_f32"])
+def test_snapshots_written_by_the_update_equal_separate_copies(case, monkeypatch):
+    """FAST mode: the re-run's E update writes the design-box snapshot from
+    its registers (W where the own-axis sigma is non-zero, else E), instead
+    of a box_copy_kernel launch after each step (KH_SNAP_LAUNCH=1, read per
+    gradient).  The same gradient bit for bit, captured and eager, with the
+    design region in the x PML, in 3D with PML on every axis, with
+    block-precomputed source corrections, and with a source inside the
+    design region (which keeps the separate copy)."""
+    make, n, every = {
+        "2d_x_pml_f64": (lambda: slab2d_pml_design_problem(mode="fast"), 24, 41),
+        "3d_f32": (lambda: box3d_problem(precision="f32", until=20.0), 20, 30),
+        "3d_plane_source_f64": (lambda: plane3d_problem(), 20, 70),
+        "3d_source_in_design_f32": (_box3d_source_in_design, 20, 30),
+    }[case]
+    opt = _time_reversed(make(), every)
+    rho = 0.3 + 0.4 * np.random.default_rng(23).random(n)
+    opt.update_design(rho)
+    opt.forward_run()
+    eng = opt._fwd_sim.engine
+    _note(f"[adjoint-tr] snapshots by the E update vs box copies, {case}, {eng.timestep()} steps, "
+          f"intervals of {every}")
+    g_fused = opt._time_reversed_gradient()
+    monkeypatch.setenv("KH_SNAP_LAUNCH", "1")
+    g_copy = opt._time_reversed_gradient()
+    eng.set_graph(0)
+    g_copy_eager = opt._time_reversed_gradient()
+    monkeypatch.delenv("KH_SNAP_LAUNCH")
+    g_fused_eager = opt._time_reversed_gradient()
+    assert np.any(g_fused != 0)
+    for g in (g_copy, g_copy_eager, g_fused_eager):
+        assert np.array_equal(g_fused, g)
