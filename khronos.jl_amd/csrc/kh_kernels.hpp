@@ -2615,32 +2615,26 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
     }
     const T sg[3] = {p.sig[0][gx], p.sig[1][gy], p.sig[2][gz]};
     const int gc[3] = {gx, gy, gz};
+    // Every load of the lane is issued before its first store (the memory
+    // counter is in order: a load issued after a store waits for the store,
+    // so per-component load/store rounds would cost a round trip each).  The
+    // arrays of different components and stages never overlap.
+    T m[3], la[3], lw[3], lt[3], lu[3];
+    int gbi[3];
+    double gr[3];
+    T fw[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const T so = sg[a], sn = sg[(a + 1) % 3], sp = sg[(a + 2) % 3];
-        const T m = p.minv[a] ? p.minv[a][idx] : p.minv_s;
-        T la = p.lA[a][idx];
-        if (p.nseed > 0) {
-            const int64_t ts = *p.clk + p.koff;
-            for (int q = 0; q < p.nseed; ++q) {
-                double v;
-                if (p.seed[q].lam == p.lA[a] && adj_seed_at(p.seed[q], gx, gy, gz, ts, p.dt, ELEC ? 1 : 0, v))
-                    la = (T)((double)la + v);
-            }
-        }
-        const T ap = la + add[a];
-        T wh;
-        if (so == T(0)) {
-            wh = ap;
-            p.lA[a][idx] = T(0);
-        } else {
-            wh = p.lW[a][idx] + (T(1) + so) * ap;
-            p.lW[a][idx] = -((T(1) - so) * ap);
-            p.lA[a][idx] = ap;
-        }
+        const T so = sg[a], sn = sg[(a + 1) % 3];
+        m[a] = p.minv[a] ? p.minv[a][idx] : p.minv_s;
+        la[a] = p.lA[a][idx];
+        lw[a] = so != T(0) ? p.lW[a][idx] : T(0);
+        lt[a] = p.lT[a][idx];
+        lu[a] = sn != T(0) ? p.lU[a][idx] : T(0);
+        gbi[a] = -1;
+        gr[a] = 0.0;
+        fw[a] = T(0);
         if constexpr (ELEC) {
-            // m enters as W' = m net: dJ/dm += What * net, with fwd = m net
-            // (A' where o = 0, W' where o != 0)
             if (p.grad[a]) {
                 int bi = 0, str = 1;
                 bool in = true;
@@ -2650,17 +2644,50 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
                     bi += o * str;
                     str *= p.gn[a][k];
                 }
-                if (in) p.grad[a][bi] += (double)wh * ((double)p.fwd[a][bi] / (double)m);
+                if (in) {
+                    gbi[a] = bi;
+                    gr[a] = p.grad[a][bi];
+                    fw[a] = p.fwd[a][bi];
+                }
             }
         }
-        const T th = p.lT[a][idx] + m * wh;
+    }
+    if (p.nseed > 0) {
+        const int64_t ts = *p.clk + p.koff;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < p.nseed; ++q) {
+                double v;
+                if (p.seed[q].lam == p.lA[a] && adj_seed_at(p.seed[q], gx, gy, gz, ts, p.dt, ELEC ? 1 : 0, v))
+                    la[a] = (T)((double)la[a] + v);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T so = sg[a], sn = sg[(a + 1) % 3], sp = sg[(a + 2) % 3];
+        const T ap = la[a] + add[a];
+        T wh;
+        if (so == T(0)) {
+            wh = ap;
+            p.lA[a][idx] = T(0);
+        } else {
+            wh = lw[a] + (T(1) + so) * ap;
+            p.lW[a][idx] = -((T(1) - so) * ap);
+            p.lA[a][idx] = ap;
+        }
+        if constexpr (ELEC) {
+            // m enters as W' = m net: dJ/dm += What * net, with fwd = m net
+            // (A' where o = 0, W' where o != 0)
+            if (gbi[a] >= 0) p.grad[a][gbi[a]] = gr[a] + (double)wh * ((double)fw[a] / (double)m[a]);
+        }
+        const T th = lt[a] + m[a] * wh;
         p.lT[a][idx] = ((T(1) - sp) / (T(1) + sp)) * th;
         const T dl = th / (T(1) + sp);
         T kb;
         if (sn == T(0)) {
             kb = dl;
         } else {
-            const T uh = p.lU[a][idx] + dl;
+            const T uh = lu[a] + dl;
             p.lU[a][idx] = ((T(1) - sn) / (T(1) + sn)) * uh - dl;
             kb = uh / (T(1) + sn);
         }
