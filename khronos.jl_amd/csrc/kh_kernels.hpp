@@ -2538,7 +2538,27 @@ struct AdjSeed {
     int s[3], d[3];
     int nf, dec;
     int first;
+    int woff;  // the monitor's columns in a row of the step weight table
 };
+
+// The seed of monitor m at grid cell (gx, gy, gz) from its columns of the
+// step weight table (row: [due, then dt cos, dt sin per frequency], as
+// adj_seed_w_kernel wrote them): the same operations on the same values as
+// adj_seed_at.
+template <typename T>
+__device__ __forceinline__ bool adj_seed_tab(const AdjSeed<T> &m, int gx, int gy, int gz, const double *row,
+                                             double &v) {
+    const int ix = gx - m.s[0], iy = gy - m.s[1], iz = gz - m.s[2];
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= m.d[0] || iy >= m.d[1] || iz >= m.d[2]) return false;
+    const double *w = row + m.woff;
+    if (w[0] == 0.0) return false;
+    const int64_t ncell = (int64_t)m.d[0] * m.d[1] * m.d[2];
+    const int64_t cell = (int64_t)ix + (int64_t)m.d[0] * ((int64_t)iy + (int64_t)m.d[1] * iz);
+    v = 0.0;
+    const double *gp = m.g + 2 * cell;
+    for (int k = 0; k < m.nf; ++k, gp += 2 * ncell) v += gp[0] * w[1 + 2 * k] + gp[1] * w[2 + 2 * k];
+    return true;
+}
 
 // The seed of monitor m at grid cell (gx, gy, gz) of the step at ts, as
 // adjoint_seed_kernel adds it (false: the cell is not on the monitor or the
@@ -2586,7 +2606,45 @@ struct AdjArgs {
     AdjSeed<T> seed[MAX_DFT];
     const int64_t *clk;
     int64_t koff;
+    // this step's row of the step weight table (null: the seeds form their
+    // weights from the device clock themselves)
+    const double *sw;
 };
+
+// Step weight table of the objective's seeds for len steps at device
+// timestep + koff0 + i (row i, stride doubles): per monitor, 1 where its
+// decimation samples the step (else 0), then dt cos and dt sin of f_k theta
+// for each frequency, as dft_kernel forms its weights.
+template <typename T>
+struct AdjSeedW {
+    int nmon;
+    const double *freqs[MAX_DFT];
+    int nf[MAX_DFT], dec[MAX_DFT], woff[MAX_DFT], elec[MAX_DFT];
+    const int64_t *clk;
+    int64_t koff0;
+    int len, stride;
+    T dt;
+    double *out;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void adj_seed_w_kernel(AdjSeedW<T> p) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)p.len * p.stride) return;
+    const int i = (int)(t / p.stride), col = (int)(t % p.stride);
+    int q = 0;
+    while (q + 1 < p.nmon && col >= p.woff[q + 1]) ++q;
+    const int j = col - p.woff[q];
+    const int64_t ts = *p.clk + p.koff0 + i;
+    if (j == 0) {
+        p.out[t] = (p.dec[q] > 1 && (ts % p.dec[q]) != 0) ? 0.0 : 1.0;
+        return;
+    }
+    const double theta = dft_theta(ts, p.dt, p.elec[q]);
+    double sn, cs;
+    sincos(p.freqs[q][(j - 1) >> 1] * theta, &sn, &cs);
+    p.out[t] = (double)p.dt * ((j & 1) ? cs : sn);
+}
 
 template <typename T, bool ELEC>
 __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
@@ -2652,7 +2710,15 @@ __global__ __launch_bounds__(256) void adjoint_half_kernel(AdjArgs<T> p) {
             }
         }
     }
-    if (p.nseed > 0) {
+    if (p.nseed > 0 && p.sw) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < p.nseed; ++q) {
+                double v;
+                if (p.seed[q].lam == p.lA[a] && adj_seed_tab(p.seed[q], gx, gy, gz, p.sw, v))
+                    la[a] = (T)((double)la[a] + v);
+            }
+    } else if (p.nseed > 0) {
         const int64_t ts = *p.clk + p.koff;
 #pragma unroll
         for (int a = 0; a < 3; ++a)

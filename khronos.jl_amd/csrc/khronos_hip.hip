@@ -2324,7 +2324,8 @@ struct Ctx final : CtxBase {
     void *adj_final = nullptr;
     double *adj_cot = nullptr, *adj_grad = nullptr;
     T *adj_seg = nullptr;
-    size_t adj_final_cap = 0, adj_cot_cap = 0, adj_grad_cap = 0, adj_seg_cap = 0;
+    double *adj_sw = nullptr;  // step weight table of the objective's seeds
+    size_t adj_final_cap = 0, adj_cot_cap = 0, adj_grad_cap = 0, adj_seg_cap = 0, adj_sw_cap = 0;
 
     // The seeds of the step at device timestep + koff; every monitor of the
     // group is launched and skips the steps its decimation does not sample,
@@ -2383,7 +2384,7 @@ struct Ctx final : CtxBase {
     // same launch (false: too many monitors for one launch, seed separately).
     bool launch_adj_half(int e, const int (*glo)[3], const int (*gn)[3], const size_t *goff, const T *seg,
                          const std::vector<const Dft *> *mons = nullptr, const std::vector<size_t> *off = nullptr,
-                         int64_t koff = 0) {
+                         int64_t koff = 0, const double *swrow = nullptr, const std::vector<int> *woff = nullptr) {
         kh::AdjArgs<T> q{};
         if (mons) {
             for (size_t i = 0; i < mons->size(); ++i) {
@@ -2397,9 +2398,11 @@ struct Ctx final : CtxBase {
                 for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
                 x.nf = m.nf;
                 x.dec = m.dec;
+                x.woff = woff ? (*woff)[i] : 0;
             }
             q.clk = clk;
             q.koff = koff;
+            q.sw = swrow;
         }
         for (int a = 0; a < 3; ++a) {
             q.lA[a] = adjA[e][a];
@@ -2504,6 +2507,46 @@ struct Ctx final : CtxBase {
         adj_grad = (double *)regrow(adj_grad, adj_grad_cap, std::max<size_t>(gtot, 1) * sizeof(double));
         HIPCHK(hipMemsetAsync(adj_grad, 0, std::max<size_t>(gtot, 1) * sizeof(double), st));
         adj_seg = (T *)regrow(adj_seg, adj_seg_cap, std::max<size_t>(gtot * (size_t)K, 1) * sizeof(T));
+        // The seeds' weights for the steps of an interval, formed by one
+        // launch before its walk back (while each group's seeds go with its
+        // transposed half step and the table stays within 64 MiB;
+        // KH_SEED_TABLE=0: never): the transposed half steps then read them
+        // instead of forming sines and cosines from the device clock per
+        // lane.
+        std::vector<int> sw_off(mons.size(), 0);
+        int sw_stride = 0;
+        bool sw_on = !mons.empty();
+        {
+            int ne = 0, nh = 0;
+            for (size_t i = 0; i < mons.size(); ++i) {
+                sw_off[i] = sw_stride;
+                sw_stride += 1 + 2 * mons[i]->nf;
+                ++(mons[i]->comp < 3 ? ne : nh);
+            }
+            sw_on = sw_on && ne <= kh::MAX_DFT && nh <= kh::MAX_DFT && (int)mons.size() <= kh::MAX_DFT &&
+                    (double)K * sw_stride * sizeof(double) <= 64.0 * (1 << 20) && env_int("KH_SEED_TABLE", 1) != 0;
+        }
+        if (sw_on) adj_sw = (double *)regrow(adj_sw, adj_sw_cap, (size_t)K * sw_stride * sizeof(double));
+        auto launch_seed_w = [&](int64_t len) {
+            kh::AdjSeedW<T> w{};
+            w.nmon = (int)mons.size();
+            for (size_t i = 0; i < mons.size(); ++i) {
+                w.freqs[i] = mons[i]->freqs;
+                w.nf[i] = mons[i]->nf;
+                w.dec[i] = mons[i]->dec;
+                w.woff[i] = sw_off[i];
+                w.elec[i] = mons[i]->comp < 3 ? 1 : 0;
+            }
+            w.clk = clk;
+            w.koff0 = -len;
+            w.len = (int)len;
+            w.stride = sw_stride;
+            w.dt = dt;
+            w.out = adj_sw;
+            const int64_t n = len * sw_stride;
+            hipLaunchKernelGGL((kh::adj_seed_w_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
+            HIPCHK(hipGetLastError());
+        };
         adj_final = regrow(adj_final, adj_final_cap, state_bytes());
         copy_state(adj_final, nullptr, true);
         const CkHost fin{ts, active, last_active, true};
@@ -2591,14 +2634,17 @@ struct Ctx final : CtxBase {
                 end_pre();
                 launch_tick(len);
             }
+            if (sw_on) launch_seed_w(len);
             for (int64_t i = len - 1; i >= 0; --i) {
                 // the seeds go with the transposed half step (or before it,
                 // as their own launches, past MAX_DFT monitors)
-                if (!launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot, &mons, &off, i - len)) {
+                const double *swrow = sw_on ? adj_sw + (size_t)i * sw_stride : nullptr;
+                if (!launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot, &mons, &off, i - len, swrow,
+                                     &sw_off)) {
                     launch_adj_seed(mons, off, true, i - len);
                     launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
                 }
-                if (!launch_adj_half(0, glo, gn, goff, nullptr, &mons, &off, i - len)) {
+                if (!launch_adj_half(0, glo, gn, goff, nullptr, &mons, &off, i - len, swrow, &sw_off)) {
                     launch_adj_seed(mons, off, false, i - len);
                     launch_adj_half(0, glo, gn, goff, nullptr);
                 }

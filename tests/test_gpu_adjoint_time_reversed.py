@@ -438,3 +438,49 @@ def test_snapshots_written_by_the_update_equal_separate_copies(case, monkeypatch
     assert np.any(g_fused != 0)
     for g in (g_copy, g_copy_eager, g_fused_eager):
         assert np.array_equal(g_fused, g)
+
+
+def test_seed_weight_table_equals_clock_weights(monkeypatch):
+    """The objective's seeds take their weights from a table formed by one
+    launch per interval (KH_SEED_TABLE=0: each lane forms them from the
+    device clock): Hy and Ez objectives sampled every 3rd step, captured and
+    eager intervals, the same gradient bit for bit."""
+    import functools
+    monkeypatch.setattr(A, "DFTMonitor", functools.partial(A.DFTMonitor, decimation=3))
+    opt = slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.5, 0.55))
+    opt.objective_arguments = [
+        A.FourierFieldsObjective(volume=Volume([0.75, 0, 0], [0, 0.6, 0.0]), component=kh.Hy()),
+        A.FourierFieldsObjective(volume=Volume([0.75, 0, 0], [0, 0.6, 0.0]), component=kh.Ez())]
+    opt.objective_functions = [lambda h, e: torch.sum(torch.abs(h) ** 2) * 100.0 + 0.1 * torch.sum(torch.abs(e) ** 2)]
+    _time_reversed(opt, 45)
+    opt.update_design(0.3 + 0.4 * np.random.default_rng(12).random(24))
+    opt.forward_run()
+    eng = opt._fwd_sim.engine
+    _note(f"[adjoint-tr] seed weight table vs clock weights, {eng.timestep()} steps, intervals of 45")
+    g_tab = opt._time_reversed_gradient()
+    monkeypatch.setenv("KH_SEED_TABLE", "0")
+    g_clk = opt._time_reversed_gradient()
+    eng.set_graph(0)
+    g_clk_eager = opt._time_reversed_gradient()
+    monkeypatch.delenv("KH_SEED_TABLE")
+    g_tab_eager = opt._time_reversed_gradient()
+    assert np.any(g_tab != 0)
+    for g in (g_clk, g_clk_eager, g_tab_eager):
+        assert np.array_equal(g_tab, g)
+
+
+def test_more_than_sixteen_objective_monitors():
+    """Eighteen Ez objective lines: past sixteen monitors of a group the
+    seeds go to their own launches before the transposed half step.  Equal
+    to central differences."""
+    opt = slab2d_problem(engine=None, precision="f64", mode="fast", freqs=(0.45, 0.55))
+    ys = np.linspace(-0.3, 0.3, 18)
+    opt.objective_arguments = [A.FourierFieldsObjective(volume=Volume([0.75, float(y), 0], [0, 0.0, 0.0]),
+                                                        component=kh.Ez()) for y in ys]
+    opt.objective_functions = [lambda *e: sum((1.0 + 0.1 * i) * torch.sum(torch.abs(x) ** 2) for i, x in enumerate(e))]
+    _time_reversed(opt, 45)
+    rho = 0.3 + 0.4 * np.random.default_rng(13).random(24)
+    res, g = _directional(opt, rho)
+    _note(f"[adjoint-tr] 18 Ez objective monitors (adjoint, fd, rel): {res}")
+    assert len(opt._fwd_spec.dfts) == 18 and np.any(g != 0)
+    assert max(r[2] for r in res) < 1e-6, res
