@@ -44,6 +44,7 @@ constexpr int TX = 64;       // lanes per inner-tile row
 constexpr int TY = 4;        // rows per inner tile
 constexpr int MAX_SRC = 8;
 constexpr int MAX_DFT = 16;   // DFT monitors per half step (one launch)
+constexpr int MAX_DFTE = 4;   // small E monitors accumulated by the E update itself
 constexpr int MAX_SLAB = 6;
 constexpr int NXCD = 8;
 constexpr int CBLK = 256;    // threads per cascade block
@@ -643,6 +644,17 @@ struct PmlPart {
     double wa, wb;
 };
 
+// A small E monitor whose DFT the E update accumulates (real fields, float64
+// accumulators): its columns in a row of the step weight table are [due,
+// then dt cos and dt sin of f_k theta per frequency].
+struct DfteMon {
+    double *acc;
+    int comp;
+    int s[3], d[3];
+    int nf, woff;
+    int64_t ncell;
+};
+
 template <typename T>
 struct PmlArgs {
     PmlPart<T> f[2];  // the part updated; [1]: the imaginary part (TWO kernels)
@@ -693,6 +705,12 @@ struct PmlArgs {
     // fastest, as box_copy_kernel would copy it after the step (null: none)
     T *snap[3];
     int snap_lo[3][3], snap_n[3][3];
+    // DFTE kernels: the DFT update of these E monitors at this step, with
+    // weights from dftw (this step's row of the step weight table), as
+    // dft_kernel would apply it after the half step
+    int ndfte;
+    DfteMon dfte[MAX_DFTE];
+    const double *dftw;
 };
 
 // the ghost value wrap_kernel would write from interior element i
@@ -825,7 +843,8 @@ template <typename T>
 __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T> &f, int a, size_t idx,
                                            const PmlComp<T> &c, T K0, T K1, T so0, T so1, T sn0, T sn1, T sp0,
                                            T sp1, bool v0, bool v1, T dw0 = T(0), T dw1 = T(0), bool h0 = false,
-                                           bool h1 = false, typename V2<T>::t *kept = nullptr) {
+                                           bool h1 = false, typename V2<T>::t *kept = nullptr,
+                                           typename V2<T>::t *stored = nullptr) {
     using V = typename V2<T>::t;
     T in0 = K0, in1 = K1;
     if (c.nc) {
@@ -898,6 +917,34 @@ __device__ __forceinline__ void pml_update(const PmlArgs<T> &p, const PmlPart<T>
     if (kept) {
         kept->x = so0 != T(0) ? wn.x : an.x;
         kept->y = so1 != T(0) ? wn.y : an.y;
+    }
+    if (stored) *stored = an;
+}
+
+// The DFT update of the E monitors of component a at the lane's pair, from
+// the values the update stored (DFTE kernels): the same operations on the
+// same values as dft_kernel (acc += w_k * value, w_k = dt exp(i f_k theta)).
+template <typename T>
+__device__ __forceinline__ void pml_dft(const PmlArgs<T> &p, int a, int x, int y, int z, bool v0, bool v1,
+                                        typename V2<T>::t v) {
+    for (int q = 0; q < p.ndfte; ++q) {
+        const DfteMon &m = p.dfte[q];
+        if (m.comp != a || y < m.s[1] || y >= m.s[1] + m.d[1] || z < m.s[2] || z >= m.s[2] + m.d[2]) continue;
+        const double *w = p.dftw + m.woff;
+        for (int e = 0; e < 2; ++e) {
+            const int i = x + e - m.s[0];
+            if (!(e ? v1 : v0) || i < 0 || i >= m.d[0]) continue;
+            if (w[0] == 0.0) continue;
+            const double u = (double)(e ? v.y : v.x);
+            double *acc = m.acc + 2 * ((int64_t)i + (int64_t)m.d[0] * ((int64_t)(y - m.s[1]) +
+                                                                      (int64_t)m.d[1] * (z - m.s[2])));
+            for (int k = 0; k < m.nf; ++k, acc += 2 * m.ncell) {
+                const double re = acc[0] + w[1 + 2 * k] * u;
+                const double im = acc[1] + w[2 + 2 * k] * u;
+                acc[0] = re;
+                acc[1] = im;
+            }
+        }
     }
 }
 
@@ -983,7 +1030,7 @@ __device__ __forceinline__ void pml_h_part(const PmlArgs<T> &p, const PmlPart<T>
 }
 
 // One part of the E update at the lane's pair.
-template <typename T, bool COND, bool SNAP = false>
+template <typename T, bool COND, bool SNAP = false, bool DFTE = false>
 __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T> &f, size_t idx, int x, int y,
                                            int z, T sx0, T sx1, T sgy, T sgz, bool v0, bool v1, PmlComp<T> c[3],
                                            T dw0 = T(0), T dw1 = T(0), bool h0 = false, bool h1 = false) {
@@ -1008,22 +1055,25 @@ __device__ __forceinline__ void pml_e_part(const PmlArgs<T> &p, const PmlPart<T>
     const T Kz0 = dt * (iy * (hx_ym.x - hx.x) - ix * (hy_xm - hy.x));
     const T Kz1 = dt * (iy * (hx_ym.y - hx.y) - ix * (hy.x - hy.y));
     const int pc = p.pre_comp;
-    typename V2<T>::t kept[3];
+    typename V2<T>::t kept[3], stored[3];
     pml_update(p, f, 0, idx, c[0], Kx0, Kx1, sx0, sx1, sgy, sgy, sgz, sgz, v0, v1, dw0, dw1, h0 && pc == 0,
-               h1 && pc == 0, SNAP ? &kept[0] : nullptr);
+               h1 && pc == 0, SNAP ? &kept[0] : nullptr, DFTE ? &stored[0] : nullptr);
     pml_update(p, f, 1, idx, c[1], Ky0, Ky1, sgy, sgy, sgz, sgz, sx0, sx1, v0, v1, dw0, dw1, h0 && pc == 1,
-               h1 && pc == 1, SNAP ? &kept[1] : nullptr);
+               h1 && pc == 1, SNAP ? &kept[1] : nullptr, DFTE ? &stored[1] : nullptr);
     pml_update(p, f, 2, idx, c[2], Kz0, Kz1, sgz, sgz, sx0, sx1, sgy, sgy, v0, v1, dw0, dw1, h0 && pc == 2,
-               h1 && pc == 2, SNAP ? &kept[2] : nullptr);
+               h1 && pc == 2, SNAP ? &kept[2] : nullptr, DFTE ? &stored[2] : nullptr);
     if (SNAP)
         for (int a = 0; a < 3; ++a) pml_snap(p, a, x, y, z, v0, v1, kept[a]);
+    if (DFTE)
+        for (int a = 0; a < 3; ++a) pml_dft(p, a, x, y, z, v0, v1, stored[a]);
 }
 
 // TWO: complex fields, p.f[0] the real and p.f[1] the imaginary part (no
 // conductivity: its C stage is per part).  PRE: the launch adds precomputed
 // source corrections (real fields without conductivity; a separate variant so
 // that the others keep their register count).  SNAP (E update, real fields
-// without conductivity): the launch also writes the design-box snapshot.
+// without conductivity): the launch also writes the design-box snapshot;
+// DFTE (likewise): it also applies the DFT update of small E monitors.
 template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false>
 __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
@@ -1046,11 +1096,11 @@ __global__ __launch_bounds__(CBLK) void pml_h_kernel(PmlArgs<T> p) {
 }
 
 template <typename T, bool PERVOX, bool COND, bool LUT = false, bool TWO = false, bool PRE = false,
-          bool SNAP = false>
+          bool SNAP = false, bool DFTE = false>
 __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     static_assert(!(COND && TWO), "the conductive C stage runs one part per launch");
     static_assert(!(PRE && (COND || TWO)), "precomputed sources: real fields without conductivity");
-    static_assert(!(SNAP && (COND || TWO)), "snapshots: real fields without conductivity");
+    static_assert(!((SNAP || DFTE) && (COND || TWO)), "snapshots, DFTs: real fields without conductivity");
     int x, y, z;
     bool v0, v1;
     if (!pml_lane(p, x, y, z, v0, v1)) return;
@@ -1063,7 +1113,7 @@ __global__ __launch_bounds__(CBLK) void pml_e_kernel(PmlArgs<T> p) {
     pml_load_m<T, PERVOX, LUT>(p, 0, idx, c[0]);
     pml_load_m<T, PERVOX, LUT>(p, 1, idx, c[1]);
     pml_load_m<T, PERVOX, LUT>(p, 2, idx, c[2]);
-    pml_e_part<T, COND, SNAP>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
+    pml_e_part<T, COND, SNAP, DFTE>(p, p.f[0], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c, dw0, dw1, h0, h1);
     if (TWO) pml_e_part<T, false>(p, p.f[1], idx, x, y, z, sx0, sx1, sgy, sgz, v0, v1, c);
     if (p.nsrc > 0) pml_sources(p, idx, x, y, z, v0, v1);
 }
@@ -2543,7 +2593,7 @@ struct AdjSeed {
 
 // The seed of monitor m at grid cell (gx, gy, gz) from its columns of the
 // step weight table (row: [due, then dt cos, dt sin per frequency], as
-// adj_seed_w_kernel wrote them): the same operations on the same values as
+// dft_weights_kernel wrote them): the same operations on the same values as
 // adj_seed_at.
 template <typename T>
 __device__ __forceinline__ bool adj_seed_tab(const AdjSeed<T> &m, int gx, int gy, int gz, const double *row,
@@ -2611,12 +2661,13 @@ struct AdjArgs {
     const double *sw;
 };
 
-// Step weight table of the objective's seeds for len steps at device
-// timestep + koff0 + i (row i, stride doubles): per monitor, 1 where its
-// decimation samples the step (else 0), then dt cos and dt sin of f_k theta
-// for each frequency, as dft_kernel forms its weights.
+// Step weight table of DFT monitors (the E update's small monitors, the
+// adjoint's seeds) for len steps at device timestep + koff0 + i (row i,
+// stride doubles): per monitor, 1 where its decimation samples the step
+// (else 0), then dt cos and dt sin of f_k theta for each frequency, as
+// dft_kernel forms its weights.
 template <typename T>
-struct AdjSeedW {
+struct DftW {
     int nmon;
     const double *freqs[MAX_DFT];
     int nf[MAX_DFT], dec[MAX_DFT], woff[MAX_DFT], elec[MAX_DFT];
@@ -2628,7 +2679,7 @@ struct AdjSeedW {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void adj_seed_w_kernel(AdjSeedW<T> p) {
+__global__ __launch_bounds__(256) void dft_weights_kernel(DftW<T> p) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)p.len * p.stride) return;
     const int i = (int)(t / p.stride), col = (int)(t % p.stride);

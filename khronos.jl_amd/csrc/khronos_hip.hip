@@ -1079,6 +1079,7 @@ struct Ctx final : CtxBase {
     bool pre_ready[2] = {false, false};  // the current block's corrections are computed
 
     void plan_pre() {
+        plan_dfte();
         if (pre_version == src_version) return;
         pre_version = src_version;
         for (int e = 0; e < 2; ++e) {
@@ -1131,6 +1132,7 @@ struct Ctx final : CtxBase {
     // plan_pre() has run since the sources last changed (it allocates, so
     // never inside a capture)
     void launch_pre(int len) {
+        launch_dfte_weights(len);
         for (int e = 0; e < 2; ++e) {
             pre_ready[e] = false;
             const PrePlan &P = pre[e];
@@ -1163,7 +1165,96 @@ struct Ctx final : CtxBase {
             pre_ready[e] = true;
         }
     }
-    void end_pre() { pre_ready[0] = pre_ready[1] = false; }
+    void end_pre() {
+        pre_ready[0] = pre_ready[1] = false;
+        dfte_ready = false;
+    }
+
+    // Small E monitors whose DFT update the E launch applies as it stores
+    // (FAST mode, whole-grid PML launch, real fields, float64 accumulators,
+    // no material pass or periodic axis, at most MAX_DFTE monitors of at most
+    // 8192 cell-frequencies inside cells 1..N, no E source writing a monitor
+    // cell of its component; KH_DFT_LAUNCH=1: never): their weights for a
+    // block of steps come from one dft_weights_kernel launch at the block's
+    // start, and dft_kernel runs only for the other monitors.
+    std::vector<int> dfte;  // indices into dfts
+    std::vector<int> dfte_off;
+    int dfte_stride = 0, dfte_cap = 0;
+    double *dfte_buf = nullptr;
+    size_t dfte_bytes = 0;
+    bool dfte_ready = false;
+
+    void plan_dfte() {
+        std::vector<int> sel, off;
+        int stride = 0;
+        bool ok = mode == KH_MODE_FAST && nparts == 1 && acc64 && !mat && plan[1].pblocks > 0 && !plan[1].inner &&
+                  env_int("KH_DFT_LAUNCH", 0) == 0;
+        for (int a = 0; a < 3; ++a) ok = ok && !periodic[a] && !msig[a];
+        for (size_t i = 0; ok && i < dfts.size() && (int)sel.size() < kh::MAX_DFTE; ++i) {
+            const Dft &m = dfts[i];
+            if (m.comp >= 3 || m.ncell * m.nf > 8192) continue;
+            bool inside = true;
+            for (int k = 0; k < 3; ++k) inside = inside && m.d[k] > 0 && m.s[k] >= 1 && m.s[k] + m.d[k] - 1 <= g.n[k];
+            if (!inside) continue;
+            bool hit = false;
+            for (auto &o : srcs) {
+                if (o.comp != m.comp) continue;
+                bool meet = true;
+                for (int k = 0; k < 3 && meet; ++k) meet = o.s[k] < m.s[k] + m.d[k] && m.s[k] < o.s[k] + o.d[k];
+                hit = hit || meet;
+            }
+            if (hit) continue;
+            sel.push_back((int)i);
+            off.push_back(stride);
+            stride += 1 + 2 * m.nf;
+        }
+        int cap = 0;
+        if (!sel.empty()) {
+            int want = std::max(std::max(graph_len, graph_steps), 1);
+            if (ckpt_every > 0 && ckpt_every <= 1024) want = std::max(want, ckpt_every);
+            const int64_t fit = ((int64_t)64 << 20) / ((int64_t)stride * (int64_t)sizeof(double));
+            cap = (int)std::min<int64_t>(want, fit);
+            if (cap < 1) sel.clear();
+        }
+        if (sel != dfte || off != dfte_off || stride != dfte_stride) drop_graph();  // launch arguments change
+        dfte = sel;
+        dfte_off = off;
+        dfte_stride = stride;
+        dfte_cap = sel.empty() ? 0 : cap;
+        const size_t bytes = (size_t)dfte_cap * stride * sizeof(double);
+        if (bytes > dfte_bytes) {
+            drop_graph();
+            dfte_buf = (double *)dalloc(bytes);
+            dfte_bytes = bytes;
+        }
+    }
+
+    // weights of the E update's monitors for the next len steps (row k:
+    // device clock + k)
+    void launch_dfte_weights(int len) {
+        dfte_ready = false;
+        if (dfte.empty() || len > dfte_cap) return;
+        kh::DftW<T> w{};
+        w.nmon = (int)dfte.size();
+        for (size_t i = 0; i < dfte.size(); ++i) {
+            const Dft &m = dfts[dfte[i]];
+            w.freqs[i] = m.freqs;
+            w.nf[i] = m.nf;
+            w.dec[i] = m.dec;
+            w.woff[i] = dfte_off[i];
+            w.elec[i] = 1;
+        }
+        w.clk = clk;
+        w.koff0 = 0;
+        w.len = len;
+        w.stride = dfte_stride;
+        w.dt = dt;
+        w.out = dfte_buf;
+        const int64_t n = (int64_t)len * dfte_stride;
+        hipLaunchKernelGGL((kh::dft_weights_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
+        HIPCHK(hipGetLastError());
+        dfte_ready = true;
+    }
 
     // design-box snapshot written by the E update of the steps launched
     // while set (the adjoint's re-run; null: none)
@@ -1450,6 +1541,21 @@ struct Ctx final : CtxBase {
                 }
             }
             const bool snap = electric && part == 0 && !fuse && !cond && (snap_out[0] || snap_out[1] || snap_out[2]);
+            const bool dfte_on = electric && part == 0 && !fuse && !cond && !snap && dfte_ready;
+            if (dfte_on) {
+                q.ndfte = (int)dfte.size();
+                for (size_t i = 0; i < dfte.size(); ++i) {
+                    const Dft &m = dfts[dfte[i]];
+                    kh::DfteMon &x = q.dfte[i];
+                    x.acc = (double *)m.acc;
+                    x.comp = m.comp;
+                    for (int k = 0; k < 3; ++k) { x.s[k] = m.s[k]; x.d[k] = m.d[k]; }
+                    x.nf = m.nf;
+                    x.woff = dfte_off[i];
+                    x.ncell = m.ncell;
+                }
+                q.dftw = dfte_buf + (size_t)koff * dfte_stride;
+            }
             if (snap)
                 for (int a = 0; a < 3; ++a) {
                     q.snap[a] = snap_out[a];
@@ -1471,7 +1577,14 @@ struct Ctx final : CtxBase {
             // template flags: per-voxel material, conductivity C stage,
             // material bytes + table, both parts of a complex field
             void (*kfn)(kh::PmlArgs<T>);
-            if (snap) {
+            if (dfte_on) {
+                if (q.pre_n > 0) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, true, false, true>
+                                     : pervox ? kh::pml_e_kernel<T, true, false, false, false, true, false, true>
+                                              : kh::pml_e_kernel<T, false, false, false, false, true, false, true>;
+                else kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, false, false, true>
+                         : pervox ? kh::pml_e_kernel<T, true, false, false, false, false, false, true>
+                                  : kh::pml_e_kernel<T, false, false, false, false, false, false, true>;
+            } else if (snap) {
                 if (q.pre_n > 0) kfn = lut_on ? kh::pml_e_kernel<T, true, false, true, false, true, true>
                                      : pervox ? kh::pml_e_kernel<T, true, false, false, false, true, true>
                                               : kh::pml_e_kernel<T, false, false, false, false, true, true>;
@@ -1513,9 +1626,13 @@ struct Ctx final : CtxBase {
         // monitors are launched; the kernel gates itself as well
         const int64_t step_ts = ts + koff;
         std::vector<const Dft *> list;
-        for (auto &m : dfts)
+        for (size_t i = 0; i < dfts.size(); ++i) {
+            const Dft &m = dfts[i];
+            if (electric && dfte_ready && std::find(dfte.begin(), dfte.end(), (int)i) != dfte.end())
+                continue;  // applied by the E launch
             if ((m.comp % 6 < 3) == electric && (dft_gate_device || m.dec <= 1 || step_ts % m.dec == 0))
                 list.push_back(&m);
+        }
         for (size_t i0 = 0; i0 < list.size(); i0 += kh::MAX_DFT) {
             kh::DftArgs<T> q{};
             int blocks = 0, nfmax = 1;
@@ -2528,7 +2645,7 @@ struct Ctx final : CtxBase {
         }
         if (sw_on) adj_sw = (double *)regrow(adj_sw, adj_sw_cap, (size_t)K * sw_stride * sizeof(double));
         auto launch_seed_w = [&](int64_t len) {
-            kh::AdjSeedW<T> w{};
+            kh::DftW<T> w{};
             w.nmon = (int)mons.size();
             for (size_t i = 0; i < mons.size(); ++i) {
                 w.freqs[i] = mons[i]->freqs;
@@ -2544,7 +2661,7 @@ struct Ctx final : CtxBase {
             w.dt = dt;
             w.out = adj_sw;
             const int64_t n = len * sw_stride;
-            hipLaunchKernelGGL((kh::adj_seed_w_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
+            hipLaunchKernelGGL((kh::dft_weights_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
             HIPCHK(hipGetLastError());
         };
         adj_final = regrow(adj_final, adj_final_cap, state_bytes());
