@@ -146,7 +146,12 @@ int kh_set_waveform(kh_ctx *ctx, int32_t source_id, int64_t n_steps, const doubl
  * arrays in KH_MODE_EXACT; in KH_MODE_FAST, which keeps no D/B, the value is
  * rebuilt from the carried stage, (W where the own-axis PML sigma is
  * non-zero, else E or H) / eps^-1 (mu^-1) -- exact outside dispersive
- * materials and source cells.  Replaces: init_monitors(DFTMonitor) and
+ * materials and source cells.  In KH_MODE_FAST the E update applies the
+ * update of up to four small E monitors itself (real fields, float64
+ * accumulators, no dispersive/chi3 media or periodic axis, at most 8192
+ * cell-frequencies inside cells 1..N, no E source on a monitor cell of its
+ * component; KH_DFT_LAUNCH=1: never); the result is the same bit for bit.
+ * Replaces: init_monitors(DFTMonitor) and
  * update_dft_monitor! (src/Monitors/Monitors.jl:202-290, :361-379); the
  * design-region D monitors of install_design_region_monitors!
  * (src/Adjoint/DesignRegion.jl:290-312). */
@@ -220,7 +225,8 @@ int kh_prepare_graphs(kh_ctx *ctx, int64_t n_steps);
 
 /* What kh_step ran since the last call (then cleared): counts[0] graph
  * replays, [1] steps inside those replays, [2] steps launched eagerly,
- * [3] DFT kernel launches (inside replays and eager).  For measurement;
+ * [3] DFT kernel launches (inside replays and eager; monitors the E
+ * update applies itself are not counted).  For measurement;
  * no counterpart in the reference. */
 int kh_get_launch_counts(kh_ctx *ctx, int64_t counts[4]);
 
