@@ -557,7 +557,8 @@ def metasurface_line(steps: int, warmup: int, graph_steps: int = 16) -> dict:
     per_step = {k: round(v[0] / k_steps, 5) for k, v in tm.items()}
     e_ms = tm["e_full"][0] / k_steps
     achieved = ab["E"] / (e_ms * 1e-3) / 1e9 if e_ms > 0 else 0.0
-    traffic = measured_traffic("kh::pml_e_kernel<float, true, false, true, true>",
+    # (the summary may name the kernel with or without its trailing flags)
+    traffic = measured_traffic("kh::pml_e_kernel<float, true, false, true, true",
                                "pmc_traffic_metasurface_256x256x384_fp32.md")
     b_alg = (ab["H"] + ab["E"]) / cells
     value = cells * steps / wall / 1e6
