@@ -22,6 +22,7 @@
 #include <vector>
 #include <mutex>
 #include <deque>
+#include <functional>
 #include <map>
 
 #include "../../include/khronos_hip.h"
@@ -555,6 +556,13 @@ struct Ctx final : CtxBase {
         for (auto &b : owned) pool_release(dev, b.p, b.bytes);
         for (void *p : ckpt) (void)hipFree(p);
         if (st) pool_stream_release(dev, st);
+        if (st_adj) {
+            (void)hipStreamSynchronize(st_adj);
+            pool_stream_release(dev, st_adj);
+        }
+        for (auto &r : adj_ev)
+            for (auto &e : r)
+                if (e) (void)hipEventDestroy(e);
     }
 
     void add_source(int comp, const int32_t *s, const int32_t *d, const void *amp, int32_t *id) override {
@@ -2442,6 +2450,12 @@ struct Ctx final : CtxBase {
     double *adj_cot = nullptr, *adj_grad = nullptr;
     T *adj_seg = nullptr;
     double *adj_sw = nullptr;  // step weight table of the objective's seeds
+    // the sweep's second stream (walk back of one interval while the next
+    // interval is re-run), its events, and the stream the transposed half
+    // steps are launched on (null: st)
+    hipStream_t st_adj = nullptr;
+    hipEvent_t adj_ev[2][2] = {};  // [re-run done / walk back done][buffer]
+    hipStream_t ast = nullptr;
     size_t adj_final_cap = 0, adj_cot_cap = 0, adj_grad_cap = 0, adj_seg_cap = 0, adj_sw_cap = 0;
 
     // The seeds of the step at device timestep + koff; every monitor of the
@@ -2491,7 +2505,7 @@ struct Ctx final : CtxBase {
             i0 = i;
             if (blocks == 0) continue;
             hipLaunchKernelGGL((kh::adjoint_seed_kernel<T>), dim3(blocks), dim3(256), (size_t)nfmax * 2 * sizeof(double),
-                               st, q);
+                               ast ? ast : st, q);
             HIPCHK(hipGetLastError());
         }
     }
@@ -2545,8 +2559,9 @@ struct Ctx final : CtxBase {
         q.sz = g.sz();
         const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
         const unsigned blocks = (unsigned)((ncell + 255) / 256);
-        if (e) hipLaunchKernelGGL((kh::adjoint_half_kernel<T, true>), dim3(blocks), dim3(256), 0, st, q);
-        else hipLaunchKernelGGL((kh::adjoint_half_kernel<T, false>), dim3(blocks), dim3(256), 0, st, q);
+        hipStream_t s = ast ? ast : st;
+        if (e) hipLaunchKernelGGL((kh::adjoint_half_kernel<T, true>), dim3(blocks), dim3(256), 0, s, q);
+        else hipLaunchKernelGGL((kh::adjoint_half_kernel<T, false>), dim3(blocks), dim3(256), 0, s, q);
         HIPCHK(hipGetLastError());
         return true;
     }
@@ -2623,7 +2638,6 @@ struct Ctx final : CtxBase {
         if (cot_len) HIPCHK(hipMemcpyAsync(adj_cot, cot, cot_len * sizeof(double), hipMemcpyHostToDevice, st));
         adj_grad = (double *)regrow(adj_grad, adj_grad_cap, std::max<size_t>(gtot, 1) * sizeof(double));
         HIPCHK(hipMemsetAsync(adj_grad, 0, std::max<size_t>(gtot, 1) * sizeof(double), st));
-        adj_seg = (T *)regrow(adj_seg, adj_seg_cap, std::max<size_t>(gtot * (size_t)K, 1) * sizeof(T));
         // The seeds' weights for the steps of an interval, formed by one
         // launch before its walk back (while each group's seeds go with its
         // transposed half step and the table stays within 64 MiB;
@@ -2643,8 +2657,24 @@ struct Ctx final : CtxBase {
             sw_on = sw_on && ne <= kh::MAX_DFT && nh <= kh::MAX_DFT && (int)mons.size() <= kh::MAX_DFT &&
                     (double)K * sw_stride * sizeof(double) <= 64.0 * (1 << 20) && env_int("KH_SEED_TABLE", 1) != 0;
         }
-        if (sw_on) adj_sw = (double *)regrow(adj_sw, adj_sw_cap, (size_t)K * sw_stride * sizeof(double));
-        auto launch_seed_w = [&](int64_t len) {
+        // Two streams: the walk back of an interval (transposed half steps on
+        // st_adj) runs while the next earlier interval is re-run (st), with
+        // the snapshots and seed weights in two buffers by interval parity.
+        // The walk back reads no device clock (its seeds take the weight
+        // table), no forward field and nothing the re-run writes but its own
+        // buffer pair; KH_SWEEP_STREAMS=0: one stream.
+        const bool pipe = env_int("KH_SWEEP_STREAMS", 1) != 0 && (mons.empty() || sw_on);
+        const size_t nbuf = pipe ? 2 : 1;
+        adj_seg = (T *)regrow(adj_seg, adj_seg_cap, std::max<size_t>(nbuf * gtot * (size_t)K, 1) * sizeof(T));
+        if (sw_on)
+            adj_sw = (double *)regrow(adj_sw, adj_sw_cap, nbuf * (size_t)K * sw_stride * sizeof(double));
+        if (pipe) {
+            if (!st_adj) st_adj = pool_stream(dev);
+            for (auto &r : adj_ev)
+                for (auto &e : r)
+                    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        auto launch_seed_w = [&](int64_t len, int par) {
             kh::DftW<T> w{};
             w.nmon = (int)mons.size();
             for (size_t i = 0; i < mons.size(); ++i) {
@@ -2659,7 +2689,7 @@ struct Ctx final : CtxBase {
             w.len = (int)len;
             w.stride = sw_stride;
             w.dt = dt;
-            w.out = adj_sw;
+            w.out = adj_sw + (size_t)par * (size_t)K * sw_stride;
             const int64_t n = len * sw_stride;
             hipLaunchKernelGGL((kh::dft_weights_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w);
             HIPCHK(hipGetLastError());
@@ -2723,12 +2753,15 @@ struct Ctx final : CtxBase {
                 snap_lo[a][k] = glo[a][k];
                 snap_n[a][k] = gn[a][k];
             }
-        auto interval = [&](int64_t len, bool captured) {
+        auto seg_of = [&](int par, int64_t i) { return adj_seg + ((size_t)par * (size_t)K + (size_t)i) * gtot; };
+        // the re-run of one interval of len steps from the restored
+        // checkpoint (snapshot i of the design boxes after step i), then the
+        // seed weights of its steps (the device clock is at its end)
+        auto rerun = [&](int64_t len, bool captured, int par) {
             if (captured) launch_pre((int)len);
             for (int64_t i = 0; i < len; ++i) {
                 if (fused_snap)
-                    for (int a = 0; a < 3; ++a)
-                        snap_out[a] = gn[a][0] > 0 ? adj_seg + (size_t)i * gtot + goff[a] : nullptr;
+                    for (int a = 0; a < 3; ++a) snap_out[a] = gn[a][0] > 0 ? seg_of(par, i) + goff[a] : nullptr;
                 if (!captured) {
                     begin_step();
                     launch_pre(1);
@@ -2741,8 +2774,7 @@ struct Ctx final : CtxBase {
                     end_step();
                 }
                 if (gtot && !fused_snap) {
-                    for (int a = 0; a < 3; ++a)
-                        bc.out[a] = gn[a][0] > 0 ? adj_seg + (size_t)i * gtot + goff[a] : nullptr;
+                    for (int a = 0; a < 3; ++a) bc.out[a] = gn[a][0] > 0 ? seg_of(par, i) + goff[a] : nullptr;
                     hipLaunchKernelGGL((kh::box_copy_kernel<T>), dim3(bmax, 3), dim3(256), 0, st, bc);
                     HIPCHK(hipGetLastError());
                 }
@@ -2751,15 +2783,18 @@ struct Ctx final : CtxBase {
                 end_pre();
                 launch_tick(len);
             }
-            if (sw_on) launch_seed_w(len);
+            if (sw_on) launch_seed_w(len, par);
+        };
+        // the walk back from the interval's last step, on ast
+        auto walk = [&](int64_t len, int par) {
+            const double *swb = sw_on ? adj_sw + (size_t)par * (size_t)K * sw_stride : nullptr;
             for (int64_t i = len - 1; i >= 0; --i) {
                 // the seeds go with the transposed half step (or before it,
                 // as their own launches, past MAX_DFT monitors)
-                const double *swrow = sw_on ? adj_sw + (size_t)i * sw_stride : nullptr;
-                if (!launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot, &mons, &off, i - len, swrow,
-                                     &sw_off)) {
+                const double *swrow = sw_on ? swb + (size_t)i * sw_stride : nullptr;
+                if (!launch_adj_half(1, glo, gn, goff, seg_of(par, i), &mons, &off, i - len, swrow, &sw_off)) {
                     launch_adj_seed(mons, off, true, i - len);
-                    launch_adj_half(1, glo, gn, goff, adj_seg + (size_t)i * gtot);
+                    launch_adj_half(1, glo, gn, goff, seg_of(par, i));
                 }
                 if (!launch_adj_half(0, glo, gn, goff, nullptr, &mons, &off, i - len, swrow, &sw_off)) {
                     launch_adj_seed(mons, off, false, i - len);
@@ -2767,64 +2802,106 @@ struct Ctx final : CtxBase {
                 }
             }
         };
-        // graphs of a full interval, with and without the FAST source
-        // correction (captured with it while the sources are or were on at
-        // the interval's start; the kernels gate themselves by the clock).
-        // Destroyed after the stream has drained, however the sweep ends.
+        struct AstOff {
+            Ctx *c;
+            ~AstOff() { c->ast = nullptr; }
+        } ast_off{this};
+        ast = pipe ? st_adj : st;
+        // graphs of a full interval: one stream, re-run and walk back
+        // together, with and without the FAST source correction (captured
+        // with it while the sources are or were on at the interval's start;
+        // the kernels gate themselves by the clock); two streams, the re-run
+        // per source state and buffer, the walk back per buffer.  Destroyed
+        // after both streams have drained, however the sweep ends.
         struct SweepGraphs {
-            hipStream_t st;
-            hipGraph_t g[2]{};
-            hipGraphExec_t x[2]{};
+            hipStream_t st, st2;
+            hipGraph_t g[6]{};
+            hipGraphExec_t x[6]{};
             ~SweepGraphs() {
                 (void)hipStreamSynchronize(st);
-                for (int i = 0; i < 2; ++i) {
+                if (st2) (void)hipStreamSynchronize(st2);
+                for (int i = 0; i < 6; ++i) {
                     if (x[i]) (void)hipGraphExecDestroy(x[i]);
                     if (g[i]) (void)hipGraphDestroy(g[i]);
                 }
             }
-        } sw{st};
+        } sw{st, pipe ? st_adj : nullptr};
         bool use_graph = graph_steps > 0 && !timing && !graph_failed && K <= 1024;
-        auto capture_interval = [&](int which) -> bool {
-            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        // slot: one stream 0/1 (source state); two streams 0..3 re-run
+        // (source state * 2 + buffer), 4/5 walk back (buffer)
+        auto capture = [&](int slot, hipStream_t cs, const std::function<void()> &body) -> bool {
+            HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             bool ok = true;
             try {
-                interval(K, true);
+                body();
             } catch (const KhError &) {
                 ok = false;
             }
             end_pre();
             hipGraph_t gr = nullptr;
-            const hipError_t e = hipStreamEndCapture(st, &gr);
+            const hipError_t e = hipStreamEndCapture(cs, &gr);
             if (!ok || e != hipSuccess || !gr) {
                 if (gr) (void)hipGraphDestroy(gr);
                 (void)hipGetLastError();
                 return false;
             }
-            sw.g[which] = gr;
-            if (hipGraphInstantiate(&sw.x[which], gr, nullptr, nullptr, 0) != hipSuccess) {
+            sw.g[slot] = gr;
+            if (hipGraphInstantiate(&sw.x[slot], gr, nullptr, nullptr, 0) != hipSuccess) {
                 (void)hipGetLastError();
-                sw.x[which] = nullptr;
+                sw.x[slot] = nullptr;
                 return false;
             }
             return true;
         };
+        bool used[2] = {false, false};
         for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
-            const int64_t s0 = sgi * K, s1 = std::min(N, s0 + K);
+            const int64_t s0 = sgi * K, s1 = std::min(N, s0 + K), len = s1 - s0;
+            const int par = pipe ? (int)(sgi & 1) : 0;
             restore_state(ckpt[(size_t)sgi], ck_host[(size_t)sgi]);
-            if (use_graph && s1 - s0 == K) {
-                const int which = (active || last_active) ? 1 : 0;
-                if (!sw.x[which] && !capture_interval(which)) use_graph = false;
+            // the walk back that read this buffer pair has finished
+            if (pipe && used[par]) HIPCHK(hipStreamWaitEvent(st, adj_ev[1][par], 0));
+            const int which = (active || last_active) ? 1 : 0;
+            bool ran = false;
+            if (use_graph && len == K) {
+                const int slot = pipe ? which * 2 + par : which;
+                if (!sw.x[slot] && !capture(slot, st, [&] {
+                        if (pipe) rerun(K, true, par);
+                        else {
+                            rerun(K, true, 0);
+                            walk(K, 0);
+                        }
+                    }))
+                    use_graph = false;
                 if (use_graph) {
-                    HIPCHK(hipGraphLaunch(sw.x[which], st));
+                    HIPCHK(hipGraphLaunch(sw.x[slot], st));
                     for (int64_t j = s0; j < s1; ++j) {
                         begin_step();
                         end_step();
                     }
-                    continue;
+                    ran = true;
                 }
             }
-            interval(s1 - s0, false);
+            if (!ran) {
+                rerun(len, false, par);
+                if (!pipe) walk(len, 0);
+            }
+            if (!pipe) continue;
+            HIPCHK(hipEventRecord(adj_ev[0][par], st));
+            HIPCHK(hipStreamWaitEvent(st_adj, adj_ev[0][par], 0));
+            bool walked = false;
+            if (use_graph && len == K) {
+                const int slot = 4 + par;
+                if (!sw.x[slot] && !capture(slot, st_adj, [&] { walk(K, par); })) use_graph = false;
+                if (use_graph) {
+                    HIPCHK(hipGraphLaunch(sw.x[slot], st_adj));
+                    walked = true;
+                }
+            }
+            if (!walked) walk(len, par);
+            HIPCHK(hipEventRecord(adj_ev[1][par], st_adj));
+            used[par] = true;
         }
+        if (pipe) HIPCHK(hipStreamSynchronize(st_adj));
         if (gtot) HIPCHK(hipMemcpyAsync(grad, adj_grad, gtot * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }

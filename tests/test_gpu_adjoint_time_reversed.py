@@ -484,3 +484,34 @@ def test_more_than_sixteen_objective_monitors():
     _note(f"[adjoint-tr] 18 Ez objective monitors (adjoint, fd, rel): {res}")
     assert len(opt._fwd_spec.dfts) == 18 and np.any(g != 0)
     assert max(r[2] for r in res) < 1e-6, res
+
+
+@pytest.mark.parametrize("case", ["2d_x_pml_f64", "3d_plane_source_f64"])
+def test_two_stream_sweep_equals_one_stream(case, monkeypatch):
+    """The walk back of each interval runs on a second stream while the next
+    earlier interval is re-run (snapshots and seed weights in two buffers by
+    interval parity; KH_SWEEP_STREAMS=0, read per gradient: one stream).
+    The same gradient bit for bit, captured and eager, with a partial last
+    interval."""
+    make, n, every = {
+        "2d_x_pml_f64": (lambda: slab2d_pml_design_problem(mode="fast"), 24, 41),
+        "3d_plane_source_f64": (lambda: plane3d_problem(), 20, 70),
+    }[case]
+    opt = _time_reversed(make(), every)
+    rho = 0.3 + 0.4 * np.random.default_rng(29).random(n)
+    opt.update_design(rho)
+    opt.forward_run()
+    eng = opt._fwd_sim.engine
+    _note(f"[adjoint-tr] two-stream vs one-stream sweep, {case}, {eng.timestep()} steps, intervals of {every}")
+    assert eng.timestep() % every != 0 and eng.timestep() // every >= 3
+    g_two = opt._time_reversed_gradient()
+    g_two_again = opt._time_reversed_gradient()
+    monkeypatch.setenv("KH_SWEEP_STREAMS", "0")
+    g_one = opt._time_reversed_gradient()
+    eng.set_graph(0)
+    g_one_eager = opt._time_reversed_gradient()
+    monkeypatch.delenv("KH_SWEEP_STREAMS")
+    g_two_eager = opt._time_reversed_gradient()
+    assert np.any(g_two != 0)
+    for g in (g_two_again, g_one, g_one_eager, g_two_eager):
+        assert np.array_equal(g_two, g)
