@@ -431,6 +431,9 @@ int kh_set_checkpoints(kh_ctx *ctx, int32_t every);
  * call; the result is the same bit for bit.  In FAST mode the re-run's E
  * update writes the design-box snapshots itself unless an E source writes a
  * box cell of its component (KH_SNAP_LAUNCH=1: always a separate copy).
+ * The walk back of each interval runs on a second stream of the context
+ * while the next earlier interval is re-run (KH_SWEEP_STREAMS=0: one
+ * stream); the call returns after both have finished.
  * Replaces: the pullback of simulate (src/Adjoint/ChainRulesIntegration.jl:
  * 38-60) with calculate_gradient! (src/Adjoint/GradientKernel.jl:35-109),
  * as dJ/d(eps^-1) per voxel before the design-grid contraction. */
