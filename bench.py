@@ -222,6 +222,32 @@ def concurrency_bench(args):
     return 0
 
 
+def stream_ceiling():
+    """kh_stream_rate for the updates' read/write mixes (FP32 arrays of 48 Mi
+    values, 192 MiB each; 20 timed launches per mix after two untimed)."""
+    import ctypes as C
+    import torch
+    if not torch.cuda.is_available():
+        sys.stderr.write("no GPU visible\n")
+        return 1
+    from khronos_hip import _lib
+    lib = _lib.load()
+    elements, iters = 48 * 1024 * 1024, 20
+    rates = {}
+    for nr, nw in ((6, 3), (3, 3), (2, 1), (1, 1)):
+        out = C.c_double()
+        _lib.check(lib.kh_stream_rate(nr, nw, elements, iters, C.byref(out)))
+        rates[f"{nr}r{nw}w"] = round(out.value, 1)
+        sys.stderr.write(f"{nr} reads, {nw} writes: {out.value:.1f} GB/s\n")
+    print(json.dumps({"metric": "streaming rate of a plain one-pass kernel (GB/s)", "unit": "GB/s",
+                      "peak": HBM_PEAK_GBS, "rates": rates,
+                      "fraction_of_peak": {k: round(v / HBM_PEAK_GBS, 4) for k, v in rates.items()},
+                      "config": {"elements_per_array": elements, "bytes_per_array": elements * 4,
+                                 "timed_launches": iters, "dtype": "f32, two values per lane, nontemporal"}}),
+          flush=True)
+    return 0
+
+
 def serving_warmup(reqs, budget):
     """Untimed: one short request per grid size of the trace, served and then
     run one at a time, so that both timed runs start with the device pool
@@ -616,6 +642,9 @@ def main(argv=None):
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"],
                     help="field precision of the strip waveguide (the headline is f32; f64 for the comparison)")
     ap.add_argument("--graph-steps", type=int, default=16, help="steps per captured hipGraph (0: eager launches)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="streaming rate of a plain one-pass kernel for several read/write mixes (the practical "
+                         "HBM ceiling the update kernels are set against)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (counter calibration)")
     ap.add_argument("--serving", action="store_true", help="serving form instead of the headline benchmark")
@@ -651,6 +680,8 @@ def main(argv=None):
         return concurrency_bench(args)
     if args.metasurface:
         return metasurface_bench(args)
+    if args.stream_ceiling:
+        return stream_ceiling()
     if args.calibrate_copy:
         import torch
         src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")

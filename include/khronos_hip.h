@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define KH_ABI_VERSION 7
+#define KH_ABI_VERSION 8
 
 enum kh_precision { KH_F32 = 0, KH_F64 = 1 };
 
@@ -456,6 +456,16 @@ int kh_get_timing(kh_ctx *ctx, double out[2 * KH_TIMING_SLOTS]);
 /* The hipStream_t the context launches on (for callers that order their own
  * work after it). */
 int kh_get_stream(kh_ctx *ctx, void **stream);
+
+/* Streaming rate of a plain one-pass kernel on the current device, the
+ * practical HBM ceiling for a read/write mix: n_read float32 arrays of
+ * `elements` values read and n_write written (two values per lane,
+ * nontemporal, 256-lane blocks in linear order), `iterations` launches timed
+ * with HIP events after two untimed ones; *gb_per_s = bytes moved / time.
+ * n_read 1..6, n_write 1..3, elements even and >= 512; the arrays are
+ * allocated and freed by the call.  No counterpart in the reference
+ * (measurement only). */
+int kh_stream_rate(int32_t n_read, int32_t n_write, int64_t elements, int32_t iterations, double *gb_per_s);
 
 #ifdef __cplusplus
 }

@@ -2895,4 +2895,30 @@ __global__ __launch_bounds__(256) void box_copy_kernel(BoxCopy<T> p) {
     p.out[a][c] = p.sig[a][g[a]] != T(0) ? p.w[a][fi] : p.f[a][fi];
 }
 
+// ------------------------------------------------------ streaming rate
+// A plain one-pass stream (measurement only): NR float32 arrays read and NW
+// written, two values per lane with nontemporal accesses, blocks in linear
+// order: the practical HBM ceiling for a read/write mix, against which the
+// updates' measured traffic is set.
+struct StreamArgs {
+    const float *in[6];
+    float *out[3];
+    int64_t n2;  // value pairs per array
+};
+
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void stream_kernel(StreamArgs p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n2) return;
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const float2 v = ldo<float, true>(p.in[r], (size_t)(2 * i));
+        acc.x = acc.x + v.x;
+        acc.y = acc.y + v.y;
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sto<float, true>(p.out[w], (size_t)(2 * i), acc);
+}
+
 }  // namespace kh

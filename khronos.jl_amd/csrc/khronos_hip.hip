@@ -3634,6 +3634,63 @@ int kh_adjoint_gradient(kh_ctx *ctx, int32_t n_monitors, const int32_t *monitor_
     });
 }
 
+int kh_stream_rate(int32_t n_read, int32_t n_write, int64_t elements, int32_t iterations, double *gb_per_s) {
+    return guarded([&] {
+        require(n_read >= 1 && n_read <= 6, "n_read must be 1..6");
+        require(n_write >= 1 && n_write <= 3, "n_write must be 1..3");
+        require(elements >= 512 && elements % 2 == 0, "elements must be even and >= 512");
+        require(elements / 2 <= (int64_t)256 * ((1LL << 31) - 1), "elements too large");
+        require(iterations >= 1 && iterations <= 1000, "iterations must be 1..1000");
+        require(gb_per_s != nullptr, "null result");
+        using SF = void (*)(kh::StreamArgs);
+        static const SF table[6][3] = {
+            {kh::stream_kernel<1, 1>, kh::stream_kernel<1, 2>, kh::stream_kernel<1, 3>},
+            {kh::stream_kernel<2, 1>, kh::stream_kernel<2, 2>, kh::stream_kernel<2, 3>},
+            {kh::stream_kernel<3, 1>, kh::stream_kernel<3, 2>, kh::stream_kernel<3, 3>},
+            {kh::stream_kernel<4, 1>, kh::stream_kernel<4, 2>, kh::stream_kernel<4, 3>},
+            {kh::stream_kernel<5, 1>, kh::stream_kernel<5, 2>, kh::stream_kernel<5, 3>},
+            {kh::stream_kernel<6, 1>, kh::stream_kernel<6, 2>, kh::stream_kernel<6, 3>}};
+        const size_t bytes = (size_t)elements * sizeof(float);
+        struct Res {
+            std::vector<void *> bufs;
+            hipStream_t s = nullptr;
+            hipEvent_t a = nullptr, b = nullptr;
+            ~Res() {
+                if (s) (void)hipStreamSynchronize(s);
+                for (void *q : bufs) (void)hipFree(q);
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } r;
+        HIPCHK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&r.a));
+        HIPCHK(hipEventCreate(&r.b));
+        kh::StreamArgs p{};
+        for (int k = 0; k < n_read + n_write; ++k) {
+            void *q = nullptr;
+            HIPCHK(hipMalloc(&q, bytes));
+            r.bufs.push_back(q);
+            HIPCHK(hipMemsetAsync(q, 0, bytes, r.s));
+            if (k < n_read) p.in[k] = (const float *)q;
+            else p.out[k - n_read] = (float *)q;
+        }
+        p.n2 = elements / 2;
+        const unsigned blocks = (unsigned)((p.n2 + 255) / 256);
+        const SF fn = table[n_read - 1][n_write - 1];
+        for (int k = 0; k < 2; ++k) hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, r.s, p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(r.a, r.s));
+        for (int k = 0; k < iterations; ++k) hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, r.s, p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(r.b, r.s));
+        HIPCHK(hipEventSynchronize(r.b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        *gb_per_s = (double)(n_read + n_write) * (double)bytes * iterations / ((double)ms * 1e-3) / 1e9;
+    });
+}
+
 int kh_set_timing(kh_ctx *ctx, int32_t enable) {
     return guarded([&] {
         CTX_OK;

@@ -41,6 +41,7 @@ class KhSimDesc(C.Structure):
 SIGNATURES = {
     "kh_last_error": (C.c_char_p, []),
     "kh_version": (C.c_int, []),
+    "kh_stream_rate": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "kh_mem_footprint": (C.c_int, [C.POINTER(KhSimDesc), C.POINTER(C.c_size_t)]),
     "kh_create": (C.c_int, [C.POINTER(KhSimDesc), C.POINTER(C.c_void_p)]),
     "kh_destroy": (C.c_int, [C.c_void_p]),
@@ -106,7 +107,7 @@ class KhronosError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 7  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
+ABI_VERSION = 8  # KH_ABI_VERSION of include/khronos_hip.h that these signatures describe
 
 
 def load(path: str = LIB_PATH):

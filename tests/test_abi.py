@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_footprint_without_device():
     lib = _lib.load()
-    assert lib.kh_version() == 7
+    assert lib.kh_version() == 8
     assert C.sizeof(_lib.KhSimDesc) == 336  # layout of kh_sim_desc in include/khronos_hip.h
     d = _lib.KhSimDesc()
     d.nx, d.ny, d.nz = 384, 192, 160
@@ -59,4 +59,14 @@ def test_adjoint_entry_points_reject_bad_arguments_without_device():
     g = (C.c_double * 1)()
     assert lib.kh_set_checkpoints(None, 100) != 0
     assert lib.kh_adjoint_gradient(None, 0, None, None, lo, n, g) != 0
+    assert lib.kh_last_error().decode() != ""
+
+
+def test_stream_rate_rejects_bad_arguments_without_device():
+    lib = _lib.load()
+    out = C.c_double()
+    for args in ((0, 1, 1024, 1), (7, 1, 1024, 1), (1, 0, 1024, 1), (1, 4, 1024, 1), (1, 1, 1023, 1),
+                 (1, 1, 256, 1), (1, 1, 1024, 0)):
+        assert lib.kh_stream_rate(*args, C.byref(out)) != 0, args
+    assert lib.kh_stream_rate(1, 1, 1024, 1, None) != 0
     assert lib.kh_last_error().decode() != ""
