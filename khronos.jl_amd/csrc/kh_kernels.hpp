@@ -2895,6 +2895,24 @@ __global__ __launch_bounds__(256) void box_copy_kernel(BoxCopy<T> p) {
     p.out[a][c] = p.sig[a][g[a]] != T(0) ? p.w[a][fi] : p.f[a][fi];
 }
 
+// ------------------------------------------------------- state copies
+// Copy of up to MAX_STATE equally sized arrays in one launch (checkpoints,
+// restores): blockIdx.y selects the array, W the word each lane moves.
+constexpr int MAX_STATE = 64;
+struct StateCopy {
+    const void *src[MAX_STATE];
+    void *dst[MAX_STATE];
+    int64_t words;  // per array
+};
+
+template <typename W>
+__global__ __launch_bounds__(256) void state_copy_kernel(StateCopy a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.words) return;
+    const int k = blockIdx.y;
+    reinterpret_cast<W *>(a.dst[k])[t] = reinterpret_cast<const W *>(a.src[k])[t];
+}
+
 // ------------------------------------------------------ streaming rate
 // A plain one-pass stream (measurement only): NR float32 arrays read and NW
 // written, two values per lane with nontemporal accesses, blocks in linear

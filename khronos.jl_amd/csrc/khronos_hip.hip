@@ -2388,13 +2388,27 @@ struct Ctx final : CtxBase {
         return v;
     }
 
+    // the whole field state to (save) or from a buffer of state_bytes(), in
+    // one copy launch for up to MAX_STATE arrays (16-byte words when the array
+    // size allows, else 4-byte words)
     void copy_state(void *dst, const void *src_state, bool save) {
         const auto arrs = state_arrays();
         const size_t fb = (size_t)g.total * sizeof(T);
-        for (size_t i = 0; i < arrs.size(); ++i) {
-            char *buf = (char *)(save ? dst : const_cast<void *>(src_state)) + i * fb;
-            if (save) HIPCHK(hipMemcpyAsync(buf, arrs[i] - g.front, fb, hipMemcpyDeviceToDevice, st));
-            else HIPCHK(hipMemcpyAsync(arrs[i] - g.front, buf, fb, hipMemcpyDeviceToDevice, st));
+        const bool w16 = fb % 16 == 0;
+        for (size_t i0 = 0; i0 < arrs.size(); i0 += kh::MAX_STATE) {
+            kh::StateCopy c{};
+            const size_t n = std::min(arrs.size() - i0, (size_t)kh::MAX_STATE);
+            for (size_t j = 0; j < n; ++j) {
+                char *buf = (char *)(save ? dst : const_cast<void *>(src_state)) + (i0 + j) * fb;
+                void *arr = arrs[i0 + j] - g.front;
+                c.src[j] = save ? (const void *)arr : (const void *)buf;
+                c.dst[j] = save ? (void *)buf : arr;
+            }
+            c.words = (int64_t)(fb / (w16 ? 16 : 4));
+            const dim3 grid((unsigned)((c.words + 255) / 256), (unsigned)n);
+            if (w16) hipLaunchKernelGGL(kh::state_copy_kernel<uint4>, grid, dim3(256), 0, st, c);
+            else hipLaunchKernelGGL(kh::state_copy_kernel<uint32_t>, grid, dim3(256), 0, st, c);
+            HIPCHK(hipGetLastError());
         }
     }
 
