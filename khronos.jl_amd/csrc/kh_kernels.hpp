@@ -1249,6 +1249,11 @@ __global__ void tick_kernel(int64_t *clk, int64_t n) {
     if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = clk[0] + n;
 }
 
+// Set the device timestep (a restored checkpoint's), in stream order.
+__global__ void set_clock_kernel(int64_t *clk, int64_t ts) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = ts;
+}
+
 // ----------------------------------------------------------- materials
 // After the E update, per cell of the material box (src/Kernels/Kernels.jl:
 // 76-83): [FAST: the terms the D-eliminated update left out], chi3

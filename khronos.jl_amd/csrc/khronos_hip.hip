@@ -2373,7 +2373,6 @@ struct Ctx final : CtxBase {
         bool active = true, last_active = false, valid = false;
     };
     std::vector<CkHost> ck_host;
-    int64_t clk_stage = 0;
 
     std::vector<T *> state_arrays() const {
         std::vector<T *> v;
@@ -2436,9 +2435,9 @@ struct Ctx final : CtxBase {
     void restore_state(const void *src, const CkHost &h) {
         copy_state(nullptr, src, false);
         if (xfold) xghost_stale = true;
-        clk_stage = h.ts;
-        HIPCHK(hipMemcpyAsync(clk, &clk_stage, sizeof(int64_t), hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
+        // the clock is set in stream order, so the host need not wait
+        hipLaunchKernelGGL(kh::set_clock_kernel, dim3(1), dim3(64), 0, st, clk, (int64_t)h.ts);
+        HIPCHK(hipGetLastError());
         ts = h.ts;
         active = h.active;
         last_active = h.last_active;
