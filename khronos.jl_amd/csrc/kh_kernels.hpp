@@ -2902,20 +2902,26 @@ __global__ __launch_bounds__(256) void box_copy_kernel(BoxCopy<T> p) {
 
 // ------------------------------------------------------- state copies
 // Copy of up to MAX_STATE equally sized arrays in one launch (checkpoints,
-// restores): blockIdx.y selects the array, W the word each lane moves.
+// restores): blockIdx.y selects the array; lanes move 16-byte words, then
+// the last few 4-byte words of an array whose size is not a multiple of 16
+// (every array starts 256-byte aligned).
 constexpr int MAX_STATE = 64;
 struct StateCopy {
     const void *src[MAX_STATE];
     void *dst[MAX_STATE];
-    int64_t words;  // per array
+    int64_t w16;  // 16-byte words per array
+    int tail;     // 4-byte words after them
 };
 
-template <typename W>
 __global__ __launch_bounds__(256) void state_copy_kernel(StateCopy a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.words) return;
     const int k = blockIdx.y;
-    reinterpret_cast<W *>(a.dst[k])[t] = reinterpret_cast<const W *>(a.src[k])[t];
+    if (t < a.w16) {
+        reinterpret_cast<uint4 *>(a.dst[k])[t] = reinterpret_cast<const uint4 *>(a.src[k])[t];
+    } else if (t < a.w16 + a.tail) {
+        const int64_t o = 4 * a.w16 + (t - a.w16);
+        reinterpret_cast<uint32_t *>(a.dst[k])[o] = reinterpret_cast<const uint32_t *>(a.src[k])[o];
+    }
 }
 
 // ------------------------------------------------------ streaming rate

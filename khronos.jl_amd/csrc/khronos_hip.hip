@@ -2387,31 +2387,31 @@ struct Ctx final : CtxBase {
         return v;
     }
 
-    // the whole field state to (save) or from a buffer of state_bytes(), in
-    // one copy launch for up to MAX_STATE arrays (16-byte words when the array
-    // size allows, else 4-byte words)
+    // the whole field state to (save) or from a buffer of state_bytes() (one
+    // array every state_stride() bytes), in one copy launch for up to
+    // MAX_STATE arrays
+    size_t state_stride() const { return ((size_t)g.total * sizeof(T) + 255) / 256 * 256; }
     void copy_state(void *dst, const void *src_state, bool save) {
         const auto arrs = state_arrays();
-        const size_t fb = (size_t)g.total * sizeof(T);
-        const bool w16 = fb % 16 == 0;
+        const size_t fb = (size_t)g.total * sizeof(T), stride = state_stride();
         for (size_t i0 = 0; i0 < arrs.size(); i0 += kh::MAX_STATE) {
             kh::StateCopy c{};
             const size_t n = std::min(arrs.size() - i0, (size_t)kh::MAX_STATE);
             for (size_t j = 0; j < n; ++j) {
-                char *buf = (char *)(save ? dst : const_cast<void *>(src_state)) + (i0 + j) * fb;
+                char *buf = (char *)(save ? dst : const_cast<void *>(src_state)) + (i0 + j) * stride;
                 void *arr = arrs[i0 + j] - g.front;
                 c.src[j] = save ? (const void *)arr : (const void *)buf;
                 c.dst[j] = save ? (void *)buf : arr;
             }
-            c.words = (int64_t)(fb / (w16 ? 16 : 4));
-            const dim3 grid((unsigned)((c.words + 255) / 256), (unsigned)n);
-            if (w16) hipLaunchKernelGGL(kh::state_copy_kernel<uint4>, grid, dim3(256), 0, st, c);
-            else hipLaunchKernelGGL(kh::state_copy_kernel<uint32_t>, grid, dim3(256), 0, st, c);
+            c.w16 = (int64_t)(fb / 16);
+            c.tail = (int)((fb % 16) / 4);
+            const dim3 grid((unsigned)((c.w16 + c.tail + 255) / 256), (unsigned)n);
+            hipLaunchKernelGGL(kh::state_copy_kernel, grid, dim3(256), 0, st, c);
             HIPCHK(hipGetLastError());
         }
     }
 
-    size_t state_bytes() const { return state_arrays().size() * (size_t)g.total * sizeof(T); }
+    size_t state_bytes() const { return state_arrays().size() * state_stride(); }
 
     void save_checkpoint() {
         const size_t k = (size_t)(ts / ckpt_every);
