@@ -425,10 +425,11 @@ int kh_set_checkpoints(kh_ctx *ctx, int32_t every);
  * boxes in order, each
  * dense with x fastest.  Real fields without dispersive, chi3 or conductive
  * media and without periodic axes.  The fields, timestep and monitors are
- * those of the forward run again on return.  Unless graphs are off
- * (kh_set_graph 0) or kernel timing is on, each full interval of up to 1024
- * steps (re-run, snapshots, walk back) replays one graph captured for the
- * call; the result is the same bit for bit.  In FAST mode the re-run's E
+ * those of the forward run again on return (the fields restored in the
+ * order of the context's stream).  Unless graphs are off (kh_set_graph 0)
+ * or kernel timing is on, each full interval of up to 1024 steps (re-run,
+ * snapshots, walk back) replays graphs captured for the call; the result is
+ * the same bit for bit.  In FAST mode the re-run's E
  * update writes the design-box snapshots itself unless an E source writes a
  * box cell of its component (KH_SNAP_LAUNCH=1: always a separate copy).
  * The walk back of each interval runs on a second stream of the context
